@@ -1,0 +1,279 @@
+/* p3m_hip.h -- C ABI of the MI355X (gfx950) P3M k-space force library.
+ *
+ * Drop-in for the method table of fweik/p3m-standalone: the six method_t
+ * objects below have the reference's names, ids, flags and call semantics
+ * (types.h:306-316; p3m-ik.c:46-49, p3m-ik-i.c:40-43, p3m-ad.c:39-41,
+ * p3m-ad-i.c:39-41, p3m-ik-real.c:43-46, p3m-ad-real.c:39-41), so a program
+ * written against the reference links against libp3m_hip.so instead of the
+ * reference's p3m-*.o files.  Every stage of Kspace_force runs as a HIP
+ * kernel on the GPU; the host side only moves data and launches.
+ *
+ * All types are byte-identical to the reference's types.h built with
+ * DOUBLE_PREC (FLOAT_TYPE = double).  If the reference's types.h has been
+ * included first (TYPES_H defined), its definitions are used as they are.
+ */
+#ifndef P3M_HIP_H
+#define P3M_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Reference types (types.h:107-316, FLOAT_TYPE = double)                  */
+/* ---------------------------------------------------------------------- */
+#ifndef TYPES_H
+
+enum {
+    METHOD_P3M_ik = 0,
+    METHOD_P3M_ik_i = 1,
+    METHOD_P3M_ad = 2,
+    METHOD_P3M_ad_i = 3,
+    METHOD_EWALD = 4,
+    METHOD_P3M_ik_cuda = 5,
+    METHOD_P3M_ik_r = 6,
+    METHOD_P3M_ad_r = 7
+};
+
+/* types.h:126-132 */
+typedef struct {
+    int size;
+    double *x;
+    double *y;
+    double *z;
+    double **fields;
+} vector_array_t;
+
+/* types.h:152-159 */
+typedef struct {
+    vector_array_t *f;   /* total force */
+    vector_array_t *f_k; /* k-space part (Kspace_force accumulates here) */
+    vector_array_t *f_r; /* real-space part */
+} forces_t;
+
+/* types.h:163-183 */
+typedef struct {
+    double length;
+    int nparticles;
+    vector_array_t *p;
+    double *q;
+    double q2;
+    vector_array_t *v;
+    forces_t *reference;
+    double epsilon;
+    double energy;
+} system_t;
+
+/* types.h:187-197 */
+typedef struct {
+    int n;
+    vector_array_t *p;
+    double *q;
+    int *id;
+} neighbor_list_t;
+
+/* types.h:202-215 */
+typedef struct {
+    double alpha;
+    double rcut;
+    double prefactor;
+    int mesh;
+    int ip;
+    int cao;
+    int cao3;
+    double precision;
+    int tuning;
+} parameters_t;
+
+/* types.h:217-226 */
+typedef struct {
+    int cao;
+    double **interpol;
+    double **interpol_d;
+    double (*U_hat)(int, double);
+} interpolation_t;
+
+/* types.h:243-248 */
+typedef struct {
+    double t;
+    double t_c;
+    double t_g;
+    double t_f;
+} runtime_t;
+
+/* types.h:252-286.  The FFTW plan handles are pointer-sized; this library
+ * never uses them (its transforms are its own HIP kernels). */
+typedef struct {
+    int mesh;
+    double *G_hat;
+    double *Qmesh;
+    vector_array_t *Fmesh;
+    double *nshift;
+    double *Dn;
+    double *dQ[2];
+    int *ca_ind[2];
+    double *cf[2];
+    interpolation_t *inter;
+    int forward_plans;
+    int backward_plans;
+    void *forward_plan[3];
+    void *backward_plan[3];
+    neighbor_list_t *neighbor_list;
+    double *self_force_corrections;
+    void *method_data;
+    runtime_t runtime;
+} data_t;
+
+/* types.h:290-303 */
+enum {
+    METHOD_FLAG_none = 0,
+    METHOD_FLAG_ik = 1,
+    METHOD_FLAG_ad = 2,
+    METHOD_FLAG_interlaced = 4,
+    METHOD_FLAG_nshift = 8,
+    METHOD_FLAG_G_hat = 16,
+    METHOD_FLAG_Qmesh = 32,
+    METHOD_FLAG_ca = 64,
+    METHOD_FLAG_self_force_correction = 128
+};
+#define METHOD_FLAG_P3M (METHOD_FLAG_nshift | METHOD_FLAG_G_hat | METHOD_FLAG_Qmesh | METHOD_FLAG_ca)
+
+/* types.h:306-316 */
+typedef struct {
+    int method_id;
+    const char *method_name;
+    const char *method_name_short;
+    char flags;
+    data_t *(*Init)(system_t *, parameters_t *);
+    void (*Influence_function)(system_t *, parameters_t *, data_t *);
+    void (*Kspace_force)(system_t *, parameters_t *, data_t *, forces_t *);
+    double (*Error)(system_t *, parameters_t *);
+    double (*Error_k)(system_t *, parameters_t *);
+} method_t;
+
+#endif /* TYPES_H */
+
+/* ---------------------------------------------------------------------- */
+/* The method table (drop-in objects)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Each object replaces the reference object of the same name:
+ *   method_p3m_ik    p3m-ik.c:46-49       ik, c2c
+ *   method_p3m_ik_i  p3m-ik-i.c:40-43     ik, c2c, interlaced
+ *   method_p3m_ad    p3m-ad.c:39-41       ad, c2c, self-force correction
+ *   method_p3m_ad_i  p3m-ad-i.c:39-41     ad, c2c, interlaced
+ *   method_p3m_ik_r  p3m-ik-real.c:43-46  ik, r2c
+ *   method_p3m_ad_r  p3m-ad-real.c:39-41  ad, r2c, self-force correction
+ * Semantics kept from the reference:
+ *   Init               allocates the method state and computes the influence
+ *                      function at p->alpha (p3m-common.c:145-232);
+ *   Influence_function recomputes G (and, for ad / ad-r, the self-force
+ *                      coefficients) for the current p->alpha; must follow
+ *                      every change of alpha (main.c:378);
+ *   Kspace_force       f->f_k += k-space force; ad / ad-r also subtract the
+ *                      self force from f->f (p3m-ad-self-forces.c:114-139);
+ *                      with p->tuning set, d->runtime.t_c/t_g/t_f receive the
+ *                      assignment / FFT+multiply / interpolation times;
+ *   Error, Error_k     rms force error estimates (total, k-space only).
+ * The data_t returned by Init must be released with p3m_hip_free(), not the
+ * reference's Free_data(). */
+extern const method_t method_p3m_ik;
+extern const method_t method_p3m_ik_i;
+extern const method_t method_p3m_ad;
+extern const method_t method_p3m_ad_i;
+extern const method_t method_p3m_ik_r;
+extern const method_t method_p3m_ad_r;
+
+/* Look a method up by id (main.c:314-344 selects by id); NULL if unknown. */
+const method_t *p3m_hip_method(int method_id);
+
+/* ---------------------------------------------------------------------- */
+/* Status-returning entry points (ABI additions)                           */
+/* ---------------------------------------------------------------------- */
+
+enum {
+    P3M_HIP_OK = 0,
+    P3M_HIP_EINVAL = 1,      /* bad argument: cao not in 1..7, odd or too small mesh, ... */
+    P3M_HIP_ERANGE = 2,      /* a particle position outside [0, L) or not finite */
+    P3M_HIP_ENOMEM = 3,      /* device or host allocation failed */
+    P3M_HIP_EDEVICE = 4,     /* a HIP runtime call failed */
+    P3M_HIP_ENODEVICE = 5    /* no usable GPU */
+};
+
+/* Arithmetic of the mesh stages. */
+enum {
+    P3M_HIP_FP64 = 0,   /* everything in FP64: reproduces the reference to ~1e-12 */
+    P3M_HIP_MIXED = 1   /* mesh, FFT, interpolation in FP32 with compensated
+                           accumulation; influence function, self force and
+                           reductions in FP64 */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *p3m_hip_last_error(void);
+
+/* Init of method `method_id` with the given precision; *out receives the
+ * data_t.  Replaces method.Init (p3m-common.c:145-232 + Init_*). */
+int p3m_hip_init(int method_id, system_t *s, parameters_t *p, int precision, data_t **out);
+/* Influence_function with a status (p3m-ik.c:120-158, p3m-ik-i.c:111-151,
+ * p3m-ad.c:96-143, p3m-ad-i.c:106-135). */
+int p3m_hip_influence_function(system_t *s, parameters_t *p, data_t *d);
+/* Kspace_force with a status (p3m-ik.c:164-230, p3m-ik-i.c:154-231,
+ * p3m-ad.c:146-195, p3m-ad-i.c:138-188, p3m-ik-real.c:68-132,
+ * p3m-ad-real.c:54-103).  Host arrays in, host arrays accumulated. */
+int p3m_hip_kspace_force(system_t *s, parameters_t *p, data_t *d, forces_t *f);
+/* Frees the data_t and its device state.  Replaces Free_data
+ * (p3m-common.c:234-287) for data_t objects made by this library. */
+int p3m_hip_free(data_t *d);
+/* Change the precision of an initialised data_t. */
+int p3m_hip_set_precision(data_t *d, int precision);
+
+/* Device-resident Kspace_force: the same computation on device arrays
+ * (SoA, length n, FP64) already in GPU memory; fk (3n: x, y, z blocks)
+ * receives += k-space force, f (3n) -= self force for ad / ad-r (may be
+ * NULL for other methods).  Runs on the library's stream; returns once the
+ * work is enqueued.  `n` must equal s->nparticles given to Init. */
+int p3m_hip_kspace_force_device(data_t *d, int n, double length, const double *x,
+                                const double *y, const double *z, const double *q,
+                                double *fk, double *f);
+/* Wait for all work enqueued on the library's stream for d. */
+int p3m_hip_synchronize(data_t *d);
+
+/* Real-space part on the GPU: the pair set and arithmetic of Realteil
+ * (realpart.c:109-152): every i != j with minimum-image distance <= rcut
+ * (rcut <= L/2).  f->f_r += forces, s->energy += energy. */
+int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f);
+/* Device-resident real-space part: fr (3n) += forces; energy (device
+ * double, may be NULL) += energy.  Uses the stream of d. */
+int p3m_hip_realteil_device(data_t *d, int n, double length, const double *x, const double *y,
+                            const double *z, const double *q, double rcut, double alpha,
+                            double *fr, double *energy);
+/* Calculate_forces (common.c:253-312) on the GPU: zero f, f_k, f_r; real
+ * space if p->rcut != 0; Kspace_force; f += f_k + f_r. */
+int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, data_t *d,
+                             forces_t *f);
+
+/* Error estimates with a status (Error / Error_k of the method table):
+ * realpart.c:343-346 with p3m-ik.c:234-286, p3m-ik-i.c:280-305,394-399,
+ * p3m-ad.c:475-505, p3m-ad-i.c:237-279.  The k-space sums run on the GPU. */
+int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, double *kspace);
+/* Supply one row of the reference's tabulated ik / ik-i error kernel
+ * (q_ik.c / q_ik_i.c: 101 values at alpha*L = 0..100) for (mesh, cao);
+ * kind 0 = Q_ik (methods 0, 6), 1 = Q_ik_i (method 1).  With a row present
+ * the estimate interpolates it as find_error.c:84-107 does. */
+int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101);
+
+/* Stage times (ms) of the last Kspace_force: assignment, FFT + multiply,
+ * back-interpolation, total (HIP events). */
+int p3m_hip_stage_times(data_t *d, double out[4]);
+/* Copy the influence function (M^3 doubles, r_ind order) to the host. */
+int p3m_hip_get_influence_function(data_t *d, double *G);
+/* Copy the self-force coefficients (27 x 3 doubles) to the host. */
+int p3m_hip_get_self_force_coefficients(data_t *d, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* P3M_HIP_H */

@@ -1,0 +1,296 @@
+"""Python mirror of the reference's method-table interface over libp3m_hip.so.
+
+The C structs below are byte-identical to fweik/p3m-standalone's types.h
+(vector_array_t :126-132, forces_t :152-159, system_t :163-183,
+parameters_t :202-215, data_t :252-286, method_t :306-316) and the calls go
+through the library's exported method_t objects, exactly as the reference's
+main.c drives a method (main.c:361 Init, :378 Influence_function,
+common.c:305 Kspace_force, main.c:432-433 Error / Error_k).
+
+The library must be built (make -C p3m-standalone_amd); loading fails loudly
+otherwise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libp3m_hip.so")
+
+METHOD_IDS = (0, 1, 2, 3, 6, 7)
+METHOD_NAMES = {0: "p3m-ik", 1: "p3m-ik-i", 2: "p3m-ad", 3: "p3m-ad-i", 6: "p3m-ik-r", 7: "p3m-ad-r"}
+FP64, MIXED = 0, 1
+
+OK, EINVAL, ERANGE, ENOMEM, EDEVICE, ENODEVICE = 0, 1, 2, 3, 4, 5
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class VectorArray(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int), ("x", _dp), ("y", _dp), ("z", _dp),
+                ("fields", ctypes.POINTER(_dp))]
+
+
+class Forces(ctypes.Structure):
+    _fields_ = [("f", ctypes.POINTER(VectorArray)), ("f_k", ctypes.POINTER(VectorArray)),
+                ("f_r", ctypes.POINTER(VectorArray))]
+
+
+class System(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_double), ("nparticles", ctypes.c_int),
+                ("p", ctypes.POINTER(VectorArray)), ("q", _dp), ("q2", ctypes.c_double),
+                ("v", ctypes.POINTER(VectorArray)), ("reference", ctypes.POINTER(Forces)),
+                ("epsilon", ctypes.c_double), ("energy", ctypes.c_double)]
+
+
+class Parameters(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_double), ("rcut", ctypes.c_double),
+                ("prefactor", ctypes.c_double), ("mesh", ctypes.c_int), ("ip", ctypes.c_int),
+                ("cao", ctypes.c_int), ("cao3", ctypes.c_int), ("precision", ctypes.c_double),
+                ("tuning", ctypes.c_int)]
+
+
+class Runtime(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("t_c", ctypes.c_double), ("t_g", ctypes.c_double),
+                ("t_f", ctypes.c_double)]
+
+
+class Data(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_int), ("G_hat", _dp), ("Qmesh", _dp),
+                ("Fmesh", ctypes.POINTER(VectorArray)), ("nshift", _dp), ("Dn", _dp),
+                ("dQ", _dp * 2), ("ca_ind", ctypes.POINTER(ctypes.c_int) * 2), ("cf", _dp * 2),
+                ("inter", ctypes.c_void_p), ("forward_plans", ctypes.c_int),
+                ("backward_plans", ctypes.c_int), ("forward_plan", ctypes.c_void_p * 3),
+                ("backward_plan", ctypes.c_void_p * 3), ("neighbor_list", ctypes.c_void_p),
+                ("self_force_corrections", _dp), ("method_data", ctypes.c_void_p),
+                ("runtime", Runtime)]
+
+
+_SP, _PP, _DPT, _FP = (ctypes.POINTER(System), ctypes.POINTER(Parameters), ctypes.POINTER(Data),
+                       ctypes.POINTER(Forces))
+InitFn = ctypes.CFUNCTYPE(_DPT, _SP, _PP)
+InfluenceFn = ctypes.CFUNCTYPE(None, _SP, _PP, _DPT)
+KspaceFn = ctypes.CFUNCTYPE(None, _SP, _PP, _DPT, _FP)
+ErrorFn = ctypes.CFUNCTYPE(ctypes.c_double, _SP, _PP)
+
+
+class Method(ctypes.Structure):
+    _fields_ = [("method_id", ctypes.c_int), ("method_name", ctypes.c_char_p),
+                ("method_name_short", ctypes.c_char_p), ("flags", ctypes.c_char),
+                ("Init", InitFn), ("Influence_function", InfluenceFn), ("Kspace_force", KspaceFn),
+                ("Error", ErrorFn), ("Error_k", ErrorFn)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libp3m_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libp3m_hip.so is not built; run `make -C %s`" % _HERE)
+        L = ctypes.CDLL(LIB_PATH)
+        i, d, vp, ip = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
+        sigs = {
+            "p3m_hip_last_error": (ctypes.c_char_p, []),
+            "p3m_hip_method": (ctypes.POINTER(Method), [i]),
+            "p3m_hip_init": (i, [i, _SP, _PP, i, ctypes.POINTER(_DPT)]),
+            "p3m_hip_influence_function": (i, [_SP, _PP, _DPT]),
+            "p3m_hip_kspace_force": (i, [_SP, _PP, _DPT, _FP]),
+            "p3m_hip_free": (i, [_DPT]),
+            "p3m_hip_set_precision": (i, [_DPT, i]),
+            "p3m_hip_kspace_force_device": (i, [_DPT, i, d, vp, vp, vp, vp, vp, vp]),
+            "p3m_hip_synchronize": (i, [_DPT]),
+            "p3m_hip_realteil": (i, [_SP, _PP, _FP]),
+            "p3m_hip_realteil_device": (i, [_DPT, i, d, vp, vp, vp, vp, d, d, vp, vp]),
+            "p3m_hip_calculate_forces": (i, [ctypes.POINTER(Method), _SP, _PP, _DPT, _FP]),
+            "p3m_hip_error": (i, [i, _SP, _PP, _dp, _dp]),
+            "p3m_hip_set_error_table": (i, [i, i, i, _dp]),
+            "p3m_hip_stage_times": (i, [_DPT, _dp]),
+            "p3m_hip_get_influence_function": (i, [_DPT, _dp]),
+            "p3m_hip_get_self_force_coefficients": (i, [_DPT, _dp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class P3MError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("p3m_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != OK:
+        raise P3MError(rc, lib().p3m_hip_last_error().decode())
+
+
+def method(method_id):
+    """The library's method_t object for a method id (same as the reference's)."""
+    m = lib().p3m_hip_method(method_id)
+    if not m:
+        raise ValueError("unknown method id %d" % method_id)
+    return m.contents
+
+
+def exported_method(name):
+    """One of the exported method_t objects by symbol name (method_p3m_ik, ...)."""
+    return Method.in_dll(lib(), name)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class _Vec:
+    """A vector_array_t over three numpy arrays (kept alive with it)."""
+
+    def __init__(self, x, y, z):
+        self.arrays = [_c(x), _c(y), _c(z)]
+        self.ptrs = (_dp * 3)(*[a.ctypes.data_as(_dp) for a in self.arrays])
+        self.c = VectorArray(len(self.arrays[0]), self.ptrs[0], self.ptrs[1], self.ptrs[2],
+                             ctypes.cast(self.ptrs, ctypes.POINTER(_dp)))
+
+    def numpy(self):
+        return np.stack(self.arrays)
+
+
+class HostSystem:
+    """system_t over numpy positions/charges (reference layout, FP64)."""
+
+    def __init__(self, x, y, z, q, length):
+        self.pos = _Vec(x, y, z)
+        self.q = _c(q)
+        n = len(self.q)
+        self.c = System(float(length), n, ctypes.pointer(self.pos.c), self.q.ctypes.data_as(_dp),
+                        float(np.sum(self.q ** 2)), None, None, 1.0, 0.0)
+        self.ptr = ctypes.pointer(self.c)
+
+    @classmethod
+    def from_dict(cls, s):
+        return cls(s["x"], s["y"], s["z"], s["q"], s["L"])
+
+    @property
+    def n(self):
+        return self.c.nparticles
+
+
+class HostForces:
+    """forces_t with zeroed f, f_k, f_r (reference layout)."""
+
+    def __init__(self, n):
+        z = np.zeros(n)
+        self.f, self.f_k, self.f_r = _Vec(z, z, z), _Vec(z, z, z), _Vec(z, z, z)
+        self.c = Forces(ctypes.pointer(self.f.c), ctypes.pointer(self.f_k.c), ctypes.pointer(self.f_r.c))
+        self.ptr = ctypes.pointer(self.c)
+
+    def total(self):
+        return self.f.numpy()
+
+    def kspace(self):
+        return self.f_k.numpy()
+
+    def real(self):
+        return self.f_r.numpy()
+
+
+def parameters(mesh, cao, alpha, rcut=0.0, tuning=0):
+    p = Parameters(float(alpha), float(rcut), 1.0, int(mesh), int(cao) - 1, int(cao), int(cao) ** 3,
+                   0.0, int(tuning))
+    return p
+
+
+class Solver:
+    """One initialised method (a data_t) -- thin object wrapper used by tests
+    and the benchmark.  Every call maps to one C-ABI entry point."""
+
+    def __init__(self, method_id, system, params, precision=FP64):
+        self.method_id = method_id
+        self.system = system
+        self.params = params
+        self.pptr = ctypes.pointer(params)
+        self.d = _DPT()
+        check(lib().p3m_hip_init(method_id, system.ptr, self.pptr, precision, ctypes.byref(self.d)))
+
+    def close(self):
+        if self.d:
+            check(lib().p3m_hip_free(self.d))
+            self.d = _DPT()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_alpha(self, alpha):
+        self.params.alpha = float(alpha)
+        check(lib().p3m_hip_influence_function(self.system.ptr, self.pptr, self.d))
+
+    def set_precision(self, precision):
+        check(lib().p3m_hip_set_precision(self.d, precision))
+
+    def kspace_force(self, forces=None):
+        forces = forces or HostForces(self.system.n)
+        check(lib().p3m_hip_kspace_force(self.system.ptr, self.pptr, self.d, forces.ptr))
+        return forces
+
+    def calculate_forces(self, forces=None):
+        forces = forces or HostForces(self.system.n)
+        m = lib().p3m_hip_method(self.method_id)
+        check(lib().p3m_hip_calculate_forces(m, self.system.ptr, self.pptr, self.d, forces.ptr))
+        return forces
+
+    def influence_function(self):
+        M = self.params.mesh
+        G = np.zeros(M ** 3)
+        check(lib().p3m_hip_get_influence_function(self.d, G.ctypes.data_as(_dp)))
+        return G
+
+    def self_force_coefficients(self):
+        b = np.zeros(81)
+        check(lib().p3m_hip_get_self_force_coefficients(self.d, b.ctypes.data_as(_dp)))
+        return b
+
+    def stage_times(self):
+        t = np.zeros(4)
+        check(lib().p3m_hip_stage_times(self.d, t.ctypes.data_as(_dp)))
+        return t
+
+    # device-resident entry points (pointers to device memory, e.g. torch tensors)
+    def kspace_force_device(self, n, length, x, y, z, q, fk, f):
+        check(lib().p3m_hip_kspace_force_device(self.d, n, length, x, y, z, q, fk, f))
+
+    def realteil_device(self, n, length, x, y, z, q, rcut, alpha, fr, energy):
+        check(lib().p3m_hip_realteil_device(self.d, n, length, x, y, z, q, rcut, alpha, fr, energy))
+
+    def synchronize(self):
+        check(lib().p3m_hip_synchronize(self.d))
+
+
+def realteil(system, rcut, alpha, forces=None):
+    """Realteil on the GPU (realpart.c:109-152): returns forces with f_r filled."""
+    forces = forces or HostForces(system.n)
+    p = parameters(2, 1, alpha, rcut)
+    check(lib().p3m_hip_realteil(system.ptr, ctypes.pointer(p), forces.ptr))
+    return forces
+
+
+def error_estimate(method_id, system, params):
+    """(total, k-space) rms error estimates of the method (Error / Error_k)."""
+    t, k = ctypes.c_double(), ctypes.c_double()
+    check(lib().p3m_hip_error(method_id, system.ptr, ctypes.pointer(params), ctypes.byref(t),
+                              ctypes.byref(k)))
+    return t.value, k.value
+
+
+def set_error_table(kind, mesh, cao, table):
+    t = _c(table)
+    assert t.shape == (101,)
+    check(lib().p3m_hip_set_error_table(kind, mesh, cao, t.ctypes.data_as(_dp)))

@@ -1,0 +1,243 @@
+// HIP runtime wrappers of the internal C ABI: streams, events, memory,
+// small utility kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "k_internal.hpp"
+
+static thread_local char g_err[512];
+
+extern "C" const char *p3mk_error(void) { return g_err; }
+
+int p3mk_fail(int code, const char *what, int hip_code) {
+    if (hip_code != 0)
+        std::snprintf(g_err, sizeof(g_err), "%s: %s", what,
+                      hipGetErrorString((hipError_t)hip_code));
+    else
+        std::snprintf(g_err, sizeof(g_err), "%s", what);
+    return code;
+}
+
+int p3mk_launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, what, (int)e);
+    return P3MK_OK;
+}
+
+struct p3mk_queue {
+    hipStream_t stream;
+    hipEvent_t ev[P3MK_NUM_EVENTS];
+};
+
+hipStream_t p3mk_stream(p3mk_queue *q) { return q ? q->stream : (hipStream_t)0; }
+
+extern "C" int p3mk_device_check(void) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1)
+        return p3mk_fail(P3MK_ENODEVICE, "no HIP device available", e == hipSuccess ? 0 : (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_queue_create(p3mk_queue **out) {
+    int rc = p3mk_device_check();
+    if (rc != P3MK_OK)
+        return rc;
+    p3mk_queue *q = new (std::nothrow) p3mk_queue;
+    if (!q)
+        return p3mk_fail(P3MK_ENOMEM, "queue allocation failed", 0);
+    hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete q;
+        return p3mk_fail(P3MK_EDEVICE, "hipStreamCreate", (int)e);
+    }
+    for (int i = 0; i < P3MK_NUM_EVENTS; i++) {
+        e = hipEventCreate(&q->ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; j++)
+                (void)hipEventDestroy(q->ev[j]);
+            (void)hipStreamDestroy(q->stream);
+            delete q;
+            return p3mk_fail(P3MK_EDEVICE, "hipEventCreate", (int)e);
+        }
+    }
+    *out = q;
+    return P3MK_OK;
+}
+
+extern "C" void p3mk_queue_destroy(p3mk_queue *q) {
+    if (!q)
+        return;
+    (void)hipStreamSynchronize(q->stream);
+    for (int i = 0; i < P3MK_NUM_EVENTS; i++)
+        (void)hipEventDestroy(q->ev[i]);
+    (void)hipStreamDestroy(q->stream);
+    delete q;
+}
+
+extern "C" int p3mk_queue_sync(p3mk_queue *q) {
+    hipError_t e = hipStreamSynchronize(q->stream);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipStreamSynchronize", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_record(p3mk_queue *q, int slot) {
+    if (slot < 0 || slot >= P3MK_NUM_EVENTS)
+        return p3mk_fail(P3MK_EINVAL, "event slot out of range", 0);
+    hipError_t e = hipEventRecord(q->ev[slot], q->stream);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipEventRecord", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_elapsed_ms(p3mk_queue *q, int a, int b, float *ms) {
+    if (a < 0 || b < 0 || a >= P3MK_NUM_EVENTS || b >= P3MK_NUM_EVENTS)
+        return p3mk_fail(P3MK_EINVAL, "event slot out of range", 0);
+    hipError_t e = hipEventElapsedTime(ms, q->ev[a], q->ev[b]);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipEventElapsedTime", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0)
+        return P3MK_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return p3mk_fail(P3MK_ENOMEM, "hipMalloc", (int)e);
+    }
+    return P3MK_OK;
+}
+
+extern "C" void p3mk_release(void *p) {
+    if (p)
+        (void)hipFree(p);
+}
+
+extern "C" int p3mk_alloc_host(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0)
+        return P3MK_OK;
+    hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return p3mk_fail(P3MK_ENOMEM, "hipHostMalloc", (int)e);
+    }
+    return P3MK_OK;
+}
+
+extern "C" void p3mk_release_host(void *p) {
+    if (p)
+        (void)hipHostFree(p);
+}
+
+static int copy(void *dst, const void *src, size_t bytes, hipMemcpyKind k, p3mk_queue *q) {
+    if (bytes == 0)
+        return P3MK_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, k, q->stream);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipMemcpyAsync", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_copy_h2d(void *dst, const void *src, size_t bytes, p3mk_queue *q) {
+    return copy(dst, src, bytes, hipMemcpyHostToDevice, q);
+}
+extern "C" int p3mk_copy_d2h(void *dst, const void *src, size_t bytes, p3mk_queue *q) {
+    return copy(dst, src, bytes, hipMemcpyDeviceToHost, q);
+}
+extern "C" int p3mk_copy_d2d(void *dst, const void *src, size_t bytes, p3mk_queue *q) {
+    return copy(dst, src, bytes, hipMemcpyDeviceToDevice, q);
+}
+
+__global__ void k_fill_zero16(uint4 *p, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (; i < n16; i += stride)
+        p[i] = z;
+}
+
+__global__ void k_fill_zero1(unsigned char *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+extern "C" int p3mk_fill_zero(void *p, size_t bytes, p3mk_queue *q) {
+    if (bytes == 0)
+        return P3MK_OK;
+    // hipMalloc returns 256-byte aligned memory; all callers pass such
+    // pointers, so the 16-byte body below covers bytes - bytes % 16.
+    const size_t n16 = bytes / 16, rem = bytes % 16;
+    if (n16) {
+        size_t blocks = (n16 + 255) / 256;
+        if (blocks > 8192)
+            blocks = 8192;
+        hipLaunchKernelGGL(k_fill_zero16, dim3((unsigned)blocks), dim3(256), 0, q->stream,
+                           (uint4 *)p, n16);
+    }
+    if (rem)
+        hipLaunchKernelGGL(k_fill_zero1, dim3(1), dim3(64), 0, q->stream,
+                           (unsigned char *)p + 16 * n16, rem);
+    return p3mk_launch_check("fill_zero");
+}
+
+__global__ void k_axpy(size_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        y[i] += a * x[i];
+}
+
+extern "C" int p3mk_axpy(size_t n, double a, const double *x, double *y, p3mk_queue *q) {
+    if (n == 0)
+        return P3MK_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(k_axpy, dim3((unsigned)blocks), dim3(256), 0, q->stream, n, a, x, y);
+    return p3mk_launch_check("axpy");
+}
+
+__global__ void k_check_positions(int n, double L, const double *__restrict__ x,
+                                  const double *__restrict__ y, const double *__restrict__ z,
+                                  int *bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = 0;
+    if (i < n) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; d++)
+            b |= !(v[d] >= 0.0 && v[d] < L);
+    }
+    // one atomic per wavefront
+    unsigned long long m = __ballot(b);
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicAdd(bad, (int)__popcll(m));
+}
+
+extern "C" int p3mk_check_positions(int n, double L, const double *x, const double *y,
+                                    const double *z, int *bad, int *scratch, p3mk_queue *q) {
+    *bad = 0;
+    if (n <= 0)
+        return P3MK_OK;
+    int rc = p3mk_fill_zero(scratch, sizeof(int), q);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, q->stream, n, L,
+                       x, y, z, scratch);
+    rc = p3mk_launch_check("check_positions");
+    if (rc)
+        return rc;
+    rc = p3mk_copy_d2h(bad, scratch, sizeof(int), q);
+    if (rc)
+        return rc;
+    return p3mk_queue_sync(q);
+}

@@ -1,0 +1,131 @@
+/* Internal C ABI between the host C layer (host/) and the HIP kernels.
+ *
+ * The host layer owns the method logic (which stages run, prefactors,
+ * argument checks, the data_t); everything below is a thin launcher around
+ * one HIP kernel or runtime call.  Device pointers are plain void* / double*;
+ * no HIP type crosses this header.  All launchers return P3MK_OK or an error
+ * code and leave a message for p3mk_error().
+ */
+#ifndef P3M_KERNELS_H
+#define P3M_KERNELS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    P3MK_OK = 0,
+    P3MK_EINVAL = 1,
+    P3MK_ENOMEM = 3,
+    P3MK_EDEVICE = 4,
+    P3MK_ENODEVICE = 5
+};
+
+#define P3MK_NUM_EVENTS 8
+
+const char *p3mk_error(void);
+
+/* ---- runtime -------------------------------------------------------- */
+typedef struct p3mk_queue p3mk_queue; /* one HIP stream + timing events */
+
+int p3mk_device_check(void);
+int p3mk_queue_create(p3mk_queue **q);
+void p3mk_queue_destroy(p3mk_queue *q);
+int p3mk_queue_sync(p3mk_queue *q);
+int p3mk_record(p3mk_queue *q, int slot);
+int p3mk_elapsed_ms(p3mk_queue *q, int a, int b, float *ms);
+int p3mk_alloc(void **p, size_t bytes);
+void p3mk_release(void *p);
+int p3mk_alloc_host(void **p, size_t bytes);
+void p3mk_release_host(void *p);
+int p3mk_copy_h2d(void *dst, const void *src, size_t bytes, p3mk_queue *q);
+int p3mk_copy_d2h(void *dst, const void *src, size_t bytes, p3mk_queue *q);
+int p3mk_copy_d2d(void *dst, const void *src, size_t bytes, p3mk_queue *q);
+/* zero fill by a kernel (safe inside captured graphs) */
+int p3mk_fill_zero(void *p, size_t bytes, p3mk_queue *q);
+/* y[i] += a * x[i] for i < n (FP64) */
+int p3mk_axpy(size_t n, double a, const double *x, double *y, p3mk_queue *q);
+/* positions inside [0, L) and finite?  *bad receives the count of others */
+int p3mk_check_positions(int n, double L, const double *x, const double *y, const double *z,
+                         int *bad, int *scratch, p3mk_queue *q);
+
+/* ---- mesh geometry --------------------------------------------------- */
+typedef struct {
+    int M;          /* mesh points per side (even) */
+    int cao;        /* charge assignment order 1..7 */
+    int r2c;        /* 1: real mesh + half spectrum; 0: complex mesh */
+    int interlaced; /* second mesh, shifted by h/2, in the imaginary part */
+    int fp32;       /* mesh, FFT and interpolation arithmetic in FP32 */
+    double L;       /* box length */
+} p3mk_geom;
+
+/* bytes of the charge / potential mesh (complex M^3 for c2c, real M^3 for r2c) */
+size_t p3mk_mesh_bytes(const p3mk_geom *g);
+/* bytes of one spectrum (complex M^3 for c2c, complex M*M*(M/2+1) for r2c) */
+size_t p3mk_spec_bytes(const p3mk_geom *g);
+
+/* ---- FFT --------------------------------------------------------------- */
+typedef struct p3mk_fft p3mk_fft;
+int p3mk_fft_create(p3mk_fft **f, const p3mk_geom *g);
+void p3mk_fft_destroy(p3mk_fft *f);
+/* forward transform of the mesh along z and y (x is done by convolve):
+ * c2c in place (spec == mesh); r2c mesh (real) -> spec */
+int p3mk_fft_forward_zy(p3mk_fft *f, void *mesh, void *spec, p3mk_queue *q);
+/* x-axis forward pass, multiplication, x-axis backward pass.
+ * ad (nout = 1): out[0] <- G * spec (out[0] may equal spec)
+ * ik (nout = 3): out[d] <- i (2 pi / L) Dn[n_d] G * spec, d = 0, 1, 2 */
+int p3mk_convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spec,
+                  void *const out[3], p3mk_queue *q);
+/* backward transform along y and z: c2c in place (mesh == spec);
+ * r2c spec -> real mesh */
+int p3mk_fft_backward_yz(p3mk_fft *f, void *spec, void *mesh, p3mk_queue *q);
+/* plain 3D transforms of a complex M^3 (c2c geometry) array, for tests:
+ * sign -1 forward, +1 backward, unnormalised, in place */
+int p3mk_fft3d(p3mk_fft *f, void *data, int sign, p3mk_queue *q);
+
+/* ---- particle <-> mesh -------------------------------------------------- */
+/* mesh += charge assignment of n particles (both meshes if interlaced) */
+int p3mk_assign(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
+                const double *q, void *mesh, p3mk_queue *qu);
+/* ik back-interpolation: fk[d*n + i] += -prefac * sum W F_d (meshes F[0..2]) */
+int p3mk_interp_ik(const p3mk_geom *g, int n, const double *x, const double *y,
+                   const double *z, const double *q, void *const F[3], double prefac,
+                   double *fk, p3mk_queue *qu);
+/* ad back-interpolation: fk -= prefac * sum phi dQ; with selfb (81 FP64 on
+ * the device) also f -= self force */
+int p3mk_interp_ad(const p3mk_geom *g, int n, const double *x, const double *y,
+                   const double *z, const double *q, const void *phi, double prefac,
+                   const double *selfb, double *fk, double *f, p3mk_queue *qu);
+
+/* ---- influence function, self force, estimates --------------------------- */
+/* G (M^3 FP64, r_ind order) of method id 0..3, 6, 7 at alpha; mc aliasing
+ * copies per side.  Dn and nshift are computed on the device too. */
+int p3mk_influence(int method, const p3mk_geom *g, double alpha, int mc, double *G,
+                   double *Dn, p3mk_queue *q);
+/* self-force coefficients b (27 x 3 FP64, device) from G; scratch must
+ * hold p3mk_reduce_scratch_doubles(M) doubles */
+int p3mk_self_force_coeffs(const p3mk_geom *g, const double *G, double *b, double *scratch,
+                           p3mk_queue *q);
+size_t p3mk_reduce_scratch_doubles(int M);
+/* k-space error sum of the estimate of method id (p3m_k_space_error_*),
+ * summed over the mesh on the device; *he_q receives the raw sum */
+int p3mk_kspace_error_sum(int method, int M, int cao, double L, double alpha, int mc,
+                          double *he_q, double *scratch, p3mk_queue *q);
+
+/* ---- real space ---------------------------------------------------------- */
+typedef struct p3mk_cells p3mk_cells; /* cell-list workspace */
+int p3mk_cells_create(p3mk_cells **c, int n, double L, double rcut);
+void p3mk_cells_destroy(p3mk_cells *c);
+/* fr += real-space force of every pair i != j with minimum-image distance
+ * <= rcut; *energy (device, may be NULL) += energy.  fp32 selects the
+ * mixed-precision pair arithmetic. */
+int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, const double *y,
+                   const double *z, const double *q, double rcut, double alpha, int fp32,
+                   double *fr, double *energy, p3mk_queue *qu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

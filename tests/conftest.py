@@ -1,0 +1,63 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import __graft_entry__ as entry  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        oracle.build()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    return entry.load_package()
+
+
+def random_system(n, L, seed, charges="alternating"):
+    rng = np.random.default_rng(seed)
+    x, y, z = (rng.uniform(0.0, L, n) for _ in range(3))
+    if charges == "alternating":
+        q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    else:
+        q = rng.normal(0.0, 0.5, n)
+        q -= q.mean()
+    return {"x": x, "y": y, "z": z, "q": q, "L": float(L)}
+
+
+def load_fixture(name):
+    d = np.load(os.path.join(GOLDEN, name))
+    s = {k: np.array(d[k]) for k in ("x", "y", "z", "q")}
+    s["L"] = float(d["L"])
+    return s, np.array(d["f_ewald"])
+
+
+def fold(s):
+    """Positions wrapped into [0, L) (forces are periodic)."""
+    out = dict(s)
+    for k in "xyz":
+        out[k] = np.mod(s[k], s["L"])
+        out[k][out[k] >= s["L"]] = 0.0
+    return out
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP library against the oracle, through the C ABI.
+
+FP64 bar: global relative difference ||F_gpu - F_oracle|| / ||F_oracle||
+<= 1e-12 (the oracle restates the reference's FP64 arithmetic; what remains
+is summation order and ~1-ulp function differences).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import fold, load_fixture, random_system, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+CASES = [
+    # method, N, L, M, cao, alpha
+    (0, 600, 10.0, 16, 5, 1.1),
+    (1, 600, 10.0, 16, 5, 1.1),
+    (2, 600, 10.0, 16, 7, 1.1),
+    (3, 600, 10.0, 16, 7, 1.1),
+    (6, 600, 10.0, 16, 4, 1.1),
+    (7, 600, 10.0, 16, 7, 1.1),
+    (0, 500, 12.0, 24, 3, 0.9),
+    (1, 500, 12.0, 20, 2, 0.9),
+    (2, 500, 12.0, 18, 6, 0.9),
+    (3, 500, 12.0, 22, 1, 0.9),
+    (6, 500, 12.0, 32, 6, 0.9),
+    (7, 500, 12.0, 28, 3, 0.9),
+    (3, 2000, 20.0, 32, 7, 0.8),
+    (1, 2000, 20.0, 32, 5, 0.8),
+]
+
+
+@pytest.mark.parametrize("method,n,L,M,cao,alpha", CASES)
+def test_kspace_force_fp64(pkg, orc, method, n, L, M, cao, alpha):
+    s = random_system(n, L, seed=method * 100 + M)
+    system = pkg.HostSystem.from_dict(s)
+    solver = pkg.Solver(method, system, pkg.parameters(M, cao, alpha))
+    forces = solver.kspace_force()
+    fk_ref, f_ref = orc.kspace_force(method, s, M, cao, alpha)
+    print("method %d M %d cao %d: rel %.3e" % (method, M, cao, rel_err(forces.kspace(), fk_ref)))
+    assert rel_err(forces.kspace(), fk_ref) < FP64_TOL
+    if method in (2, 7):
+        # self-force subtraction lands in f->f
+        assert rel_err(forces.total(), f_ref) < 1e-11
+    else:
+        assert np.all(forces.total() == 0.0)
+    solver.close()
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
+def test_influence_function(pkg, orc, method):
+    s = random_system(50, 9.0, seed=1)
+    M, cao, alpha = 16, 5, 1.3
+    solver = pkg.Solver(method, pkg.HostSystem.from_dict(s), pkg.parameters(M, cao, alpha))
+    G = solver.influence_function()
+    Gref = orc.influence(method, M, cao, 9.0, alpha)
+    assert rel_err(G, Gref) < 1e-13
+    assert np.all((G == 0.0) == (Gref == 0.0))
+    if method in (2, 7):
+        b = solver.self_force_coefficients()
+        bref = orc.self_force_coeffs(M, cao, 9.0, Gref)
+        assert rel_err(b, bref) < 1e-10
+    solver.close()
+
+
+def test_alpha_sweep_accumulates(pkg, orc):
+    """Influence_function per alpha, Kspace_force accumulating into f_k
+    (main.c:375-386, common.c:305)."""
+    s = random_system(300, 10.0, seed=3)
+    M, cao = 16, 7
+    solver = pkg.Solver(7, pkg.HostSystem.from_dict(s), pkg.parameters(M, cao, 0.0))
+    forces = pkg.HostForces(300)
+    fk_sum = np.zeros((3, 300))
+    f_sum = np.zeros((3, 300))
+    for alpha in (0.6, 1.2):
+        solver.set_alpha(alpha)
+        solver.kspace_force(forces)
+        fk, f = orc.kspace_force(7, s, M, cao, alpha)
+        fk_sum += fk
+        f_sum += f
+    assert rel_err(forces.kspace(), fk_sum) < FP64_TOL
+    assert rel_err(forces.total(), f_sum) < 1e-11
+    solver.close()
+
+
+@pytest.mark.parametrize("n,L,rcut,alpha", [(700, 10.0, 3.0, 1.0), (400, 7.0, 3.5, 0.9),
+                                             (300, 10.0, 4.9, 0.7)])
+def test_realteil(pkg, orc, n, L, rcut, alpha):
+    s = random_system(n, L, seed=n)
+    system = pkg.HostSystem.from_dict(s)
+    forces = pkg.realteil(system, rcut, alpha)
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    assert rel_err(forces.real(), fr_ref) < 1e-13
+    assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+def test_calculate_forces_tests_wall(pkg, orc):
+    """Full Calculate_forces (real + k space) on the reference's 300-particle
+    fixture: agrees with the oracle's P3M and with the fixture's Ewald forces
+    at the method's accuracy."""
+    s, f_ewald = load_fixture("tests_wall.npz")
+    s = fold(s)
+    M, cao, alpha, rcut = 32, 7, 1.2, 4.0
+    system = pkg.HostSystem.from_dict(s)
+    solver = pkg.Solver(3, system, pkg.parameters(M, cao, alpha, rcut))
+    forces = solver.calculate_forces()
+    fk_ref, _ = orc.kspace_force(3, s, M, cao, alpha)
+    fr_ref, _ = orc.realteil(s, rcut, alpha)
+    assert rel_err(forces.total(), fk_ref + fr_ref) < FP64_TOL
+    assert orc.rms(forces.total(), f_ewald) < 1e-5
+    solver.close()
+
+
+@pytest.mark.parametrize("method", [1, 3, 7])
+def test_mixed_precision(pkg, orc, method):
+    """FP32 mesh / FFT / interpolation: close to the FP64 result, far below
+    the method's own error."""
+    s = random_system(2000, 16.0, seed=11)
+    M, cao, alpha = 32, 5, 1.0
+    system = pkg.HostSystem.from_dict(s)
+    solver = pkg.Solver(method, system, pkg.parameters(M, cao, alpha), pkg.MIXED)
+    fk32 = solver.kspace_force().kspace()
+    solver.set_precision(pkg.FP64)
+    fk64 = solver.kspace_force().kspace()
+    solver.close()
+    rel = rel_err(fk32, fk64)
+    print("mixed vs fp64 rel %.3e" % rel)
+    assert rel < 2e-5
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
+def test_error_estimates(pkg, orc, method):
+    s = random_system(1000, 10.0, seed=5)
+    M, cao, alpha, rcut = 16, 5, 1.0, 3.0
+    system = pkg.HostSystem.from_dict(s)
+    total, kspace = pkg.error_estimate(method, system, pkg.parameters(M, cao, alpha, rcut))
+    q2 = float(np.sum(s["q"] ** 2))
+    kref = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha)
+    rref = orc.realspace_error(1000, 10.0, q2, rcut, alpha)
+    assert abs(kspace - kref) <= 1e-10 * abs(kref)
+    assert abs(total - np.hypot(kref, rref)) <= 1e-10 * np.hypot(kref, rref)
+
+
+@pytest.mark.parametrize("M", [16, 18, 20, 22, 24, 26, 28, 30, 36, 42, 44, 48, 64, 128])
+def test_fft3d_vs_numpy(pkg, M):
+    """The Stockham transform (radices 2..13) against numpy's FFT."""
+    L = pkg.lib()
+    fn = {}
+    for name, res, args in [
+        ("p3mk_queue_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+        ("p3mk_queue_destroy", None, [ctypes.c_void_p]),
+        ("p3mk_queue_sync", ctypes.c_int, [ctypes.c_void_p]),
+        ("p3mk_fft_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
+        ("p3mk_fft_destroy", None, [ctypes.c_void_p]),
+        ("p3mk_fft3d", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+        ("p3mk_alloc", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+        ("p3mk_release", None, [ctypes.c_void_p]),
+        ("p3mk_copy_h2d", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+        ("p3mk_copy_d2h", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ]:
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+        fn[name] = f
+
+    class Geom(ctypes.Structure):
+        _fields_ = [("M", ctypes.c_int), ("cao", ctypes.c_int), ("r2c", ctypes.c_int),
+                    ("interlaced", ctypes.c_int), ("fp32", ctypes.c_int), ("L", ctypes.c_double)]
+
+    rng = np.random.default_rng(M)
+    a = (rng.standard_normal((M, M, M)) + 1j * rng.standard_normal((M, M, M))).astype(np.complex128)
+    q, plan, buf = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    g = Geom(M, 1, 0, 0, 0, 1.0)
+    assert fn["p3mk_queue_create"](ctypes.byref(q)) == 0
+    assert fn["p3mk_fft_create"](ctypes.byref(plan), ctypes.byref(g)) == 0
+    assert fn["p3mk_alloc"](ctypes.byref(buf), a.nbytes) == 0
+    try:
+        for sign in (-1, 1):
+            assert fn["p3mk_copy_h2d"](buf, a.ctypes.data, a.nbytes, q) == 0
+            assert fn["p3mk_fft3d"](plan, buf, sign, q) == 0
+            out = np.empty_like(a)
+            assert fn["p3mk_copy_d2h"](out.ctypes.data, buf, a.nbytes, q) == 0
+            assert fn["p3mk_queue_sync"](q) == 0
+            ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a) * M ** 3
+            assert rel_err(out, ref) < 1e-14
+    finally:
+        fn["p3mk_release"](buf)
+        fn["p3mk_fft_destroy"](plan)
+        fn["p3mk_queue_destroy"](q)
