@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X P3M force path.
+
+One step = one full force evaluation of the workload (Calculate_forces:
+real-space erfc part + k-space part: charge assignment, FFTs with the
+influence-function multiply, back-interpolation), device-resident, on the
+library's stream.  Prints one JSON line.
+
+Default workload: 1,000,000 charges Gaussian about the centre of a box of
+side 100 (sigma 12.5), +-1 neutral, P3M ad interlaced (method 3), mesh 128^3,
+cao 7, rcut 4, alpha 0.75 -- the configuration the headline target is quoted
+on.  `value` is the mixed-precision form; the FP64 form is timed beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+WORKLOADS = {
+    "gaussian-1M-ad-i-m128": dict(method=3, N=1_000_000, L=100.0, M=128, cao=7, rcut=4.0, alpha=0.75,
+                                  gen="gaussian", seed=3),
+    "uniform-100k-ik-i-m64": dict(method=1, N=100_000, L=46.4, M=64, cao=5, rcut=3.0, alpha=0.9,
+                                  gen="uniform", seed=2),
+    "uniform-1k-ad-r-m16": dict(method=7, N=1000, L=10.0, M=16, cao=7, rcut=3.0, alpha=1.0,
+                                gen="alternating", seed=1),
+    "uniform-32k-ik-m32": dict(method=0, N=32768, L=32.0, M=32, cao=5, rcut=3.0, alpha=1.0,
+                               gen="normal", seed=5),
+}
+DEFAULT = "gaussian-1M-ad-i-m128"
+
+
+def make_system(w):
+    rng = np.random.default_rng(w["seed"])
+    N, L = w["N"], w["L"]
+    if w["gen"] == "gaussian":
+        def coord():
+            out = np.empty(0)
+            while out.size < N:
+                v = rng.normal(L / 2, L / 8, 2 * N)
+                out = np.concatenate([out, v[(v >= 0.0) & (v < L)]])
+            return out[:N]
+        x, y, z = coord(), coord(), coord()
+    else:
+        x, y, z = (rng.uniform(0.0, L, N) for _ in range(3))
+    if w["gen"] == "alternating":
+        q = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+    elif w["gen"] == "normal":
+        q = rng.normal(0.0, 0.5, N)
+        q -= q.mean()
+    else:
+        q = rng.permutation(np.repeat([1.0, -1.0], N // 2 + 1)[:N])
+    return {"x": x, "y": y, "z": z, "q": q, "L": L}
+
+
+def algorithmic_bytes(w, s):
+    """Bytes per stage from the per-unit figures of the k-space/real-space
+    pipeline: s = bytes per value (8 FP64, 4 FP32)."""
+    N, M, method = w["N"], w["M"], w["method"]
+    nm = 2 if method in (1, 3) else 1
+    nout = 1 if method in (2, 3, 7) else 3
+    r2c = method in (6, 7)
+    C = M * M * (M // 2 + 1) if r2c else M ** 3
+    return {
+        "assign": N * 4 * s + nm * M ** 3 * s,
+        "fft_multiply": 2 * C * 2 * s + C * 8 + nout * 2 * C * 2 * s,
+        "interp": N * 4 * s + N * 3 * s + nout * nm * M ** 3 * s,
+        "realspace": N * 4 * s + N * 3 * s,
+    }
+
+
+def run_form(pkg, torch, w, system, dev, precision, steps, warmup):
+    params = pkg.parameters(w["M"], w["cao"], w["alpha"], w["rcut"])
+    solver = pkg.Solver(w["method"], system, params, precision)
+    N, L = w["N"], float(w["L"])
+    f = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    fk = torch.zeros_like(f)
+    fr = torch.zeros_like(f)
+    en = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ptr = [t.data_ptr() for t in (dev["x"], dev["y"], dev["z"], dev["q"])]
+
+    def step():
+        solver.calculate_forces_device(N, L, *ptr, w["rcut"], w["alpha"], f.data_ptr(), fk.data_ptr(),
+                                       fr.data_ptr(), en.data_ptr())
+
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        step()
+    solver.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    solver.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # per-stage kernel durations: HIP events recorded by the library on its
+    # own stream around each stage, read back after every step
+    stages = []
+    for _ in range(steps):
+        step()
+        stages.append(solver.stage_times())
+    st = np.mean(np.array(stages), axis=0)
+    forces = f.cpu().numpy().reshape(3, N)
+    solver.close()
+    return {
+        "ms_per_step": 1e3 * elapsed / steps,
+        "stage_ms": {"assign": st[0], "fft_multiply": st[1], "interp": st[2], "kspace": st[3],
+                     "realspace": st[4]},
+        "forces": forces,
+    }
+
+
+def roofline(w, res, s):
+    byts = algorithmic_bytes(w, s)
+    stages = {k: res["stage_ms"][k] for k in ("assign", "fft_multiply", "interp", "realspace")}
+    dom = max(stages, key=stages.get)
+    per = {}
+    for k, ms in stages.items():
+        gbs = byts[k] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        per[k] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                  "bytes": int(byts[k])}
+    return dom, per
+
+
+def cpu_baseline(w, sysd):
+    """The oracle (plain-C FP64 restatement of the reference) on this host:
+    one full k-space evaluation plus the real-space force of a 2000-particle
+    sample (cell-list restatement of Realteil), scaled to N."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    N, L, M, cao, alpha = w["N"], w["L"], w["M"], w["cao"], w["alpha"]
+    G = orc.influence(w["method"], M, cao, L, alpha)
+    b = orc.self_force_coeffs(M, cao, L, G) if w["method"] in (2, 7) else None
+    t0 = time.perf_counter()
+    orc.kspace_force(w["method"], sysd, M, cao, alpha, G=G, b=b)
+    tk = time.perf_counter() - t0
+    sample = min(N, 2000)
+    t0 = time.perf_counter()
+    orc.realteil(sysd, w["rcut"], alpha, cells=True, i_range=(0, sample))
+    tr = (time.perf_counter() - t0) * N / sample
+    t_eval = tk + tr
+    return {"value": N / t_eval, "unit": "particle-force evals/s", "cores": 1, "kind": "port",
+            "sample": "oracle (FP64 C, 1 thread): full k-space evaluation (%.1f s) + real space of "
+                      "%d of %d particles on a cell list, scaled (%.1f s)" % (tk, sample, N, tr)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the multi-GPU path is design-only "
+              "(--gpus %d not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__ as entry
+
+    pkg = entry.load_package()
+    w = WORKLOADS[args.workload]
+    sysd = make_system(w)
+    system = pkg.HostSystem.from_dict(sysd)
+    dev = {k: torch.tensor(sysd[k], dtype=torch.float64, device="cuda") for k in "xyzq"}
+    N = w["N"]
+
+    r64 = run_form(pkg, torch, w, system, dev, pkg.FP64, args.steps, args.warmup)
+    rmx = run_form(pkg, torch, w, system, dev, pkg.MIXED, args.steps, args.warmup)
+
+    f64, fmx = r64["forces"], rmx["forces"]
+    rms_mixed_vs_fp64 = float(np.sqrt(np.sum((fmx - f64) ** 2) / N))
+    rms_force = float(np.sqrt(np.sum(f64 ** 2) / N))
+    params = pkg.parameters(w["M"], w["cao"], w["alpha"], w["rcut"])
+    est_total, est_k = pkg.error_estimate(w["method"], system, params)
+
+    dom, per = roofline(w, rmx, 4)
+    _, per64 = roofline(w, r64, 8)
+    value = N / (rmx["ms_per_step"] * 1e-3)
+    value64 = N / (r64["ms_per_step"] * 1e-3)
+    out = {
+        "metric": "particle-force evals/sec at rms err <=1e-4 (mixed vs FP64 GPU), HBM GB/s",
+        "value": round(value, 1),
+        "unit": "particle-force evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 mesh/FFT/interp/pairs, f64 influence+accumulation",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": args.workload, "method": w["method"], "N": N, "box": w["L"],
+                   "mesh": w["M"], "cao": w["cao"], "rcut": w["rcut"], "alpha": w["alpha"],
+                   "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": per[dom]["GB/s"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": per[dom]["frac_hbm"],
+            "traffic": None,
+        },
+        "stages_mixed": per,
+        "fp64": {"value": round(value64, 1), "ms_per_step": round(r64["ms_per_step"], 4),
+                 "stages": per64},
+        "speedup_mixed_vs_fp64": round(r64["ms_per_step"] / rmx["ms_per_step"], 3),
+        "accuracy": {"rms_mixed_vs_fp64": rms_mixed_vs_fp64, "rms_force": rms_force,
+                     "estimate_total": est_total, "estimate_kspace": est_k},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, sysd)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -249,6 +249,15 @@ int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f);
 int p3m_hip_realteil_device(data_t *d, int n, double length, const double *x, const double *y,
                             const double *z, const double *q, double rcut, double alpha,
                             double *fr, double *energy);
+/* Device-resident Calculate_forces (common.c:253-312): zero f, fk, fr
+ * (3n each, device, FP64, x/y/z blocks); fr += real space if rcut != 0;
+ * fk += k space (f -= self force for ad / ad-r); f += fk + fr.  energy
+ * (device double, may be NULL) receives the real-space energy.  Enqueued on
+ * the library's stream; see p3m_hip_synchronize. */
+int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const double *x,
+                                    const double *y, const double *z, const double *q, double rcut,
+                                    double alpha, double *f, double *fk, double *fr,
+                                    double *energy);
 /* Calculate_forces (common.c:253-312) on the GPU: zero f, f_k, f_r; real
  * space if p->rcut != 0; Kspace_force; f += f_k + f_r. */
 int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, data_t *d,
@@ -264,9 +273,11 @@ int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, do
  * the estimate interpolates it as find_error.c:84-107 does. */
 int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101);
 
-/* Stage times (ms) of the last Kspace_force: assignment, FFT + multiply,
- * back-interpolation, total (HIP events). */
-int p3m_hip_stage_times(data_t *d, double out[4]);
+/* Stage times (ms, HIP events on the library's stream) of the last calls:
+ * [0] charge assignment, [1] FFTs + influence-function multiply,
+ * [2] back-interpolation, [3] whole Kspace_force, [4] last real-space call
+ * (0 if none).  Waits for the stream. */
+int p3m_hip_stage_times(data_t *d, double out[5]);
 /* Copy the influence function (M^3 doubles, r_ind order) to the host. */
 int p3m_hip_get_influence_function(data_t *d, double *G);
 /* Copy the self-force coefficients (27 x 3 doubles) to the host. */

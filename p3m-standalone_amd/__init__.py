@@ -106,6 +106,7 @@ def lib():
             "p3m_hip_realteil": (i, [_SP, _PP, _FP]),
             "p3m_hip_realteil_device": (i, [_DPT, i, d, vp, vp, vp, vp, d, d, vp, vp]),
             "p3m_hip_calculate_forces": (i, [ctypes.POINTER(Method), _SP, _PP, _DPT, _FP]),
+            "p3m_hip_calculate_forces_device": (i, [_DPT, i, d, vp, vp, vp, vp, d, d, vp, vp, vp, vp]),
             "p3m_hip_error": (i, [i, _SP, _PP, _dp, _dp]),
             "p3m_hip_set_error_table": (i, [i, i, i, _dp]),
             "p3m_hip_stage_times": (i, [_DPT, _dp]),
@@ -152,7 +153,8 @@ class _Vec:
     """A vector_array_t over three numpy arrays (kept alive with it)."""
 
     def __init__(self, x, y, z):
-        self.arrays = [_c(x), _c(y), _c(z)]
+        # private copies: the C side writes through these pointers
+        self.arrays = [np.array(a, dtype=np.float64, copy=True) for a in (x, y, z)]
         self.ptrs = (_dp * 3)(*[a.ctypes.data_as(_dp) for a in self.arrays])
         self.c = VectorArray(len(self.arrays[0]), self.ptrs[0], self.ptrs[1], self.ptrs[2],
                              ctypes.cast(self.ptrs, ctypes.POINTER(_dp)))
@@ -187,6 +189,7 @@ class HostForces:
     def __init__(self, n):
         z = np.zeros(n)
         self.f, self.f_k, self.f_r = _Vec(z, z, z), _Vec(z, z, z), _Vec(z, z, z)
+        assert len({a.ctypes.data for v in (self.f, self.f_k, self.f_r) for a in v.arrays}) == 9
         self.c = Forces(ctypes.pointer(self.f.c), ctypes.pointer(self.f_k.c), ctypes.pointer(self.f_r.c))
         self.ptr = ctypes.pointer(self.c)
 
@@ -259,7 +262,7 @@ class Solver:
         return b
 
     def stage_times(self):
-        t = np.zeros(4)
+        t = np.zeros(5)
         check(lib().p3m_hip_stage_times(self.d, t.ctypes.data_as(_dp)))
         return t
 
@@ -269,6 +272,10 @@ class Solver:
 
     def realteil_device(self, n, length, x, y, z, q, rcut, alpha, fr, energy):
         check(lib().p3m_hip_realteil_device(self.d, n, length, x, y, z, q, rcut, alpha, fr, energy))
+
+    def calculate_forces_device(self, n, length, x, y, z, q, rcut, alpha, f, fk, fr, energy):
+        check(lib().p3m_hip_calculate_forces_device(self.d, n, length, x, y, z, q, rcut, alpha, f,
+                                                    fk, fr, energy))
 
     def synchronize(self):
         check(lib().p3m_hip_synchronize(self.d))

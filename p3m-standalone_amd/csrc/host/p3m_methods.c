@@ -83,7 +83,8 @@ typedef struct {
     p3mk_cells *cells;
     double cells_L, cells_rc;
     int have_G;
-    float stage_ms[4];
+    int have_rs_events;
+    float stage_ms[5];
 } p3m_ctx;
 
 static p3m_ctx *ctx_of(data_t *d) { return d ? (p3m_ctx *)d->method_data : NULL; }
@@ -341,6 +342,11 @@ static int collect_stage_times(p3m_ctx *c) {
     }
     TRYK(p3mk_elapsed_ms(c->q, 0, 3, &t), "event timing");
     c->stage_ms[3] = t;
+    c->stage_ms[4] = 0.0f;
+    if (c->have_rs_events) {
+        TRYK(p3mk_elapsed_ms(c->q, 4, 5, &t), "event timing");
+        c->stage_ms[4] = t;
+    }
     return P3M_HIP_OK;
 }
 
@@ -439,14 +445,14 @@ int p3m_hip_kspace_force(system_t *s, parameters_t *p, data_t *d, forces_t *f) {
     return P3M_HIP_OK;
 }
 
-int p3m_hip_stage_times(data_t *d, double out[4]) {
+int p3m_hip_stage_times(data_t *d, double out[5]) {
     p3m_ctx *c = ctx_of(d);
     int k;
     if (c == NULL || out == NULL)
         return fail(P3M_HIP_EINVAL, "bad arguments");
     TRYK(p3mk_queue_sync(c->q), "sync");
     TRY(collect_stage_times(c));
-    for (k = 0; k < 4; k++)
+    for (k = 0; k < 5; k++)
         out[k] = c->stage_ms[k];
     return P3M_HIP_OK;
 }
@@ -497,9 +503,38 @@ int p3m_hip_realteil_device(data_t *d, int n, double length, const double *x, co
     if (!(rcut > 0.0) || rcut > 0.5 * length)
         return fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
     TRY(ensure_cells(c, c->n, length, rcut));
+    TRYK(p3mk_record(c->q, 4), "event");
     TRYK(p3mk_realspace(c->cells, n, length, x, y, z, q, rcut, alpha, c->geom.fp32, fr, energy,
                         c->q),
          "real space");
+    TRYK(p3mk_record(c->q, 5), "event");
+    c->have_rs_events = 1;
+    return P3M_HIP_OK;
+}
+
+int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const double *x,
+                                    const double *y, const double *z, const double *q, double rcut,
+                                    double alpha, double *f, double *fk, double *fr,
+                                    double *energy) {
+    p3m_ctx *c = ctx_of(d);
+    if (c == NULL)
+        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+    if (f == NULL || fk == NULL || fr == NULL)
+        return fail(P3M_HIP_EINVAL, "force arrays must not be NULL");
+    if (n != c->n)
+        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
+    /* common.c:259-263 */
+    TRYK(p3mk_fill_zero(f, sizeof(double) * 3 * (size_t)n, c->q), "zero");
+    TRYK(p3mk_fill_zero(fk, sizeof(double) * 3 * (size_t)n, c->q), "zero");
+    TRYK(p3mk_fill_zero(fr, sizeof(double) * 3 * (size_t)n, c->q), "zero");
+    if (energy != NULL)
+        TRYK(p3mk_fill_zero(energy, sizeof(double), c->q), "zero");
+    if (rcut != 0.0) /* common.c:275-279 */
+        TRY(p3m_hip_realteil_device(d, n, length, x, y, z, q, rcut, alpha, fr, energy));
+    TRY(p3m_hip_kspace_force_device(d, n, length, x, y, z, q, fk, f));
+    /* common.c:300-304: f += f_k + f_r */
+    TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fk, f, c->q), "sum");
+    TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fr, f, c->q), "sum");
     return P3M_HIP_OK;
 }
 
