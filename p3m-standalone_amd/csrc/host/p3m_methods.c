@@ -81,6 +81,7 @@ typedef struct {
     double *energy; /* device scalar */
     int *iscratch;
     p3mk_cells *cells;
+    p3mk_psort *psort;
     double cells_L, cells_rc;
     int have_G;
     int have_rs_events;
@@ -113,6 +114,7 @@ static void ctx_free(p3m_ctx *c) {
     p3mk_release(c->iscratch);
     p3mk_release_host(c->hforce);
     p3mk_cells_destroy(c->cells);
+    p3mk_psort_destroy(c->psort);
     p3mk_queue_destroy(c->q);
     free(c);
 }
@@ -151,6 +153,7 @@ static int ctx_alloc(p3m_ctx *c) {
         TRYK(p3mk_fft_create(&c->fft[1], &g32), "FFT plan");
     }
     TRYK(p3mk_alloc(&c->mesh, mesh_b), "mesh");
+    TRYK(p3mk_psort_create(&c->psort, &g64, c->n), "particle sort");
     if (is_r2c(c->method)) {
         int nspec = is_ad(c->method) ? 1 : 3;
         for (k = 0; k < nspec; k++)
@@ -293,8 +296,7 @@ static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, c
     const p3mk_geom *g = &c->geom;
     const double L = g->L;
     TRYK(p3mk_record(c->q, 0), "event");
-    TRYK(p3mk_fill_zero(c->mesh, p3mk_mesh_bytes(g), c->q), "zero mesh");
-    TRYK(p3mk_assign(g, n, x, y, z, q, c->mesh, c->q), "charge assignment");
+    TRYK(p3mk_assign(g, n, x, y, z, q, c->psort, c->mesh, c->q), "charge assignment");
     TRYK(p3mk_record(c->q, 1), "event");
     if (!is_ad(c->method)) {
         /* ik: F_d = i (2 pi/L) Dn_d G Q^ (p3m-ik.c:196-216, p3m-ik-i.c:187-212,
@@ -310,7 +312,7 @@ static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, c
             TRYK(p3mk_fft_backward_yz(fft, outs[k], c->F[k], c->q), "backward FFT");
         TRYK(p3mk_record(c->q, 2), "event");
         /* assign_forces* with prefactor 1/(2 L^3) (p3m-ik.c:227) */
-        TRYK(p3mk_interp_ik(g, n, x, y, z, q, c->F, 1.0 / (2.0 * L * L * L), fk, c->q),
+        TRYK(p3mk_interp_ik(g, n, c->psort, c->F, 1.0 / (2.0 * L * L * L), fk, c->q),
              "back-interpolation");
     } else {
         /* ad: phi = FFT^-1[G Q^] (p3m-ad.c:167-181, p3m-ad-i.c:161-175,
@@ -325,7 +327,7 @@ static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, c
         TRYK(p3mk_record(c->q, 2), "event");
         /* assign_forces_ad* with prefactor M/L^3 (p3m-ad.c:187) and the
          * self-force subtraction of methods 2 and 7 (p3m-ad.c:189-191) */
-        TRYK(p3mk_interp_ad(g, n, x, y, z, q, c->mesh, (double)g->M / (L * L * L),
+        TRYK(p3mk_interp_ad(g, n, c->psort, c->mesh, (double)g->M / (L * L * L),
                             has_self_force(c->method) ? c->selfb : NULL, fk, f, c->q),
              "back-interpolation");
     }

@@ -6,10 +6,21 @@
 // the imaginary part).  Weights are evaluated at the reference's quantised
 // offsets (device_common.hpp) instead of read from 7 MB tables.
 //
+// Assignment is owner-computes: particles are counting-sorted by the mesh
+// tile (T^3 points, T | M, T >= P + interlaced) of their mesh-0 stencil
+// base; one workgroup owns one tile, walks the particles of its own and its
+// lower neighbour tiles (the only ones whose stencils reach it), accumulates
+// the in-tile part of each stencil in LDS and writes the tile once with
+// plain stores.  No HBM atomics, no zero fill of the mesh, and clustered
+// particle distributions only cost LDS atomics inside the owning workgroup.
+//
 // Back-interpolation (charge-assign.c:601-643, 646-688, 727-769, 849-885,
-// 1352-1391, 1468-1502) recomputes the same weights instead of caching
-// cf / dQ / ca_ind per particle, so it streams positions, not N*P^3 caches.
+// 1352-1391, 1468-1502) walks the same sorted order (neighbouring threads
+// gather from neighbouring mesh points) and recomputes the weights instead
+// of caching cf / dQ / ca_ind per particle.
 #include <hip/hip_runtime.h>
+
+#include <new>
 
 #include "device_common.hpp"
 #include "k_internal.hpp"
@@ -20,74 +31,193 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// ---- assignment ---------------------------------------------------------
+// ---- sort by mesh tile ------------------------------------------------------
 
-template <int P, typename T, bool R2C>
-__global__ void __launch_bounds__(kThreads)
-k_assign_atomic(int n, const double *__restrict__ x, const double *__restrict__ y,
-                const double *__restrict__ z, const double *__restrict__ q, double Hi, int M,
-                T *__restrict__ mesh) {
+__device__ __forceinline__ int axis_base(double r, double Hi, int P, int M) {
+    int b;
+    double xq;
+    locate(r, Hi, (double)((P - 1) / 2), 0.0, M, b, xq);
+    return b;
+}
+
+__global__ void k_tile_count(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, double Hi, int P, int M, int T, int nt,
+                             int *__restrict__ key, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ii = blockIdx.y; // 0: mesh 0, 1: h/2-shifted mesh 1
     if (i >= n)
         return;
-    const double half = 0.5 * ii;
-    int b0, b1, b2;
-    T w0[P], w1[P], w2[P];
-    axis_weights<P, T>(x[i], Hi, half, M, b0, w0);
-    axis_weights<P, T>(y[i], Hi, half, M, b1, w1);
-    axis_weights<P, T>(z[i], Hi, half, M, b2, w2);
-    const T qi = (T)q[i];
-    int kk[P];
-#pragma unroll
-    for (int c = 0; c < P; c++)
-        kk[c] = wrap1(b2 + c, M);
+    const int tx = axis_base(x[i], Hi, P, M) / T, ty = axis_base(y[i], Hi, P, M) / T,
+              tz = axis_base(z[i], Hi, P, M) / T;
+    const int k = (tx * nt + ty) * nt + tz;
+    key[i] = k;
+    atomicAdd(&count[k], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_scan_tiles(const int *__restrict__ count, int m,
+                                                     int *__restrict__ start) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (m + 1023) / 1024;
+    const int b = t * chunk, e = min(m, b + chunk);
+    int s = 0;
+    for (int k = b; k < e; k++)
+        s += count[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = (t == 0) ? 0 : part[t - 1];
+    for (int k = b; k < e; k++) {
+        start[k] = run;
+        run += count[k];
+    }
+    if (t == 1023)
+        start[m] = part[1023];
+}
+
+__global__ void k_tile_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ z, const double *__restrict__ q,
+                               const int *__restrict__ key, const int *__restrict__ start,
+                               int *__restrict__ cursor, int *__restrict__ perm,
+                               double4 *__restrict__ sorted) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int k = key[i];
+    const int p = start[k] + atomicAdd(&cursor[k], 1);
+    perm[p] = i;
+    sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+}
+
+// ---- owner-computes assignment -----------------------------------------------
+
+// LDS accumulators: FP64 atomics, or FP32 atomics with the TwoSum rounding
+// error of every add collected in a second FP32 array (compensated sum).
+template <typename T> struct LdsAcc;
+template <> struct LdsAcc<double> {
+    static constexpr int kArrays = 1;
+    __device__ static __forceinline__ void add(double *a, int npts, int idx, double v) {
+        (void)npts;
+        atomicAdd(&a[idx], v);
+    }
+    __device__ static __forceinline__ double value(const double *a, int npts, int idx) {
+        (void)npts;
+        return a[idx];
+    }
+};
+template <> struct LdsAcc<float> {
+    static constexpr int kArrays = 2;
+    __device__ static __forceinline__ void add(float *a, int npts, int idx, float v) {
+        const float old = atomicAdd(&a[idx], v);
+        float s, e;
+        two_sum(old, v, s, e);
+        if (e != 0.0f)
+            atomicAdd(&a[npts + idx], e);
+    }
+    __device__ static __forceinline__ float value(const float *a, int npts, int idx) {
+        return a[idx] + a[npts + idx];
+    }
+};
+
+// In-tile part of one axis of a stencil: a in [lo, hi) maps to local
+// points off + a (the stencil wraps at most once; the tile is contiguous).
+template <int P>
+__device__ __forceinline__ void clip_axis(int base, int t0, int T, int M, int &lo, int &hi,
+                                          int &off) {
+    lo = P;
+    hi = 0;
+    off = 0;
 #pragma unroll
     for (int a = 0; a < P; a++) {
-        const int ia = wrap1(b0 + a, M);
-        const T t0 = qi * w0[a];
-#pragma unroll
-        for (int b = 0; b < P; b++) {
-            const int jb = wrap1(b1 + b, M);
-            const T t1 = t0 * w1[b];
-            const long row = ((long)ia * M + jb) * M;
-#pragma unroll
-            for (int c = 0; c < P; c++) {
-                const T v = t1 * w2[c];
-                if (R2C)
-                    atomicAdd(&mesh[row + kk[c]], v);
-                else
-                    atomicAdd(&mesh[2 * (row + kk[c]) + ii], v);
+        int m = base + a;
+        if (m >= M)
+            m -= M;
+        const int l = m - t0;
+        if (l >= 0 && l < T) {
+            if (a < lo) {
+                lo = a;
+                off = l - a;
             }
+            hi = a + 1;
         }
     }
 }
 
-template <typename T, bool R2C>
-int launch_assign(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
-                  const double *q, void *mesh, p3mk_queue *qu) {
-    const double Hi = (double)g->M / g->L;
-    dim3 grid((n + kThreads - 1) / kThreads, g->interlaced ? 2 : 1);
-    T *m = (T *)mesh;
-    hipStream_t s = p3mk_stream(qu);
-    switch (g->cao) {
-#define CASE(P)                                                                                  \
-    case P:                                                                                      \
-        hipLaunchKernelGGL((k_assign_atomic<P, T, R2C>), grid, dim3(kThreads), 0, s, n, x, y, z, \
-                           q, Hi, g->M, m);                                                     \
-        break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-#undef CASE
-    default:
-        return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
+template <int P, typename T, bool R2C, bool IL>
+__global__ void __launch_bounds__(kThreads)
+k_assign_tiles(const double4 *__restrict__ sorted, const int *__restrict__ tstart, int nt, int Tt,
+               int M, double Hi, T *__restrict__ mesh) {
+    extern __shared__ unsigned char smem_raw[];
+    T *acc = reinterpret_cast<T *>(smem_raw);
+    constexpr int NM = IL ? 2 : 1;
+    const int npts = Tt * Tt * Tt;
+    const int nall = NM * npts; // points of all meshes
+    for (int k = threadIdx.x; k < LdsAcc<T>::kArrays * nall; k += blockDim.x)
+        acc[k] = T(0);
+    __syncthreads();
+
+    const int tile = blockIdx.x;
+    const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+    const int ds = (nt > 1) ? 1 : 0; // source tiles t and t-1 per axis
+    for (int sx = 0; sx <= ds; sx++)
+        for (int sy = 0; sy <= ds; sy++)
+            for (int sz = 0; sz <= ds; sz++) {
+                const int ux = (tx - sx + nt) % nt, uy = (ty - sy + nt) % nt, uz = (tz - sz + nt) % nt;
+                const int src = (ux * nt + uy) * nt + uz;
+                const int pb = tstart[src], pe = tstart[src + 1];
+                for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+                    const double4 r = sorted[p];
+#pragma unroll
+                    for (int ii = 0; ii < NM; ii++) {
+                        int b0, b1, b2;
+                        T w0[P], w1[P], w2[P];
+                        const double half = 0.5 * ii;
+                        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+                        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+                        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
+                        int l0, h0, o0, l1, h1, o1, l2, h2, o2;
+                        clip_axis<P>(b0, t0x, Tt, M, l0, h0, o0);
+                        clip_axis<P>(b1, t0y, Tt, M, l1, h1, o1);
+                        clip_axis<P>(b2, t0z, Tt, M, l2, h2, o2);
+                        if (h0 <= l0 || h1 <= l1 || h2 <= l2)
+                            continue;
+                        const T qv = (T)r.w;
+                        T *a = acc + ii * npts;
+                        for (int a0 = l0; a0 < h0; a0++) {
+                            const T t0 = qv * w0[a0];
+                            for (int a1 = l1; a1 < h1; a1++) {
+                                const T t1 = t0 * w1[a1];
+                                const int row = ((a0 + o0) * Tt + (a1 + o1)) * Tt + o2;
+                                for (int a2 = l2; a2 < h2; a2++)
+                                    LdsAcc<T>::add(a, nall, row + a2, t1 * w2[a2]);
+                            }
+                        }
+                    }
+                }
+            }
+    __syncthreads();
+    // write the tile: every mesh point has exactly one owner
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+        const int lz = k % Tt, ly = (k / Tt) % Tt, lx = k / (Tt * Tt);
+        const long g = ((long)(t0x + lx) * M + (t0y + ly)) * M + (t0z + lz);
+        if (R2C) {
+            mesh[g] = LdsAcc<T>::value(acc, nall, k);
+        } else {
+            c2_t<T> v;
+            v.x = LdsAcc<T>::value(acc, nall, k);
+            v.y = IL ? LdsAcc<T>::value(acc, nall, npts + k) : T(0);
+            reinterpret_cast<c2_t<T> *>(mesh)[g] = v;
+        }
     }
-    return p3mk_launch_check("assign");
 }
 
-// ---- ik back-interpolation ----------------------------------------------
+// ---- back-interpolation -----------------------------------------------------
 
-// Reads mesh value of component ii (c2c: re / im of a complex mesh) or the
-// real mesh (r2c).
 template <typename T, bool R2C>
 __device__ __forceinline__ T mesh_val(const T *__restrict__ m, long idx, int ii) {
     return R2C ? m[idx] : m[2 * idx + ii];
@@ -107,31 +237,28 @@ template <> struct Acc<float> {
 
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
-k_interp_ik(int n, const double *__restrict__ x, const double *__restrict__ y,
-            const double *__restrict__ z, const double *__restrict__ q, double Hi, int M,
-            const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
+k_interp_ik(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
+            int M, const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
             double prefac, double *__restrict__ fk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
         return;
     constexpr int NM = IL ? 2 : 1;
     Acc<T> acc[3];
-    const double xi = x[i], yi = y[i], zi = z[i];
+    const double4 r = sorted[t];
     // interlaced: each mesh contributes with 0.5 (charge-assign.c:674-675)
-    const T scale = (T)(IL ? 0.5 * prefac : prefac) * (T)q[i];
-#pragma unroll
+    const T scale = (T)(IL ? 0.5 * prefac : prefac) * (T)r.w;
     for (int ii = 0; ii < NM; ii++) {
         int b0, b1, b2;
         T w0[P], w1[P], w2[P];
         const double half = 0.5 * ii;
-        axis_weights<P, T>(xi, Hi, half, M, b0, w0);
-        axis_weights<P, T>(yi, Hi, half, M, b1, w1);
-        axis_weights<P, T>(zi, Hi, half, M, b2, w2);
+        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
         int kk[P];
 #pragma unroll
         for (int c = 0; c < P; c++)
             kk[c] = wrap1(b2 + c, M);
-#pragma unroll
         for (int a = 0; a < P; a++) {
             const int ia = wrap1(b0 + a, M);
 #pragma unroll
@@ -150,56 +277,28 @@ k_interp_ik(int n, const double *__restrict__ x, const double *__restrict__ y,
             }
         }
     }
+    const int i = perm[t];
     fk[i] += acc[0].value();
     fk[(long)n + i] += acc[1].value();
     fk[2L * n + i] += acc[2].value();
 }
 
-template <typename T, bool R2C>
-int launch_interp_ik(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
-                     const double *q, void *const F[3], double prefac, double *fk,
-                     p3mk_queue *qu) {
-    const double Hi = (double)g->M / g->L;
-    dim3 grid((n + kThreads - 1) / kThreads);
-    const T *F0 = (const T *)F[0], *F1 = (const T *)F[1], *F2 = (const T *)F[2];
-    hipStream_t s = p3mk_stream(qu);
-    const bool il = g->interlaced != 0;
-    switch (g->cao) {
-#define CASE(P)                                                                                   \
-    case P:                                                                                       \
-        if (il)                                                                                   \
-            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, true>), grid, dim3(kThreads), 0, s, n, x, \
-                               y, z, q, Hi, g->M, F0, F1, F2, prefac, fk);                       \
-        else                                                                                      \
-            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, false>), grid, dim3(kThreads), 0, s, n, x, \
-                               y, z, q, Hi, g->M, F0, F1, F2, prefac, fk);                       \
-        break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
-#undef CASE
-    default:
-        return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
-    }
-    return p3mk_launch_check("interp_ik");
-}
-
-// ---- ad back-interpolation + self force ----------------------------------
-
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
-k_interp_ad(int n, const double *__restrict__ x, const double *__restrict__ y,
-            const double *__restrict__ z, const double *__restrict__ q, double Hi, double Leni,
-            int M, const T *__restrict__ phi, double prefac, const double *__restrict__ selfb,
-            double h, double *__restrict__ fk, double *__restrict__ f) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
+            double Leni, int M, const T *__restrict__ phi, double prefac,
+            const double *__restrict__ selfb, double h, double *__restrict__ fk,
+            double *__restrict__ f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
         return;
     constexpr int NM = IL ? 2 : 1;
     Acc<T> acc[3];
-    const double xi = x[i], yi = y[i], zi = z[i], qi = q[i];
+    const double4 r = sorted[t];
+    const double xi = r.x, yi = r.y, zi = r.z, qi = r.w;
     // dQ_d = W'.W.W q/L; force_d -= prefac * phi * dQ_d, interlaced x 0.5
     // (charge-assign.c:1123-1125, 1374-1377, 758-760)
     const T scale = (T)((IL ? 0.5 * prefac : prefac) * (qi * Leni));
-#pragma unroll
     for (int ii = 0; ii < NM; ii++) {
         int b0, b1, b2;
         T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
@@ -211,7 +310,6 @@ k_interp_ad(int n, const double *__restrict__ x, const double *__restrict__ y,
 #pragma unroll
         for (int c = 0; c < P; c++)
             kk[c] = wrap1(b2 + c, M);
-#pragma unroll
         for (int a = 0; a < P; a++) {
             const int ia = wrap1(b0 + a, M);
 #pragma unroll
@@ -229,6 +327,7 @@ k_interp_ad(int n, const double *__restrict__ x, const double *__restrict__ y,
             }
         }
     }
+    const int i = perm[t];
     fk[i] += acc[0].value();
     fk[(long)n + i] += acc[1].value();
     fk[2L * n + i] += acc[2].value();
@@ -253,25 +352,143 @@ k_interp_ad(int n, const double *__restrict__ x, const double *__restrict__ y,
     }
 }
 
+// tile side: a divisor of M, at least P + interlaced (a stencil then spans at
+// most two tiles per axis), preferring 8
+int tile_side(int M, int P, int il) {
+    const int need = P + il;
+    if (need <= 8 && M % 8 == 0 && M >= 16)
+        return 8;
+    for (int t = need; t < M; t++)
+        if (M % t == 0 && M / t >= 2)
+            return t;
+    return M;
+}
+
+}  // namespace
+
+struct p3mk_psort {
+    int nmax, M, T, nt;
+    long ntiles;
+    int *key, *count, *start, *cursor, *perm;
+    double4 *sorted;
+    int n; // particles of the last sort
+};
+
+extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
+    if (!s)
+        return;
+    p3mk_release(s->key);
+    p3mk_release(s->count);
+    p3mk_release(s->start);
+    p3mk_release(s->cursor);
+    p3mk_release(s->perm);
+    p3mk_release(s->sorted);
+    delete s;
+}
+
+extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
+    *out = nullptr;
+    p3mk_psort *s = new (std::nothrow) p3mk_psort();
+    if (!s)
+        return p3mk_fail(P3MK_ENOMEM, "particle sort", 0);
+    s->nmax = n;
+    s->M = g->M;
+    s->T = tile_side(g->M, g->cao, g->interlaced ? 1 : 0);
+    s->nt = g->M / s->T;
+    s->ntiles = (long)s->nt * s->nt * s->nt;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    int rc = P3MK_OK;
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->key, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->count, sizeof(int) * (s->ntiles + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->start, sizeof(int) * (s->ntiles + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cursor, sizeof(int) * (s->ntiles + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->perm, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->sorted, sizeof(double4) * nn);
+    if (rc != P3MK_OK) {
+        p3mk_psort_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return P3MK_OK;
+}
+
+namespace {
+
 template <typename T, bool R2C>
-int launch_interp_ad(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
-                     const double *q, const void *phi, double prefac, const double *selfb,
-                     double *fk, double *f, p3mk_queue *qu) {
+int launch_assign(const p3mk_geom *g, p3mk_psort *s, void *mesh, p3mk_queue *qu) {
+    const double Hi = (double)g->M / g->L;
+    const int NM = g->interlaced ? 2 : 1;
+    const size_t lds = sizeof(T) * LdsAcc<T>::kArrays * NM * (size_t)s->T * s->T * s->T;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+    const dim3 grid((unsigned)s->ntiles);
+    T *m = (T *)mesh;
+    hipStream_t st = p3mk_stream(qu);
+    const bool il = g->interlaced != 0;
+    switch (g->cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        if (il)                                                                                    \
+            hipLaunchKernelGGL((k_assign_tiles<P, T, R2C, true>), grid, dim3(kThreads), lds, st,   \
+                               s->sorted, s->start, s->nt, s->T, g->M, Hi, m);                     \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_assign_tiles<P, T, R2C, false>), grid, dim3(kThreads), lds, st,  \
+                               s->sorted, s->start, s->nt, s->T, g->M, Hi, m);                     \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+    default:
+        return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
+    }
+    return p3mk_launch_check("assign");
+}
+
+template <typename T, bool R2C>
+int launch_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
+                     double prefac, double *fk, p3mk_queue *qu) {
+    const double Hi = (double)g->M / g->L;
+    dim3 grid((n + kThreads - 1) / kThreads);
+    const T *F0 = (const T *)F[0], *F1 = (const T *)F[1], *F2 = (const T *)F[2];
+    hipStream_t st = p3mk_stream(qu);
+    const bool il = g->interlaced != 0;
+    switch (g->cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        if (il)                                                                                    \
+            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n,    \
+                               s->sorted, s->perm, Hi, g->M, F0, F1, F2, prefac, fk);            \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, false>), grid, dim3(kThreads), 0, st, n,   \
+                               s->sorted, s->perm, Hi, g->M, F0, F1, F2, prefac, fk);            \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+    default:
+        return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
+    }
+    return p3mk_launch_check("interp_ik");
+}
+
+template <typename T, bool R2C>
+int launch_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void *phi,
+                     double prefac, const double *selfb, double *fk, double *f, p3mk_queue *qu) {
     const double Hi = (double)g->M / g->L;
     const double h = g->L / g->M;
     dim3 grid((n + kThreads - 1) / kThreads);
     const T *ph = (const T *)phi;
-    hipStream_t s = p3mk_stream(qu);
+    hipStream_t st = p3mk_stream(qu);
     const bool il = g->interlaced != 0;
     switch (g->cao) {
-#define CASE(P)                                                                                   \
-    case P:                                                                                       \
-        if (il)                                                                                   \
-            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, true>), grid, dim3(kThreads), 0, s, n, x, \
-                               y, z, q, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, f);      \
-        else                                                                                      \
-            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, false>), grid, dim3(kThreads), 0, s, n, x, \
-                               y, z, q, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, f);      \
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        if (il)                                                                                    \
+            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n,    \
+                               s->sorted, s->perm, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, \
+                               f);                                                                 \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, false>), grid, dim3(kThreads), 0, st, n,   \
+                               s->sorted, s->perm, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, \
+                               f);                                                                 \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -284,38 +501,58 @@ int launch_interp_ad(const p3mk_geom *g, int n, const double *x, const double *y
 }  // namespace
 
 extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const double *y,
-                           const double *z, const double *q, void *mesh, p3mk_queue *qu) {
-    if (n <= 0)
-        return P3MK_OK;
+                           const double *z, const double *q, p3mk_psort *s, void *mesh,
+                           p3mk_queue *qu) {
+    if (n > s->nmax || g->M != s->M)
+        return p3mk_fail(P3MK_EINVAL, "particle sort was made for another size", 0);
+    hipStream_t st = p3mk_stream(qu);
+    s->n = n;
+    if (n > 0) {
+        const double Hi = (double)g->M / g->L;
+        const int nb = (n + 255) / 256;
+        P3MK_TRY(p3mk_fill_zero(s->count, sizeof(int) * (s->ntiles + 1), qu));
+        P3MK_TRY(p3mk_fill_zero(s->cursor, sizeof(int) * (s->ntiles + 1), qu));
+        hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
+                           s->nt, s->key, s->count);
+        P3MK_TRY(p3mk_launch_check("tile count"));
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, s->count, (int)s->ntiles, s->start);
+        P3MK_TRY(p3mk_launch_check("tile scan"));
+        hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key, s->start,
+                           s->cursor, s->perm, s->sorted);
+        P3MK_TRY(p3mk_launch_check("tile scatter"));
+    } else {
+        P3MK_TRY(p3mk_fill_zero(s->start, sizeof(int) * (s->ntiles + 1), qu));
+    }
     if (g->fp32)
-        return g->r2c ? launch_assign<float, true>(g, n, x, y, z, q, mesh, qu)
-                      : launch_assign<float, false>(g, n, x, y, z, q, mesh, qu);
-    return g->r2c ? launch_assign<double, true>(g, n, x, y, z, q, mesh, qu)
-                  : launch_assign<double, false>(g, n, x, y, z, q, mesh, qu);
+        return g->r2c ? launch_assign<float, true>(g, s, mesh, qu) : launch_assign<float, false>(g, s, mesh, qu);
+    return g->r2c ? launch_assign<double, true>(g, s, mesh, qu) : launch_assign<double, false>(g, s, mesh, qu);
 }
 
-extern "C" int p3mk_interp_ik(const p3mk_geom *g, int n, const double *x, const double *y,
-                              const double *z, const double *q, void *const F[3], double prefac,
-                              double *fk, p3mk_queue *qu) {
+extern "C" int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
+                              double prefac, double *fk, p3mk_queue *qu) {
     if (n <= 0)
         return P3MK_OK;
+    if (n != s->n)
+        return p3mk_fail(P3MK_EINVAL, "interpolation needs the sort of the same particles", 0);
     if (g->fp32)
-        return g->r2c ? launch_interp_ik<float, true>(g, n, x, y, z, q, F, prefac, fk, qu)
-                      : launch_interp_ik<float, false>(g, n, x, y, z, q, F, prefac, fk, qu);
-    return g->r2c ? launch_interp_ik<double, true>(g, n, x, y, z, q, F, prefac, fk, qu)
-                  : launch_interp_ik<double, false>(g, n, x, y, z, q, F, prefac, fk, qu);
+        return g->r2c ? launch_interp_ik<float, true>(g, n, s, F, prefac, fk, qu)
+                      : launch_interp_ik<float, false>(g, n, s, F, prefac, fk, qu);
+    return g->r2c ? launch_interp_ik<double, true>(g, n, s, F, prefac, fk, qu)
+                  : launch_interp_ik<double, false>(g, n, s, F, prefac, fk, qu);
 }
 
-extern "C" int p3mk_interp_ad(const p3mk_geom *g, int n, const double *x, const double *y,
-                              const double *z, const double *q, const void *phi, double prefac,
-                              const double *selfb, double *fk, double *f, p3mk_queue *qu) {
+extern "C" int p3mk_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void *phi,
+                              double prefac, const double *selfb, double *fk, double *f,
+                              p3mk_queue *qu) {
     if (n <= 0)
         return P3MK_OK;
+    if (n != s->n)
+        return p3mk_fail(P3MK_EINVAL, "interpolation needs the sort of the same particles", 0);
     if (selfb != nullptr && f == nullptr)
         return p3mk_fail(P3MK_EINVAL, "self-force correction needs the total-force array", 0);
     if (g->fp32)
-        return g->r2c ? launch_interp_ad<float, true>(g, n, x, y, z, q, phi, prefac, selfb, fk, f, qu)
-                      : launch_interp_ad<float, false>(g, n, x, y, z, q, phi, prefac, selfb, fk, f, qu);
-    return g->r2c ? launch_interp_ad<double, true>(g, n, x, y, z, q, phi, prefac, selfb, fk, f, qu)
-                  : launch_interp_ad<double, false>(g, n, x, y, z, q, phi, prefac, selfb, fk, f, qu);
+        return g->r2c ? launch_interp_ad<float, true>(g, n, s, phi, prefac, selfb, fk, f, qu)
+                      : launch_interp_ad<float, false>(g, n, s, phi, prefac, selfb, fk, f, qu);
+    return g->r2c ? launch_interp_ad<double, true>(g, n, s, phi, prefac, selfb, fk, f, qu)
+                  : launch_interp_ad<double, false>(g, n, s, phi, prefac, selfb, fk, f, qu);
 }

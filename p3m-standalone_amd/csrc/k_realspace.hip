@@ -1,15 +1,23 @@
 // Real-space (short-range) Ewald force on a linked-cell list.
 //
 // Pair set and arithmetic of Realteil (realpart.c:109-152): every ordered
-// pair i != j whose minimum-image distance r (dx -= round(dx/L) L per axis)
-// satisfies r <= rcut contributes
+// pair i != j whose minimum-image distance r satisfies r <= rcut contributes
 //   f_i += q_i q_j (erfc(alpha r)/r + 2 alpha/sqrt(pi) exp(-alpha^2 r^2)) / r^2 * d
-// and 0.5 q_i q_j erfc(alpha r)/r to the energy.  The cell side is >= rcut,
-// so with rcut <= L/2 every such pair lies in one of the 27 neighbour cells.
+// and 0.5 q_i q_j erfc(alpha r)/r to the energy.
 //
-// Particles are counting-sorted by cell; one thread owns one particle (in
-// cell order, so neighbouring threads read the same neighbour cells) and
-// writes its force once, without atomics.
+// Layout: particles are counting-sorted into cells of side >= rcut/R (R = 2
+// when the box holds >= 5 cells per side), folded into [0, L).  One
+// wavefront owns one home cell; its 64 lanes take 64 home particles at a
+// time and walk the (2R+1)^3 neighbour cells in the same order, so neighbour
+// positions are wave-uniform loads.  Periodic images are handled by a
+// per-neighbour-cell shift of 0 or +-L, which equals the reference's
+// round(dx/L)*L for every pair inside the cutoff (|dx| <= rcut <= L/2), so
+// dx = (x_i - x_j) - shift is the reference's dx bit for bit.
+//
+// A cheap distance test accepts ~30% of the candidates; accepted neighbour
+// indices go to a per-lane queue in LDS, and the expensive erfc/exp part
+// runs when the queue is drained, so lanes stay busy instead of idling
+// through other lanes' accepted pairs.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -20,26 +28,32 @@
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kMaxCellsPerSide = 256;
+constexpr int kWave = 64;
+constexpr int kQueue = 32;
+constexpr int kMaxCellsPerSide = 512;
 
-__device__ __forceinline__ int cell_coord(double r, double L, int nc) {
-    int c = (int)floor(r / L * nc);
-    c %= nc;
-    if (c < 0)
-        c += nc;
-    return c;
+__device__ __forceinline__ double fold(double r, double L) {
+    // into [0, L); identity for positions already inside the box
+    if (r >= 0.0 && r < L)
+        return r;
+    double f = r - floor(r / L) * L;
+    return (f >= L || f < 0.0) ? 0.0 : f;
+}
+
+__device__ __forceinline__ int cell_of(double rf, double inv_side, int nc) {
+    int c = (int)(rf * inv_side);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
-                             const double *__restrict__ z, double L, int nc,
-                             int *__restrict__ cell_of, int *__restrict__ count) {
+                             const double *__restrict__ z, double L, double inv_side, int nc,
+                             int *__restrict__ cell_id, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int c = (cell_coord(x[i], L, nc) * nc + cell_coord(y[i], L, nc)) * nc +
-                  cell_coord(z[i], L, nc);
-    cell_of[i] = c;
+    const int c = (cell_of(fold(x[i], L), inv_side, nc) * nc + cell_of(fold(y[i], L), inv_side, nc)) * nc +
+                  cell_of(fold(z[i], L), inv_side, nc);
+    cell_id[i] = c;
     atomicAdd(&count[c], 1);
 }
 
@@ -71,100 +85,175 @@ __global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ count, in
 }
 
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, const double *__restrict__ q,
-                               const int *__restrict__ cell_of, const int *__restrict__ start,
+                               const double *__restrict__ z, const double *__restrict__ q, double L,
+                               const int *__restrict__ cell_id, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int c = cell_of[i];
+    const int c = cell_id[i];
     const int pos = start[c] + atomicAdd(&cursor[c], 1);
     perm[pos] = i;
-    sorted[pos] = make_double4(x[i], y[i], z[i], q[i]);
+    sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, __shfl_xor(v, off, kWave));
+    return v;
 }
 
 // 2 alpha / sqrt(pi) with the reference's sqrt(pi) literal (realpart.c:120)
-__device__ __forceinline__ double wupi() { return 1.77245385090551602729816748334; }
+__device__ __forceinline__ double sqrt_pi() { return 1.77245385090551602729816748334; }
 
+struct PairAcc {
+    double f0, f1, f2, en;
+};
+
+// Full pair evaluation with the reference's inclusion test r <= rcut.
 template <bool FP32>
-__global__ void __launch_bounds__(kThreads)
-k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
-            const int *__restrict__ start, int nc, double L, double rcut, double alpha,
-            double *__restrict__ fr, double *__restrict__ energy) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0, en = 0.0;
-    if (t < n) {
-        const double4 pi = sorted[t];
-        const double Li = 1.0 / L;
-        const double two_a_spi = 2.0 * alpha / wupi();
-        const int cx = cell_coord(pi.x, L, nc), cy = cell_coord(pi.y, L, nc),
-                  cz = cell_coord(pi.z, L, nc);
-        const int span = (nc >= 3) ? 1 : 0; // nc < 3: single cell (nc == 1) or 2 cells per side
-        const int lo = (nc >= 3) ? -1 : 0, hi = (nc >= 3) ? 1 : nc - 1;
-        for (int a = lo; a <= hi; a++) {
-            const int ca = span ? (cx + a + nc) % nc : a;
-            for (int b = lo; b <= hi; b++) {
-                const int cb = span ? (cy + b + nc) % nc : b;
-                for (int c = lo; c <= hi; c++) {
-                    const int cc = span ? (cz + c + nc) % nc : c;
-                    const int cell = (ca * nc + cb) * nc + cc;
-                    const int jb = start[cell], je = start[cell + 1];
-                    for (int j = jb; j < je; j++) {
-                        if (j == t)
-                            continue;
-                        const double4 pj = sorted[j];
-                        double dx, dy, dz, r2;
-                        {
+__device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, double sx, double sy,
+                                     double sz, double rcut, double alpha, double two_a_spi,
+                                     PairAcc &acc) {
+    double dx, dy, dz, r2;
+    {
 #pragma clang fp contract(off)
-                            dx = pi.x - pj.x;
-                            dx -= round(dx * Li) * L;
-                            dy = pi.y - pj.y;
-                            dy -= round(dy * Li) * L;
-                            dz = pi.z - pj.z;
-                            dz -= round(dz * Li) * L;
-                            r2 = dx * dx + dy * dy + dz * dz;
+        dx = (pi.x - pj.x) - sx;
+        dy = (pi.y - pj.y) - sy;
+        dz = (pi.z - pj.z) - sz;
+        r2 = dx * dx + dy * dy + dz * dz;
+    }
+    if (FP32) {
+        if (r2 <= rcut * rcut) {
+            const float rf = sqrtf((float)r2);
+            const float ar = (float)alpha * rf;
+            const float ec = erfcf(ar);
+            const float fak =
+                (float)(pi.w * pj.w) * (ec / rf + (float)two_a_spi * expf(-ar * ar)) / (float)r2;
+            acc.f0 += (double)(fak * (float)dx);
+            acc.f1 += (double)(fak * (float)dy);
+            acc.f2 += (double)(fak * (float)dz);
+            acc.en += 0.5 * pi.w * pj.w * (double)(ec / rf);
+        }
+    } else {
+        const double r = sqrt(r2);
+        if (r <= rcut) {
+            const double ar = alpha * r;
+            const double ec = erfc(ar);
+            const double fak = pi.w * pj.w * (ec / r + two_a_spi * exp(-ar * ar)) / (r * r);
+            acc.f0 += fak * dx;
+            acc.f1 += fak * dy;
+            acc.f2 += fak * dz;
+            acc.en += 0.5 * pi.w * pj.w * ec / r;
+        }
+    }
+}
+
+// One wavefront per home cell.  Lane queue entries store the neighbour's
+// sorted index and the index of its periodic shift (0..26 for -1/0/+1 per axis).
+template <bool FP32>
+__global__ void __launch_bounds__(kWave)
+k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
+            const int *__restrict__ start, int nc, int reach, double L, double rcut, double alpha,
+            double *__restrict__ fr, double *__restrict__ energy) {
+    __shared__ int qj[kQueue][kWave];
+    __shared__ unsigned char qs[kQueue][kWave];
+    const int lane = threadIdx.x;
+    const int home = blockIdx.x;
+    const int hb = start[home], he = start[home + 1];
+    if (hb == he)
+        return;
+    const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
+    const double two_a_spi = 2.0 * alpha / sqrt_pi();
+    // prefilter threshold: slightly above rcut^2, the exact test runs in pair()
+    const double rc2 = rcut * rcut * (1.0 + 1e-12);
+    const bool all_cells = (2 * reach + 1 > nc);
+    const int lo = all_cells ? 0 : -reach, hi = all_cells ? nc - 1 : reach;
+    double en_tot = 0.0;
+
+    for (int base = hb; base < he; base += kWave) {
+        const int i = base + lane;
+        const bool active = i < he;
+        const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        PairAcc acc = {0.0, 0.0, 0.0, 0.0};
+        int tail = 0;
+
+        auto drain = [&]() {
+            const int m = wave_max(tail);
+            for (int k = 0; k < m; k++) {
+                if (k < tail) {
+                    const int j = qj[k][lane];
+                    const int s = qs[k][lane];
+                    const double sx = (double)(s / 9 - 1) * L, sy = (double)((s / 3) % 3 - 1) * L,
+                                 sz = (double)(s % 3 - 1) * L;
+                    pair<FP32>(pi, sorted[j], sx, sy, sz, rcut, alpha, two_a_spi, acc);
+                }
+            }
+            tail = 0;
+        };
+
+        for (int a = lo; a <= hi; a++) {
+            int cx = all_cells ? a : hx + a, wx = 0;
+            if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
+            for (int b = lo; b <= hi; b++) {
+                int cy = all_cells ? b : hy + b, wy = 0;
+                if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
+                for (int c = lo; c <= hi; c++) {
+                    int cz = all_cells ? c : hz + c, wz = 0;
+                    if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
+                    const int cell = (cx * nc + cy) * nc + cz;
+                    const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
+                    const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+                    int sidx, sxw = wx, syw = wy, szw = wz;
+                    double sx, sy, sz;
+                    if (all_cells) {
+                        // small box: the image is chosen per pair below
+                        sxw = syw = szw = 0;
+                    }
+                    sidx = (sxw + 1) * 9 + (syw + 1) * 3 + (szw + 1);
+                    sx = sxw * L;
+                    sy = syw * L;
+                    sz = szw * L;
+                    for (int j = jb; j < je; j++) {
+                        const double4 pj = sorted[j];
+                        double dx = (pi.x - pj.x) - sx, dy = (pi.y - pj.y) - sy,
+                               dz = (pi.z - pj.z) - sz;
+                        int s = sidx;
+                        if (all_cells) {
+                            // nearest image per axis (the reference's round(dx/L))
+                            const double rx = round(dx / L), ry = round(dy / L), rz = round(dz / L);
+                            dx -= rx * L;
+                            dy -= ry * L;
+                            dz -= rz * L;
+                            s = ((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1);
                         }
-                        if (FP32) {
-                            if (r2 <= rcut * rcut) {
-                                const float rf = sqrtf((float)r2);
-                                const float ar = (float)alpha * rf;
-                                const float ec = erfcf(ar);
-                                const float fak = (float)(pi.w * pj.w) *
-                                                  (ec / rf + (float)two_a_spi * expf(-ar * ar)) /
-                                                  (float)r2;
-                                f0 += (double)(fak * (float)dx);
-                                f1 += (double)(fak * (float)dy);
-                                f2 += (double)(fak * (float)dz);
-                                en += 0.5 * pi.w * pj.w * (double)(ec / rf);
-                            }
-                        } else {
-                            const double r = sqrt(r2);
-                            if (r <= rcut) {
-                                const double ar = alpha * r;
-                                const double ec = erfc(ar);
-                                const double fak =
-                                    pi.w * pj.w * (ec / r + two_a_spi * exp(-ar * ar)) / (r * r);
-                                f0 += fak * dx;
-                                f1 += fak * dy;
-                                f2 += fak * dz;
-                                en += 0.5 * pi.w * pj.w * ec / r;
-                            }
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (active && j != i && r2 <= rc2) {
+                            qj[tail][lane] = j;
+                            qs[tail][lane] = (unsigned char)s;
+                            tail++;
                         }
+                        if (__ballot(tail == kQueue))
+                            drain();
                     }
                 }
             }
         }
-        const int i = perm[t];
-        fr[i] += f0;
-        fr[(long)n + i] += f1;
-        fr[2L * n + i] += f2;
+        drain();
+        if (active) {
+            const int p = perm[i];
+            fr[p] += acc.f0;
+            fr[(long)n + p] += acc.f1;
+            fr[2L * n + p] += acc.f2;
+        }
+        en_tot += acc.en;
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
-            en += __shfl_down(en, off, 64);
-        if ((threadIdx.x & 63) == 0 && en != 0.0)
-            atomicAdd(energy, en);
+            en_tot += __shfl_down(en_tot, off, kWave);
+        if (lane == 0 && en_tot != 0.0)
+            atomicAdd(energy, en_tot);
     }
 }
 
@@ -172,17 +261,17 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
 
 struct p3mk_cells {
     int nmax;
-    int nc;
+    int nc, reach;
     long ncell;
-    double L, rcut;
-    int *cell_of, *count, *start, *cursor, *perm;
+    double L, rcut, inv_side;
+    int *cell_id, *count, *start, *cursor, *perm;
     double4 *sorted;
 };
 
 extern "C" void p3mk_cells_destroy(p3mk_cells *c) {
     if (!c)
         return;
-    p3mk_release(c->cell_of);
+    p3mk_release(c->cell_id);
     p3mk_release(c->count);
     p3mk_release(c->start);
     p3mk_release(c->cursor);
@@ -199,18 +288,27 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     if (!c)
         return p3mk_fail(P3MK_ENOMEM, "cell list", 0);
     c->nmax = n;
-    int nc = (int)std::floor(L / rcut);
+    // cells of side >= rcut/reach (with a margin, so that rounding of
+    // x*nc/L never puts a pair inside the cutoff more than `reach` cells apart)
+    int reach = 2;
+    int nc = (int)std::floor(L * reach / rcut * (1.0 - 1e-12));
+    if (nc < 2 * reach + 1) {
+        reach = 1;
+        nc = (int)std::floor(L / rcut * (1.0 - 1e-12));
+    }
     if (nc < 1)
         nc = 1;
     if (nc > kMaxCellsPerSide)
         nc = kMaxCellsPerSide;
     c->nc = nc;
+    c->reach = reach;
     c->ncell = (long)nc * nc * nc;
     c->L = L;
     c->rcut = rcut;
+    c->inv_side = (double)nc / L;
     int rc = P3MK_OK;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cell_of, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cell_id, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->count, sizeof(int) * (c->ncell + 1));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->start, sizeof(int) * (c->ncell + 1));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cursor, sizeof(int) * (c->ncell + 1));
@@ -232,22 +330,23 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     if (n > c->nmax || L != c->L || rcut != c->rcut)
         return p3mk_fail(P3MK_EINVAL, "cell list was built for another system size, box or cutoff", 0);
     hipStream_t s = p3mk_stream(qu);
-    const int nb = (n + kThreads - 1) / kThreads;
+    const int nb = (n + 255) / 256;
     P3MK_TRY(p3mk_fill_zero(c->count, sizeof(int) * (c->ncell + 1), qu));
     P3MK_TRY(p3mk_fill_zero(c->cursor, sizeof(int) * (c->ncell + 1), qu));
-    hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(kThreads), 0, s, n, x, y, z, L, c->nc,
-                       c->cell_of, c->count);
+    hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, s, n, x, y, z, L, c->inv_side, c->nc,
+                       c->cell_id, c->count);
     P3MK_TRY(p3mk_launch_check("cell count"));
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->count, (int)c->ncell, c->start);
     P3MK_TRY(p3mk_launch_check("cell scan"));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(kThreads), 0, s, n, x, y, z, q, c->cell_of,
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, c->cell_id,
                        c->start, c->cursor, c->perm, c->sorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
+    const dim3 grid((unsigned)c->ncell);
     if (fp32)
-        hipLaunchKernelGGL(k_realspace<true>, dim3(nb), dim3(kThreads), 0, s, n, c->sorted, c->perm,
-                           c->start, c->nc, L, rcut, alpha, fr, energy);
+        hipLaunchKernelGGL(k_realspace<true>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
+                           c->start, c->nc, c->reach, L, rcut, alpha, fr, energy);
     else
-        hipLaunchKernelGGL(k_realspace<false>, dim3(nb), dim3(kThreads), 0, s, n, c->sorted,
-                           c->perm, c->start, c->nc, L, rcut, alpha, fr, energy);
+        hipLaunchKernelGGL(k_realspace<false>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
+                           c->start, c->nc, c->reach, L, rcut, alpha, fr, energy);
     return p3mk_launch_check("realspace");
 }

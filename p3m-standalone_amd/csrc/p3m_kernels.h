@@ -86,18 +86,22 @@ int p3mk_fft_backward_yz(p3mk_fft *f, void *spec, void *mesh, p3mk_queue *q);
 int p3mk_fft3d(p3mk_fft *f, void *data, int sign, p3mk_queue *q);
 
 /* ---- particle <-> mesh -------------------------------------------------- */
-/* mesh += charge assignment of n particles (both meshes if interlaced) */
+/* Particles sorted by the mesh tile of their stencil base: made by
+ * p3mk_assign, reused by the interpolation of the same particles. */
+typedef struct p3mk_psort p3mk_psort;
+int p3mk_psort_create(p3mk_psort **s, const p3mk_geom *g, int n);
+void p3mk_psort_destroy(p3mk_psort *s);
+/* mesh = charge assignment of n particles (both meshes if interlaced);
+ * every mesh point is written, no prior zeroing needed */
 int p3mk_assign(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
-                const double *q, void *mesh, p3mk_queue *qu);
+                const double *q, p3mk_psort *s, void *mesh, p3mk_queue *qu);
 /* ik back-interpolation: fk[d*n + i] += -prefac * sum W F_d (meshes F[0..2]) */
-int p3mk_interp_ik(const p3mk_geom *g, int n, const double *x, const double *y,
-                   const double *z, const double *q, void *const F[3], double prefac,
-                   double *fk, p3mk_queue *qu);
+int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
+                   double prefac, double *fk, p3mk_queue *qu);
 /* ad back-interpolation: fk -= prefac * sum phi dQ; with selfb (81 FP64 on
  * the device) also f -= self force */
-int p3mk_interp_ad(const p3mk_geom *g, int n, const double *x, const double *y,
-                   const double *z, const double *q, const void *phi, double prefac,
-                   const double *selfb, double *fk, double *f, p3mk_queue *qu);
+int p3mk_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void *phi,
+                   double prefac, const double *selfb, double *fk, double *f, p3mk_queue *qu);
 
 /* ---- influence function, self force, estimates --------------------------- */
 /* G (M^3 FP64, r_ind order) of method id 0..3, 6, 7 at alpha; mc aliasing

@@ -31,9 +31,18 @@ def pkg():
     return entry.load_package()
 
 
-def random_system(n, L, seed, charges="alternating"):
+def random_system(n, L, seed, charges="alternating", layout="uniform"):
     rng = np.random.default_rng(seed)
-    x, y, z = (rng.uniform(0.0, L, n) for _ in range(3))
+    if layout == "gaussian":
+        def coord():
+            out = np.empty(0)
+            while out.size < n:
+                v = rng.normal(L / 2, L / 8, 2 * n)
+                out = np.concatenate([out, v[(v >= 0.0) & (v < L)]])
+            return out[:n]
+        x, y, z = coord(), coord(), coord()
+    else:
+        x, y, z = (rng.uniform(0.0, L, n) for _ in range(3))
     if charges == "alternating":
         q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     else:

@@ -55,6 +55,19 @@ def test_kspace_force_fp64(pkg, orc, method, n, L, M, cao, alpha):
     solver.close()
 
 
+@pytest.mark.parametrize("method,M,cao", [(3, 32, 7), (0, 32, 5), (7, 24, 6), (1, 28, 7)])
+def test_kspace_force_clustered(pkg, orc, method, M, cao):
+    """Gaussian-clustered charges: dense mesh tiles, many particles per tile."""
+    s = random_system(20000, 20.0, seed=M + cao, layout="gaussian")
+    solver = pkg.Solver(method, pkg.HostSystem.from_dict(s), pkg.parameters(M, cao, 1.0))
+    forces = solver.kspace_force()
+    fk_ref, f_ref = orc.kspace_force(method, s, M, cao, 1.0)
+    assert rel_err(forces.kspace(), fk_ref) < FP64_TOL
+    if method in (2, 7):
+        assert rel_err(forces.total(), f_ref) < 1e-11
+    solver.close()
+
+
 @pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
 def test_influence_function(pkg, orc, method):
     s = random_system(50, 9.0, seed=1)
@@ -98,6 +111,16 @@ def test_realteil(pkg, orc, n, L, rcut, alpha):
     system = pkg.HostSystem.from_dict(s)
     forces = pkg.realteil(system, rcut, alpha)
     fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    assert rel_err(forces.real(), fr_ref) < 1e-13
+    assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+def test_realteil_clustered(pkg, orc):
+    """Dense home cells (several 64-particle chunks, full neighbour queues)."""
+    s = random_system(4000, 12.0, seed=9, layout="gaussian")
+    system = pkg.HostSystem.from_dict(s)
+    forces = pkg.realteil(system, 3.0, 1.1)
+    fr_ref, e_ref = orc.realteil(s, 3.0, 1.1)
     assert rel_err(forces.real(), fr_ref) < 1e-13
     assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
 
