@@ -15,3 +15,8 @@ hipStream_t p3mk_stream(p3mk_queue *q);
         if (rc_ != P3MK_OK)           \
             return rc_;               \
     } while (0)
+
+// Exclusive prefix sum: start[0..m] (start[m] = total) of count[0..m);
+// scratch holds p3mk_scan_scratch_ints(m) ints.
+size_t p3mk_scan_scratch_ints(long m);
+int p3mk_exclusive_scan(const int *count, long m, int *start, int *scratch, p3mk_queue *q);

@@ -6,14 +6,16 @@
 // the imaginary part).  Weights are evaluated at the reference's quantised
 // offsets (device_common.hpp) instead of read from 7 MB tables.
 //
-// Assignment is owner-computes: particles are counting-sorted by the mesh
-// tile (T^3 points, T | M, T >= P + interlaced) of their mesh-0 stencil
-// base; one workgroup owns one tile, walks the particles of its own and its
-// lower neighbour tiles (the only ones whose stencils reach it), accumulates
-// the in-tile part of each stencil in LDS and writes the tile once with
-// plain stores.  No HBM atomics, no zero fill of the mesh, and clustered
-// particle distributions only cost LDS atomics inside the owning workgroup.
-//
+// Assignment: particles are counting-sorted by the mesh tile (T^3 points) of
+// their mesh-0 stencil base.  A work item is a chunk of at most kChunk
+// particles of one tile; its workgroup accumulates the full P^3 stencils of
+// its particles in an LDS copy of the tile plus halo ((T+P)^3 points per
+// mesh, every stencil fits, so the P^3 loop has no clipping and no
+// divergence) and then adds the non-zero LDS points to the mesh with global
+// atomics.  Dense (clustered) tiles are split over many workgroups; the
+// global atomics are one per LDS point, not one per particle and stencil
+// point.
+
 // Back-interpolation (charge-assign.c:601-643, 646-688, 727-769, 849-885,
 // 1352-1391, 1468-1502) walks the same sorted order (neighbouring threads
 // gather from neighbouring mesh points) and recomputes the weights instead
@@ -32,6 +34,8 @@ namespace {
 constexpr int kThreads = 256;
 
 // ---- sort by mesh tile ------------------------------------------------------
+
+constexpr int kChunk = 1024;
 
 __device__ __forceinline__ int axis_base(double r, double Hi, int P, int M) {
     int b;
@@ -53,32 +57,6 @@ __global__ void k_tile_count(int n, const double *__restrict__ x, const double *
     atomicAdd(&count[k], 1);
 }
 
-__global__ void __launch_bounds__(1024) k_scan_tiles(const int *__restrict__ count, int m,
-                                                     int *__restrict__ start) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int chunk = (m + 1023) / 1024;
-    const int b = t * chunk, e = min(m, b + chunk);
-    int s = 0;
-    for (int k = b; k < e; k++)
-        s += count[k];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = (t == 0) ? 0 : part[t - 1];
-    for (int k = b; k < e; k++) {
-        start[k] = run;
-        run += count[k];
-    }
-    if (t == 1023)
-        start[m] = part[1023];
-}
-
 __global__ void k_tile_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q,
                                const int *__restrict__ key, const int *__restrict__ start,
@@ -93,125 +71,122 @@ __global__ void k_tile_scatter(int n, const double *__restrict__ x, const double
     sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
 }
 
-// ---- owner-computes assignment -----------------------------------------------
+// chunks per tile: ceil(count / kChunk)
+__global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *__restrict__ chunks) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles)
+        chunks[t] = (count[t] + kChunk - 1) / kChunk;
+}
+
+// ---- halo-tile assignment ------------------------------------------------------
 
 // LDS accumulators: FP64 atomics, or FP32 atomics with the TwoSum rounding
 // error of every add collected in a second FP32 array (compensated sum).
 template <typename T> struct LdsAcc;
 template <> struct LdsAcc<double> {
     static constexpr int kArrays = 1;
-    __device__ static __forceinline__ void add(double *a, int npts, int idx, double v) {
-        (void)npts;
+    __device__ static __forceinline__ void add(double *a, int nall, int idx, double v) {
+        (void)nall;
         atomicAdd(&a[idx], v);
     }
-    __device__ static __forceinline__ double value(const double *a, int npts, int idx) {
-        (void)npts;
+    __device__ static __forceinline__ double value(const double *a, int nall, int idx) {
+        (void)nall;
         return a[idx];
     }
 };
 template <> struct LdsAcc<float> {
     static constexpr int kArrays = 2;
-    __device__ static __forceinline__ void add(float *a, int npts, int idx, float v) {
+    __device__ static __forceinline__ void add(float *a, int nall, int idx, float v) {
         const float old = atomicAdd(&a[idx], v);
         float s, e;
         two_sum(old, v, s, e);
         if (e != 0.0f)
-            atomicAdd(&a[npts + idx], e);
+            atomicAdd(&a[nall + idx], e);
     }
-    __device__ static __forceinline__ float value(const float *a, int npts, int idx) {
-        return a[idx] + a[npts + idx];
+    __device__ static __forceinline__ float value(const float *a, int nall, int idx) {
+        return a[idx] + a[nall + idx];
     }
 };
 
-// In-tile part of one axis of a stencil: a in [lo, hi) maps to local
-// points off + a (the stencil wraps at most once; the tile is contiguous).
-template <int P>
-__device__ __forceinline__ void clip_axis(int base, int t0, int T, int M, int &lo, int &hi,
-                                          int &off) {
-    lo = P;
-    hi = 0;
-    off = 0;
-#pragma unroll
-    for (int a = 0; a < P; a++) {
-        int m = base + a;
-        if (m >= M)
-            m -= M;
-        const int l = m - t0;
-        if (l >= 0 && l < T) {
-            if (a < lo) {
-                lo = a;
-                off = l - a;
-            }
-            hi = a + 1;
-        }
-    }
-}
-
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
-k_assign_tiles(const double4 *__restrict__ sorted, const int *__restrict__ tstart, int nt, int Tt,
-               int M, double Hi, T *__restrict__ mesh) {
+k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
+              const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
+              T *__restrict__ mesh) {
     extern __shared__ unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     constexpr int NM = IL ? 2 : 1;
-    const int npts = Tt * Tt * Tt;
-    const int nall = NM * npts; // points of all meshes
+    const int R = Tt + P - 1 + (IL ? 1 : 0); // halo region side
+    const int npts = R * R * R;
+    const int nall = NM * npts;
+    // work item -> (tile, chunk): binary search in the chunk prefix sums
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
+        return;
+    long lo = 0, hi = ntiles;
+    while (hi - lo > 1) {
+        const long mid = (lo + hi) / 2;
+        if (cstart[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const long tile = lo;
+    const int chunk = item - cstart[tile];
+    const int pb = tstart[tile] + chunk * kChunk;
+    const int pe = min(tstart[tile + 1], pb + kChunk);
+    const int tz = (int)(tile % nt), ty = (int)((tile / nt) % nt), tx = (int)(tile / ((long)nt * nt));
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+
     for (int k = threadIdx.x; k < LdsAcc<T>::kArrays * nall; k += blockDim.x)
         acc[k] = T(0);
     __syncthreads();
-
-    const int tile = blockIdx.x;
-    const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
-    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
-    const int ds = (nt > 1) ? 1 : 0; // source tiles t and t-1 per axis
-    for (int sx = 0; sx <= ds; sx++)
-        for (int sy = 0; sy <= ds; sy++)
-            for (int sz = 0; sz <= ds; sz++) {
-                const int ux = (tx - sx + nt) % nt, uy = (ty - sy + nt) % nt, uz = (tz - sz + nt) % nt;
-                const int src = (ux * nt + uy) * nt + uz;
-                const int pb = tstart[src], pe = tstart[src + 1];
-                for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-                    const double4 r = sorted[p];
+    for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+        const double4 r = sorted[p];
+        const T qv = (T)r.w;
 #pragma unroll
-                    for (int ii = 0; ii < NM; ii++) {
-                        int b0, b1, b2;
-                        T w0[P], w1[P], w2[P];
-                        const double half = 0.5 * ii;
-                        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
-                        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
-                        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
-                        int l0, h0, o0, l1, h1, o1, l2, h2, o2;
-                        clip_axis<P>(b0, t0x, Tt, M, l0, h0, o0);
-                        clip_axis<P>(b1, t0y, Tt, M, l1, h1, o1);
-                        clip_axis<P>(b2, t0z, Tt, M, l2, h2, o2);
-                        if (h0 <= l0 || h1 <= l1 || h2 <= l2)
-                            continue;
-                        const T qv = (T)r.w;
-                        T *a = acc + ii * npts;
-                        for (int a0 = l0; a0 < h0; a0++) {
-                            const T t0 = qv * w0[a0];
-                            for (int a1 = l1; a1 < h1; a1++) {
-                                const T t1 = t0 * w1[a1];
-                                const int row = ((a0 + o0) * Tt + (a1 + o1)) * Tt + o2;
-                                for (int a2 = l2; a2 < h2; a2++)
-                                    LdsAcc<T>::add(a, nall, row + a2, t1 * w2[a2]);
-                            }
-                        }
-                    }
+        for (int ii = 0; ii < NM; ii++) {
+            int b0, b1, b2;
+            T w0[P], w1[P], w2[P];
+            const double half = 0.5 * ii;
+            axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+            axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+            axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
+            // local offsets inside the halo region (base is within T + 1 of
+            // the tile start, modulo the periodic wrap)
+            int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+            o0 += (o0 < 0) ? M : 0;
+            o1 += (o1 < 0) ? M : 0;
+            o2 += (o2 < 0) ? M : 0;
+            T *a = acc + ii * npts;
+            for (int a0 = 0; a0 < P; a0++) {
+                const T t0 = qv * w0[a0];
+#pragma unroll
+                for (int a1 = 0; a1 < P; a1++) {
+                    const T t1 = t0 * w1[a1];
+                    const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
+#pragma unroll
+                    for (int a2 = 0; a2 < P; a2++)
+                        LdsAcc<T>::add(a, nall, row + a2, t1 * w2[a2]);
                 }
             }
+        }
+    }
     __syncthreads();
-    // write the tile: every mesh point has exactly one owner
+    // add the region to the mesh (periodic wrap of the halo)
     for (int k = threadIdx.x; k < npts; k += blockDim.x) {
-        const int lz = k % Tt, ly = (k / Tt) % Tt, lx = k / (Tt * Tt);
-        const long g = ((long)(t0x + lx) * M + (t0y + ly)) * M + (t0z + lz);
-        if (R2C) {
-            mesh[g] = LdsAcc<T>::value(acc, nall, k);
-        } else {
-            c2_t<T> v;
-            v.x = LdsAcc<T>::value(acc, nall, k);
-            v.y = IL ? LdsAcc<T>::value(acc, nall, npts + k) : T(0);
-            reinterpret_cast<c2_t<T> *>(mesh)[g] = v;
+        const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
+        const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
+        const long g = ((long)gx * M + gy) * M + gz;
+#pragma unroll
+        for (int ii = 0; ii < NM; ii++) {
+            const T v = LdsAcc<T>::value(acc, nall, ii * npts + k);
+            if (v != T(0)) {
+                if (R2C)
+                    atomicAdd(&mesh[g], v);
+                else
+                    atomicAdd(&mesh[2 * g + ii], v);
+            }
         }
     }
 }
@@ -352,24 +327,15 @@ k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
     }
 }
 
-// tile side: a divisor of M, at least P + interlaced (a stencil then spans at
-// most two tiles per axis), preferring 8
-int tile_side(int M, int P, int il) {
-    const int need = P + il;
-    if (need <= 8 && M % 8 == 0 && M >= 16)
-        return 8;
-    for (int t = need; t < M; t++)
-        if (M % t == 0 && M / t >= 2)
-            return t;
-    return M;
-}
+// tile side: 8 mesh points (LDS region (8 + P)^3 per mesh)
+int tile_side(int M) { return M < 8 ? M : 8; }
 
 }  // namespace
 
 struct p3mk_psort {
     int nmax, M, T, nt;
     long ntiles;
-    int *key, *count, *start, *cursor, *perm;
+    int *key, *count, *start, *cursor, *perm, *chunks, *cstart, *scan_scratch;
     double4 *sorted;
     int n; // particles of the last sort
 };
@@ -382,6 +348,9 @@ extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
     p3mk_release(s->start);
     p3mk_release(s->cursor);
     p3mk_release(s->perm);
+    p3mk_release(s->chunks);
+    p3mk_release(s->cstart);
+    p3mk_release(s->scan_scratch);
     p3mk_release(s->sorted);
     delete s;
 }
@@ -393,15 +362,20 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
         return p3mk_fail(P3MK_ENOMEM, "particle sort", 0);
     s->nmax = n;
     s->M = g->M;
-    s->T = tile_side(g->M, g->cao, g->interlaced ? 1 : 0);
-    s->nt = g->M / s->T;
+    s->T = tile_side(g->M);
+    s->nt = (g->M + s->T - 1) / s->T;
     s->ntiles = (long)s->nt * s->nt * s->nt;
     const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t nt1 = (size_t)s->ntiles + 1;
     int rc = P3MK_OK;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->key, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->count, sizeof(int) * (s->ntiles + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->start, sizeof(int) * (s->ntiles + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cursor, sizeof(int) * (s->ntiles + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->count, sizeof(int) * nt1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->start, sizeof(int) * nt1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cursor, sizeof(int) * nt1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->chunks, sizeof(int) * nt1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cstart, sizeof(int) * nt1);
+    if (rc == P3MK_OK)
+        rc = p3mk_alloc((void **)&s->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(s->ntiles));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->perm, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->sorted, sizeof(double4) * nn);
     if (rc != P3MK_OK) {
@@ -415,13 +389,16 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
 namespace {
 
 template <typename T, bool R2C>
-int launch_assign(const p3mk_geom *g, p3mk_psort *s, void *mesh, p3mk_queue *qu) {
+int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, p3mk_queue *qu) {
     const double Hi = (double)g->M / g->L;
     const int NM = g->interlaced ? 2 : 1;
-    const size_t lds = sizeof(T) * LdsAcc<T>::kArrays * NM * (size_t)s->T * s->T * s->T;
+    const int R = s->T + g->cao - 1 + (g->interlaced ? 1 : 0);
+    const size_t lds = sizeof(T) * LdsAcc<T>::kArrays * NM * (size_t)R * R * R;
     if (lds > 64 * 1024)
         return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
-    const dim3 grid((unsigned)s->ntiles);
+    // upper bound of the number of work items; surplus workgroups exit
+    const long items = (long)(n + kChunk - 1) / kChunk + s->ntiles;
+    const dim3 grid((unsigned)items);
     T *m = (T *)mesh;
     hipStream_t st = p3mk_stream(qu);
     const bool il = g->interlaced != 0;
@@ -429,11 +406,13 @@ int launch_assign(const p3mk_geom *g, p3mk_psort *s, void *mesh, p3mk_queue *qu)
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_assign_tiles<P, T, R2C, true>), grid, dim3(kThreads), lds, st,   \
-                               s->sorted, s->start, s->nt, s->T, g->M, Hi, m);                     \
+            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, true>), grid, dim3(kThreads), lds, st,    \
+                               s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M, Hi,   \
+                               m);                                                                 \
         else                                                                                       \
-            hipLaunchKernelGGL((k_assign_tiles<P, T, R2C, false>), grid, dim3(kThreads), lds, st,  \
-                               s->sorted, s->start, s->nt, s->T, g->M, Hi, m);                     \
+            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, false>), grid, dim3(kThreads), lds, st,   \
+                               s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M, Hi,   \
+                               m);                                                                 \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -507,25 +486,28 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
         return p3mk_fail(P3MK_EINVAL, "particle sort was made for another size", 0);
     hipStream_t st = p3mk_stream(qu);
     s->n = n;
-    if (n > 0) {
-        const double Hi = (double)g->M / g->L;
-        const int nb = (n + 255) / 256;
-        P3MK_TRY(p3mk_fill_zero(s->count, sizeof(int) * (s->ntiles + 1), qu));
-        P3MK_TRY(p3mk_fill_zero(s->cursor, sizeof(int) * (s->ntiles + 1), qu));
-        hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
-                           s->nt, s->key, s->count);
-        P3MK_TRY(p3mk_launch_check("tile count"));
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, s->count, (int)s->ntiles, s->start);
-        P3MK_TRY(p3mk_launch_check("tile scan"));
-        hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key, s->start,
-                           s->cursor, s->perm, s->sorted);
-        P3MK_TRY(p3mk_launch_check("tile scatter"));
-    } else {
-        P3MK_TRY(p3mk_fill_zero(s->start, sizeof(int) * (s->ntiles + 1), qu));
-    }
+    P3MK_TRY(p3mk_fill_zero(mesh, p3mk_mesh_bytes(g), qu));
+    if (n <= 0)
+        return P3MK_OK;
+    const double Hi = (double)g->M / g->L;
+    const int nb = (n + 255) / 256;
+    const size_t nt1 = (size_t)s->ntiles + 1;
+    P3MK_TRY(p3mk_fill_zero(s->count, sizeof(int) * nt1, qu));
+    P3MK_TRY(p3mk_fill_zero(s->cursor, sizeof(int) * nt1, qu));
+    hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
+                       s->nt, s->key, s->count);
+    P3MK_TRY(p3mk_launch_check("tile count"));
+    P3MK_TRY(p3mk_exclusive_scan(s->count, s->ntiles, s->start, s->scan_scratch, qu));
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key, s->start,
+                       s->cursor, s->perm, s->sorted);
+    P3MK_TRY(p3mk_launch_check("tile scatter"));
+    hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
+                       s->count, s->ntiles, s->chunks);
+    P3MK_TRY(p3mk_launch_check("tile chunks"));
+    P3MK_TRY(p3mk_exclusive_scan(s->chunks, s->ntiles, s->cstart, s->scan_scratch, qu));
     if (g->fp32)
-        return g->r2c ? launch_assign<float, true>(g, s, mesh, qu) : launch_assign<float, false>(g, s, mesh, qu);
-    return g->r2c ? launch_assign<double, true>(g, s, mesh, qu) : launch_assign<double, false>(g, s, mesh, qu);
+        return g->r2c ? launch_assign<float, true>(g, n, s, mesh, qu) : launch_assign<float, false>(g, n, s, mesh, qu);
+    return g->r2c ? launch_assign<double, true>(g, n, s, mesh, qu) : launch_assign<double, false>(g, n, s, mesh, qu);
 }
 
 extern "C" int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],

@@ -57,33 +57,6 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
     atomicAdd(&count[c], 1);
 }
 
-// Exclusive scan of count[0..m) into start[0..m], one workgroup of 1024.
-__global__ void __launch_bounds__(1024) k_scan(const int *__restrict__ count, int m,
-                                               int *__restrict__ start) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int chunk = (m + 1023) / 1024;
-    const int b = t * chunk, e = min(m, b + chunk);
-    int s = 0;
-    for (int k = b; k < e; k++)
-        s += count[k];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = (t == 0) ? 0 : part[t - 1];
-    for (int k = b; k < e; k++) {
-        start[k] = run;
-        run += count[k];
-    }
-    if (t == 1023)
-        start[m] = part[1023];
-}
-
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
                                const int *__restrict__ cell_id, const int *__restrict__ start,
@@ -96,6 +69,14 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     const int pos = start[c] + atomicAdd(&cursor[c], 1);
     perm[pos] = i;
     sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+}
+
+// value of lane k (uniform) in every lane
+__device__ __forceinline__ double bcast(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 __device__ __forceinline__ int wave_max(int v) {
@@ -166,6 +147,7 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         return;
     const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
+    const double Li = 1.0 / L;
     // prefilter threshold: slightly above rcut^2, the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
     const bool all_cells = (2 * reach + 1 > nc);
@@ -215,27 +197,36 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                     sx = sxw * L;
                     sy = syw * L;
                     sz = szw * L;
-                    for (int j = jb; j < je; j++) {
-                        const double4 pj = sorted[j];
-                        double dx = (pi.x - pj.x) - sx, dy = (pi.y - pj.y) - sy,
-                               dz = (pi.z - pj.z) - sz;
-                        int s = sidx;
-                        if (all_cells) {
-                            // nearest image per axis (the reference's round(dx/L))
-                            const double rx = round(dx / L), ry = round(dy / L), rz = round(dz / L);
-                            dx -= rx * L;
-                            dy -= ry * L;
-                            dz -= rz * L;
-                            s = ((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1);
+                    // neighbours in chunks of 64: one coalesced load, then
+                    // lane k's particle is broadcast to all lanes
+                    for (int cb = jb; cb < je; cb += kWave) {
+                        const int jj = cb + lane;
+                        const double4 mine = (jj < je) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
+                        const int cnt = min(kWave, je - cb);
+                        for (int k = 0; k < cnt; k++) {
+                            const int j = cb + k;
+                            const double xj = bcast(mine.x, k), yj = bcast(mine.y, k),
+                                         zj = bcast(mine.z, k);
+                            double dx = (pi.x - xj) - sx, dy = (pi.y - yj) - sy, dz = (pi.z - zj) - sz;
+                            int s = sidx;
+                            if (all_cells) {
+                                // nearest image per axis (the reference's round(dx/L))
+                                const double rx = round(dx * Li), ry = round(dy * Li),
+                                             rz = round(dz * Li);
+                                dx -= rx * L;
+                                dy -= ry * L;
+                                dz -= rz * L;
+                                s = ((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1);
+                            }
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            if (active && j != i && r2 <= rc2) {
+                                qj[tail][lane] = j;
+                                qs[tail][lane] = (unsigned char)s;
+                                tail++;
+                            }
+                            if (__ballot(tail == kQueue))
+                                drain();
                         }
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        if (active && j != i && r2 <= rc2) {
-                            qj[tail][lane] = j;
-                            qs[tail][lane] = (unsigned char)s;
-                            tail++;
-                        }
-                        if (__ballot(tail == kQueue))
-                            drain();
                     }
                 }
             }
@@ -264,7 +255,7 @@ struct p3mk_cells {
     int nc, reach;
     long ncell;
     double L, rcut, inv_side;
-    int *cell_id, *count, *start, *cursor, *perm;
+    int *cell_id, *count, *start, *cursor, *perm, *scan_scratch;
     double4 *sorted;
 };
 
@@ -276,6 +267,7 @@ extern "C" void p3mk_cells_destroy(p3mk_cells *c) {
     p3mk_release(c->start);
     p3mk_release(c->cursor);
     p3mk_release(c->perm);
+    p3mk_release(c->scan_scratch);
     p3mk_release(c->sorted);
     delete c;
 }
@@ -313,6 +305,8 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->start, sizeof(int) * (c->ncell + 1));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cursor, sizeof(int) * (c->ncell + 1));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->perm, sizeof(int) * nn);
+    if (rc == P3MK_OK)
+        rc = p3mk_alloc((void **)&c->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(c->ncell));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->sorted, sizeof(double4) * nn);
     if (rc != P3MK_OK) {
         p3mk_cells_destroy(c);
@@ -336,8 +330,7 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, s, n, x, y, z, L, c->inv_side, c->nc,
                        c->cell_id, c->count);
     P3MK_TRY(p3mk_launch_check("cell count"));
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->count, (int)c->ncell, c->start);
-    P3MK_TRY(p3mk_launch_check("cell scan"));
+    P3MK_TRY(p3mk_exclusive_scan(c->count, c->ncell, c->start, c->scan_scratch, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, c->cell_id,
                        c->start, c->cursor, c->perm, c->sorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
