@@ -80,41 +80,17 @@ __global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *_
 
 // ---- halo-tile assignment ------------------------------------------------------
 
-// LDS accumulators: FP64 atomics, or FP32 atomics with the TwoSum rounding
-// error of every add collected in a second FP32 array (compensated sum).
-template <typename T> struct LdsAcc;
-template <> struct LdsAcc<double> {
-    static constexpr int kArrays = 1;
-    __device__ static __forceinline__ void add(double *a, int nall, int idx, double v) {
-        (void)nall;
-        atomicAdd(&a[idx], v);
-    }
-    __device__ static __forceinline__ double value(const double *a, int nall, int idx) {
-        (void)nall;
-        return a[idx];
-    }
-};
-template <> struct LdsAcc<float> {
-    static constexpr int kArrays = 2;
-    __device__ static __forceinline__ void add(float *a, int nall, int idx, float v) {
-        const float old = atomicAdd(&a[idx], v);
-        float s, e;
-        two_sum(old, v, s, e);
-        if (e != 0.0f)
-            atomicAdd(&a[nall + idx], e);
-    }
-    __device__ static __forceinline__ float value(const float *a, int nall, int idx) {
-        return a[idx] + a[nall + idx];
-    }
-};
-
+// LDS accumulation is FP64 in both forms: fire-and-forget ds_add_f64 (no
+// returned value to wait for), and the FP32 form's products are summed
+// without rounding error growth; the mesh itself is stored in the form's
+// precision.
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
 k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
               const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
               T *__restrict__ mesh) {
     extern __shared__ unsigned char smem_raw[];
-    T *acc = reinterpret_cast<T *>(smem_raw);
+    double *acc = reinterpret_cast<double *>(smem_raw);
     constexpr int NM = IL ? 2 : 1;
     const int R = Tt + P - 1 + (IL ? 1 : 0); // halo region side
     const int npts = R * R * R;
@@ -138,8 +114,8 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
     const int tz = (int)(tile % nt), ty = (int)((tile / nt) % nt), tx = (int)(tile / ((long)nt * nt));
     const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
 
-    for (int k = threadIdx.x; k < LdsAcc<T>::kArrays * nall; k += blockDim.x)
-        acc[k] = T(0);
+    for (int k = threadIdx.x; k < nall; k += blockDim.x)
+        acc[k] = 0.0;
     __syncthreads();
     for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         const double4 r = sorted[p];
@@ -158,7 +134,7 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
             o0 += (o0 < 0) ? M : 0;
             o1 += (o1 < 0) ? M : 0;
             o2 += (o2 < 0) ? M : 0;
-            T *a = acc + ii * npts;
+            double *a = acc + ii * npts;
             for (int a0 = 0; a0 < P; a0++) {
                 const T t0 = qv * w0[a0];
 #pragma unroll
@@ -167,7 +143,7 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
                     const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
 #pragma unroll
                     for (int a2 = 0; a2 < P; a2++)
-                        LdsAcc<T>::add(a, nall, row + a2, t1 * w2[a2]);
+                        atomicAdd(&a[row + a2], (double)(t1 * w2[a2]));
                 }
             }
         }
@@ -180,7 +156,7 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
         const long g = ((long)gx * M + gy) * M + gz;
 #pragma unroll
         for (int ii = 0; ii < NM; ii++) {
-            const T v = LdsAcc<T>::value(acc, nall, ii * npts + k);
+            const T v = (T)acc[ii * npts + k];
             if (v != T(0)) {
                 if (R2C)
                     atomicAdd(&mesh[g], v);
@@ -393,7 +369,7 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, p3mk_que
     const double Hi = (double)g->M / g->L;
     const int NM = g->interlaced ? 2 : 1;
     const int R = s->T + g->cao - 1 + (g->interlaced ? 1 : 0);
-    const size_t lds = sizeof(T) * LdsAcc<T>::kArrays * NM * (size_t)R * R * R;
+    const size_t lds = sizeof(double) * NM * (size_t)R * R * R;
     if (lds > 64 * 1024)
         return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
     // upper bound of the number of work items; surplus workgroups exit

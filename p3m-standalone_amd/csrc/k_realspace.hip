@@ -6,18 +6,25 @@
 // and 0.5 q_i q_j erfc(alpha r)/r to the energy.
 //
 // Layout: particles are counting-sorted into cells of side >= rcut/R (R = 2
-// when the box holds >= 5 cells per side), folded into [0, L).  One
-// wavefront owns one home cell; its 64 lanes take 64 home particles at a
-// time and walk the (2R+1)^3 neighbour cells in the same order, so neighbour
-// positions are wave-uniform loads.  Periodic images are handled by a
-// per-neighbour-cell shift of 0 or +-L, which equals the reference's
-// round(dx/L)*L for every pair inside the cutoff (|dx| <= rcut <= L/2), so
-// dx = (x_i - x_j) - shift is the reference's dx bit for bit.
+// when the box holds >= 5 cells per side), folded into [0, L).  A work item
+// is (home cell, chunk of 64 home particles, one x-slab of the (2R+1)^3
+// neighbour cells); a persistent grid of single-wavefront workgroups claims
+// items from an atomic counter, so the dense cells of a clustered system are
+// spread over the whole GPU.  Inside an item the 64 lanes own 64 home
+// particles and walk the slab's neighbour cells in the same order:
+// neighbours arrive in chunks of 64 with one coalesced load and are
+// broadcast lane by lane (readlane), so the candidate loop touches no
+// memory.  Periodic images are handled by a per-neighbour-cell shift of 0 or
+// +-L, which equals the reference's round(dx/L)*L for every pair inside the
+// cutoff (|dx| <= rcut <= L/2), so dx = (x_i - x_j) - shift is the
+// reference's dx bit for bit.
 //
 // A cheap distance test accepts ~30% of the candidates; accepted neighbour
 // indices go to a per-lane queue in LDS, and the expensive erfc/exp part
 // runs when the queue is drained, so lanes stay busy instead of idling
-// through other lanes' accepted pairs.
+// through other lanes' accepted pairs.  Forces of an item are added to fr
+// with FP64 atomics (one per particle and slab).
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -69,6 +76,14 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     const int pos = start[c] + atomicAdd(&cursor[c], 1);
     perm[pos] = i;
     sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+}
+
+// work items of a cell: ceil(count / 64) chunks times nslab slabs
+__global__ void k_cell_items(const int *__restrict__ count, long ncell, int nslab,
+                             int *__restrict__ items) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell)
+        items[c] = ((count[c] + 63) / 64) * nslab;
 }
 
 // value of lane k (uniform) in every lane
@@ -131,31 +146,50 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     }
 }
 
-// One wavefront per home cell.  Lane queue entries store the neighbour's
-// sorted index and the index of its periodic shift (0..26 for -1/0/+1 per axis).
+// Persistent kernel over work items (home cell, chunk, slab).  Lane queue
+// entries store the neighbour's sorted index and the index of its periodic
+// shift (0..26 for -1/0/+1 per axis).
 template <bool FP32>
 __global__ void __launch_bounds__(kWave)
 k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
-            const int *__restrict__ start, int nc, int reach, double L, double rcut, double alpha,
+            const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
+            int reach, double L, double rcut, double alpha, int *__restrict__ counter,
             double *__restrict__ fr, double *__restrict__ energy) {
     __shared__ int qj[kQueue][kWave];
     __shared__ unsigned char qs[kQueue][kWave];
     const int lane = threadIdx.x;
-    const int home = blockIdx.x;
-    const int hb = start[home], he = start[home + 1];
-    if (hb == he)
-        return;
-    const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
+    const int total = istart[ncell];
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
     const double Li = 1.0 / L;
     // prefilter threshold: slightly above rcut^2, the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
     const bool all_cells = (2 * reach + 1 > nc);
+    const int nslab = all_cells ? 1 : 2 * reach + 1;
     const int lo = all_cells ? 0 : -reach, hi = all_cells ? nc - 1 : reach;
     double en_tot = 0.0;
 
-    for (int base = hb; base < he; base += kWave) {
-        const int i = base + lane;
+    for (;;) {
+        int item = 0;
+        if (lane == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        // item -> home cell (binary search in the item prefix sums)
+        long clo = 0, chi = ncell;
+        while (chi - clo > 1) {
+            const long mid = (clo + chi) / 2;
+            if (istart[mid] <= item)
+                clo = mid;
+            else
+                chi = mid;
+        }
+        const int home = (int)clo;
+        const int local = item - istart[home];
+        const int chunk = local / nslab, slab = local - chunk * nslab;
+        const int hb = start[home], he = start[home + 1];
+        const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
+        const int i = hb + chunk * kWave + lane;
         const bool active = i < he;
         const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
         PairAcc acc = {0.0, 0.0, 0.0, 0.0};
@@ -175,7 +209,8 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
             tail = 0;
         };
 
-        for (int a = lo; a <= hi; a++) {
+        const int a_lo = all_cells ? lo : lo + slab, a_hi = all_cells ? hi : lo + slab;
+        for (int a = a_lo; a <= a_hi; a++) {
             int cx = all_cells ? a : hx + a, wx = 0;
             if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
             for (int b = lo; b <= hi; b++) {
@@ -187,16 +222,11 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                     const int cell = (cx * nc + cy) * nc + cz;
                     const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
                     const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-                    int sidx, sxw = wx, syw = wy, szw = wz;
-                    double sx, sy, sz;
-                    if (all_cells) {
-                        // small box: the image is chosen per pair below
-                        sxw = syw = szw = 0;
-                    }
-                    sidx = (sxw + 1) * 9 + (syw + 1) * 3 + (szw + 1);
-                    sx = sxw * L;
-                    sy = syw * L;
-                    sz = szw * L;
+                    // small box: the image is chosen per pair below
+                    const int sxw = all_cells ? 0 : wx, syw = all_cells ? 0 : wy,
+                              szw = all_cells ? 0 : wz;
+                    const int sidx = (sxw + 1) * 9 + (syw + 1) * 3 + (szw + 1);
+                    const double sx = sxw * L, sy = syw * L, sz = szw * L;
                     // neighbours in chunks of 64: one coalesced load, then
                     // lane k's particle is broadcast to all lanes
                     for (int cb = jb; cb < je; cb += kWave) {
@@ -234,9 +264,9 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         drain();
         if (active) {
             const int p = perm[i];
-            fr[p] += acc.f0;
-            fr[(long)n + p] += acc.f1;
-            fr[2L * n + p] += acc.f2;
+            atomicAdd(&fr[p], acc.f0);
+            atomicAdd(&fr[(long)n + p], acc.f1);
+            atomicAdd(&fr[2L * n + p], acc.f2);
         }
         en_tot += acc.en;
     }
@@ -255,8 +285,9 @@ struct p3mk_cells {
     int nc, reach;
     long ncell;
     double L, rcut, inv_side;
-    int *cell_id, *count, *start, *cursor, *perm, *scan_scratch;
+    int *cell_id, *count, *start, *cursor, *perm, *scan_scratch, *items, *istart, *counter;
     double4 *sorted;
+    int grid;
 };
 
 extern "C" void p3mk_cells_destroy(p3mk_cells *c) {
@@ -268,6 +299,9 @@ extern "C" void p3mk_cells_destroy(p3mk_cells *c) {
     p3mk_release(c->cursor);
     p3mk_release(c->perm);
     p3mk_release(c->scan_scratch);
+    p3mk_release(c->items);
+    p3mk_release(c->istart);
+    p3mk_release(c->counter);
     p3mk_release(c->sorted);
     delete c;
 }
@@ -307,6 +341,16 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->perm, sizeof(int) * nn);
     if (rc == P3MK_OK)
         rc = p3mk_alloc((void **)&c->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(c->ncell));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->items, sizeof(int) * (c->ncell + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->istart, sizeof(int) * (c->ncell + 1));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->counter, sizeof(int) * 4);
+    if (rc == P3MK_OK) {
+        // persistent grid: enough single-wave workgroups to fill every CU
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        c->grid = (cus > 0 ? cus : 256) * 32;
+    }
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->sorted, sizeof(double4) * nn);
     if (rc != P3MK_OK) {
         p3mk_cells_destroy(c);
@@ -334,12 +378,21 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, c->cell_id,
                        c->start, c->cursor, c->perm, c->sorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
-    const dim3 grid((unsigned)c->ncell);
+    const bool all_cells = (2 * c->reach + 1 > c->nc);
+    const int nslab = all_cells ? 1 : 2 * c->reach + 1;
+    hipLaunchKernelGGL(k_cell_items, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, s,
+                       c->count, c->ncell, nslab, c->items);
+    P3MK_TRY(p3mk_launch_check("cell items"));
+    P3MK_TRY(p3mk_exclusive_scan(c->items, c->ncell, c->istart, c->scan_scratch, qu));
+    P3MK_TRY(p3mk_fill_zero(c->counter, sizeof(int) * 4, qu));
+    const dim3 grid((unsigned)c->grid);
     if (fp32)
         hipLaunchKernelGGL(k_realspace<true>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
-                           c->start, c->nc, c->reach, L, rcut, alpha, fr, energy);
+                           c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                           c->counter, fr, energy);
     else
         hipLaunchKernelGGL(k_realspace<false>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
-                           c->start, c->nc, c->reach, L, rcut, alpha, fr, energy);
+                           c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                           c->counter, fr, energy);
     return p3mk_launch_check("realspace");
 }
