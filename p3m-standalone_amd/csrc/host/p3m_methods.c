@@ -84,6 +84,7 @@ typedef struct {
     p3mk_psort *psort;
     double cells_L, cells_rc;
     int have_G;
+    int have_ks_events;
     int have_rs_events;
     float stage_ms[5];
 } p3m_ctx;
@@ -332,18 +333,23 @@ static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, c
              "back-interpolation");
     }
     TRYK(p3mk_record(c->q, 3), "event");
+    c->have_ks_events = 1;
     return P3M_HIP_OK;
 }
 
 static int collect_stage_times(p3m_ctx *c) {
     float t;
     int k;
-    for (k = 0; k < 3; k++) {
-        TRYK(p3mk_elapsed_ms(c->q, k, k + 1, &t), "event timing");
-        c->stage_ms[k] = t;
+    for (k = 0; k < 4; k++)
+        c->stage_ms[k] = 0.0f;
+    if (c->have_ks_events) {
+        for (k = 0; k < 3; k++) {
+            TRYK(p3mk_elapsed_ms(c->q, k, k + 1, &t), "event timing");
+            c->stage_ms[k] = t;
+        }
+        TRYK(p3mk_elapsed_ms(c->q, 0, 3, &t), "event timing");
+        c->stage_ms[3] = t;
     }
-    TRYK(p3mk_elapsed_ms(c->q, 0, 3, &t), "event timing");
-    c->stage_ms[3] = t;
     c->stage_ms[4] = 0.0f;
     if (c->have_rs_events) {
         TRYK(p3mk_elapsed_ms(c->q, 4, 5, &t), "event timing");
