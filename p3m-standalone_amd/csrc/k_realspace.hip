@@ -31,6 +31,7 @@
 #include <new>
 
 #include "device_common.hpp"
+#include "erfcx_coeffs.hpp"
 #include "k_internal.hpp"
 
 namespace {
@@ -94,6 +95,10 @@ __device__ __forceinline__ double bcast(double v, int k) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+__device__ __forceinline__ float bcastf(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
 __device__ __forceinline__ int wave_max(int v) {
     for (int off = 32; off > 0; off >>= 1)
         v = max(v, __shfl_xor(v, off, kWave));
@@ -107,7 +112,39 @@ struct PairAcc {
     double f0, f1, f2, en;
 };
 
-// Full pair evaluation with the reference's inclusion test r <= rcut.
+// erfcx(x) = exp(x^2) erfc(x), x >= 0, from the generated piecewise tables
+__device__ __forceinline__ double erfcx_d(double x) {
+    const double y = 1.0 / (1.0 + 0.5 * x);
+    const int K = p3m::kErfcxDoublePieces;
+    int k = (int)(y * K);
+    k = k > K - 1 ? K - 1 : k;
+    const double t = (y * K - (double)k) - 0.5;
+    const double *c = p3m::kErfcxDoubleCoef[k];
+    double acc = c[p3m::kErfcxDoubleDegree];
+#pragma unroll
+    for (int d = p3m::kErfcxDoubleDegree - 1; d >= 0; d--)
+        acc = fma(acc, t, c[d]);
+    return acc;
+}
+
+__device__ __forceinline__ float erfcx_f(float x) {
+    const float y = 1.0f / (1.0f + 0.5f * x);
+    const int K = p3m::kErfcxFloatPieces;
+    int k = (int)(y * K);
+    k = k > K - 1 ? K - 1 : k;
+    const float t = (y * K - (float)k) - 0.5f;
+    const float *c = p3m::kErfcxFloatCoef[k];
+    float acc = c[p3m::kErfcxFloatDegree];
+#pragma unroll
+    for (int d = p3m::kErfcxFloatDegree - 1; d >= 0; d--)
+        acc = fmaf(acc, t, c[d]);
+    return acc;
+}
+
+// Full pair evaluation with the reference's inclusion test r <= rcut
+// (realpart.c:134-148).  erfc(x) = exp(-x^2) erfcx(x) shares the one
+// exponential with the exp(-x^2) term of the force; x^2 is split exactly
+// (hi + lo by FMA) so exp(-x^2) keeps its accuracy for large x.
 template <bool FP32>
 __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, double sx, double sy,
                                      double sz, double rcut, double alpha, double two_a_spi,
@@ -120,52 +157,59 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
         dz = (pi.z - pj.z) - sz;
         r2 = dx * dx + dy * dy + dz * dz;
     }
+    // r <= rcut decided exactly (correctly rounded sqrt) only near the edge
+    const double rc2 = rcut * rcut;
+    bool inside = r2 < rc2 * (1.0 - 1e-12);
+    if (!inside && r2 <= rc2 * (1.0 + 1e-12))
+        inside = sqrt(r2) <= rcut;
+    if (!inside)
+        return;
+    const double qq = pi.w * pj.w;
     if (FP32) {
-        if (r2 <= rcut * rcut) {
-            const float rf = sqrtf((float)r2);
-            const float ar = (float)alpha * rf;
-            const float ec = erfcf(ar);
-            const float fak =
-                (float)(pi.w * pj.w) * (ec / rf + (float)two_a_spi * expf(-ar * ar)) / (float)r2;
-            acc.f0 += (double)(fak * (float)dx);
-            acc.f1 += (double)(fak * (float)dy);
-            acc.f2 += (double)(fak * (float)dz);
-            acc.en += 0.5 * pi.w * pj.w * (double)(ec / rf);
-        }
+        const float rinv = rsqrtf((float)r2);
+        const float x = (float)alpha * ((float)r2 * rinv);
+        const float e = expf(-x * x);
+        const float ec = e * erfcx_f(x);
+        const float fak = (float)qq * (ec * rinv + (float)two_a_spi * e) * (rinv * rinv);
+        acc.f0 += (double)(fak * (float)dx);
+        acc.f1 += (double)(fak * (float)dy);
+        acc.f2 += (double)(fak * (float)dz);
+        acc.en += 0.5 * qq * (double)(ec * rinv);
     } else {
-        const double r = sqrt(r2);
-        if (r <= rcut) {
-            const double ar = alpha * r;
-            const double ec = erfc(ar);
-            const double fak = pi.w * pj.w * (ec / r + two_a_spi * exp(-ar * ar)) / (r * r);
-            acc.f0 += fak * dx;
-            acc.f1 += fak * dy;
-            acc.f2 += fak * dz;
-            acc.en += 0.5 * pi.w * pj.w * ec / r;
-        }
+        const double rinv = rsqrt(r2);
+        const double x = alpha * (r2 * rinv);
+        const double x2 = x * x;
+        const double x2lo = fma(x, x, -x2);
+        const double e0 = exp(-x2);
+        const double e = fma(-e0, x2lo, e0); // exp(-(x2 + x2lo))
+        const double ec = e * erfcx_d(x);
+        const double fak = qq * (ec * rinv + two_a_spi * e) * (rinv * rinv);
+        acc.f0 += fak * dx;
+        acc.f1 += fak * dy;
+        acc.f2 += fak * dz;
+        acc.en += 0.5 * qq * ec * rinv;
     }
 }
 
-// Persistent kernel over work items (home cell, chunk, slab).  Lane queue
-// entries store the neighbour's sorted index and the index of its periodic
-// shift (0..26 for -1/0/+1 per axis).
-template <bool FP32>
+// Persistent kernel over work items (home cell, chunk, slab).  A queue
+// entry packs the neighbour's sorted index (bits 5..31) and the index of its
+// periodic shift (bits 0..4: 0..26 for -1/0/+1 per axis).
+template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
 k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
             int reach, double L, double rcut, double alpha, int *__restrict__ counter,
             double *__restrict__ fr, double *__restrict__ energy) {
-    __shared__ int qj[kQueue][kWave];
-    __shared__ unsigned char qs[kQueue][kWave];
+    __shared__ unsigned int queue[kQueue][kWave];
     const int lane = threadIdx.x;
     const int total = istart[ncell];
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
-    const double Li = 1.0 / L;
-    // prefilter threshold: slightly above rcut^2, the exact test runs in pair()
+    const double Li = 1.0 / L, side = L / nc;
+    // prefilter thresholds; the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
-    const bool all_cells = (2 * reach + 1 > nc);
-    const int nslab = all_cells ? 1 : 2 * reach + 1;
-    const int lo = all_cells ? 0 : -reach, hi = all_cells ? nc - 1 : reach;
+    const float rc2f = (float)(rcut * rcut) * (1.0f + 1e-5f);
+    const int nslab = SMALL_BOX ? 1 : 2 * reach + 1;
+    const int lo = SMALL_BOX ? 0 : -reach, hi = SMALL_BOX ? nc - 1 : reach;
     double en_tot = 0.0;
 
     for (;;) {
@@ -192,6 +236,9 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         const int i = hb + chunk * kWave + lane;
         const bool active = i < he;
         const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        // home-cell-relative single-precision coordinates for the prefilter
+        const double ox = hx * side, oy = hy * side, oz = hz * side;
+        const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
         PairAcc acc = {0.0, 0.0, 0.0, 0.0};
         int tail = 0;
 
@@ -199,8 +246,8 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
             const int m = wave_max(tail);
             for (int k = 0; k < m; k++) {
                 if (k < tail) {
-                    const int j = qj[k][lane];
-                    const int s = qs[k][lane];
+                    const unsigned int e = queue[k][lane];
+                    const int j = (int)(e >> 5), s = (int)(e & 31u);
                     const double sx = (double)(s / 9 - 1) * L, sy = (double)((s / 3) % 3 - 1) * L,
                                  sz = (double)(s % 3 - 1) * L;
                     pair<FP32>(pi, sorted[j], sx, sy, sz, rcut, alpha, two_a_spi, acc);
@@ -209,53 +256,63 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
             tail = 0;
         };
 
-        const int a_lo = all_cells ? lo : lo + slab, a_hi = all_cells ? hi : lo + slab;
+        const int a_lo = SMALL_BOX ? lo : lo + slab, a_hi = SMALL_BOX ? hi : lo + slab;
         for (int a = a_lo; a <= a_hi; a++) {
-            int cx = all_cells ? a : hx + a, wx = 0;
+            int cx = SMALL_BOX ? a : hx + a, wx = 0;
             if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
             for (int b = lo; b <= hi; b++) {
-                int cy = all_cells ? b : hy + b, wy = 0;
+                int cy = SMALL_BOX ? b : hy + b, wy = 0;
                 if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
                 for (int c = lo; c <= hi; c++) {
-                    int cz = all_cells ? c : hz + c, wz = 0;
+                    int cz = SMALL_BOX ? c : hz + c, wz = 0;
                     if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
                     const int cell = (cx * nc + cy) * nc + cz;
                     const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
                     const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-                    // small box: the image is chosen per pair below
-                    const int sxw = all_cells ? 0 : wx, syw = all_cells ? 0 : wy,
-                              szw = all_cells ? 0 : wz;
-                    const int sidx = (sxw + 1) * 9 + (syw + 1) * 3 + (szw + 1);
-                    const double sx = sxw * L, sy = syw * L, sz = szw * L;
+                    const double sx = wx * L, sy = wy * L, sz = wz * L;
+                    const unsigned int sidx = (unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1));
                     // neighbours in chunks of 64: one coalesced load, then
                     // lane k's particle is broadcast to all lanes
                     for (int cb = jb; cb < je; cb += kWave) {
                         const int jj = cb + lane;
                         const double4 mine = (jj < je) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
                         const int cnt = min(kWave, je - cb);
-                        for (int k = 0; k < cnt; k++) {
-                            const int j = cb + k;
-                            const double xj = bcast(mine.x, k), yj = bcast(mine.y, k),
-                                         zj = bcast(mine.z, k);
-                            double dx = (pi.x - xj) - sx, dy = (pi.y - yj) - sy, dz = (pi.z - zj) - sz;
-                            int s = sidx;
-                            if (all_cells) {
+                        if (SMALL_BOX) {
+                            for (int k = 0; k < cnt; k++) {
+                                const int j = cb + k;
+                                double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
+                                       dz = pi.z - bcast(mine.z, k);
                                 // nearest image per axis (the reference's round(dx/L))
-                                const double rx = round(dx * Li), ry = round(dy * Li),
-                                             rz = round(dz * Li);
+                                const double rx = round(dx * Li), ry = round(dy * Li), rz = round(dz * Li);
                                 dx -= rx * L;
                                 dy -= ry * L;
                                 dz -= rz * L;
-                                s = ((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1);
+                                const unsigned int s = (unsigned)(((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1));
+                                const double r2 = dx * dx + dy * dy + dz * dz;
+                                if (active && j != i && r2 <= rc2) {
+                                    queue[tail][lane] = ((unsigned)j << 5) | s;
+                                    tail++;
+                                }
+                                if (__ballot(tail == kQueue))
+                                    drain();
                             }
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            if (active && j != i && r2 <= rc2) {
-                                qj[tail][lane] = j;
-                                qs[tail][lane] = (unsigned char)s;
-                                tail++;
+                        } else {
+                            // neighbour in the home cell's frame, image shift included
+                            const float mxf = (float)((mine.x + sx) - ox), myf = (float)((mine.y + sy) - oy),
+                                        mzf = (float)((mine.z + sz) - oz);
+                            for (int k = 0; k < cnt; k++) {
+                                const int j = cb + k;
+                                const float dx = pxf - bcastf(mxf, k);
+                                const float dy = pyf - bcastf(myf, k);
+                                const float dz = pzf - bcastf(mzf, k);
+                                const float r2 = dx * dx + dy * dy + dz * dz;
+                                if (active && j != i && r2 <= rc2f) {
+                                    queue[tail][lane] = ((unsigned)j << 5) | sidx;
+                                    tail++;
+                                }
+                                if (__ballot(tail == kQueue))
+                                    drain();
                             }
-                            if (__ballot(tail == kQueue))
-                                drain();
                         }
                     }
                 }
@@ -386,13 +443,15 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     P3MK_TRY(p3mk_exclusive_scan(c->items, c->ncell, c->istart, c->scan_scratch, qu));
     P3MK_TRY(p3mk_fill_zero(c->counter, sizeof(int) * 4, qu));
     const dim3 grid((unsigned)c->grid);
-    if (fp32)
-        hipLaunchKernelGGL(k_realspace<true>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
-                           c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha,
-                           c->counter, fr, energy);
-    else
-        hipLaunchKernelGGL(k_realspace<false>, grid, dim3(kWave), 0, s, n, c->sorted, c->perm,
-                           c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha,
-                           c->counter, fr, energy);
+#define LAUNCH(F32, SB)                                                                            \
+    hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, c->sorted, c->perm,     \
+                       c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha, c->counter, \
+                       fr, energy)
+    if (fp32) {
+        if (all_cells) LAUNCH(true, true); else LAUNCH(true, false);
+    } else {
+        if (all_cells) LAUNCH(false, true); else LAUNCH(false, false);
+    }
+#undef LAUNCH
     return p3mk_launch_check("realspace");
 }
