@@ -7,8 +7,8 @@
 //
 // Layout: particles are counting-sorted into cells of side >= rcut/R (R = 2
 // when the box holds >= 5 cells per side), folded into [0, L).  A work item
-// is (home cell, chunk of 64 home particles, one x-slab of the (2R+1)^3
-// neighbour cells); a persistent grid of single-wavefront workgroups claims
+// is (home cell, chunk of 64 home particles) with its (2R+1)^3 neighbour
+// cells; a persistent grid of single-wavefront workgroups claims
 // items from an atomic counter, so the dense cells of a clustered system are
 // spread over the whole GPU.  Inside an item the 64 lanes own 64 home
 // particles and walk the slab's neighbour cells in the same order:
@@ -22,8 +22,9 @@
 // A cheap distance test accepts ~30% of the candidates; accepted neighbour
 // indices go to a per-lane queue in LDS, and the expensive erfc/exp part
 // runs when the queue is drained, so lanes stay busy instead of idling
-// through other lanes' accepted pairs.  Forces of an item are added to fr
-// with FP64 atomics (one per particle and slab).
+// through other lanes' accepted pairs (all lanes of an item see the whole
+// neighbourhood, so their acceptance rates and queue lengths are alike).
+// Forces of an item are added to fr with FP64 atomics.
 
 #include <hip/hip_runtime.h>
 
@@ -37,7 +38,7 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kQueue = 32;
+constexpr int kQueue = 2 * kWave; // flat queue: at most one round pending plus one append
 constexpr int kMaxCellsPerSide = 512;
 
 __device__ __forceinline__ double fold(double r, double L) {
@@ -191,16 +192,24 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     }
 }
 
-// Persistent kernel over work items (home cell, chunk, slab).  A queue
-// entry packs the neighbour's sorted index (bits 5..31) and the index of its
-// periodic shift (bits 0..4: 0..26 for -1/0/+1 per axis).
+// Persistent kernel over work items (home cell, chunk of 64 home
+// particles).  Accepted candidates go to one flat queue per wavefront:
+// lanes that accept the same neighbour append at count + (their rank among
+// the accepting lanes), so entries are dense; every drain round hands 64
+// entries to the 64 lanes, whoever owns them, and the pair forces are added
+// to the owners' accumulators in LDS.  An entry packs the neighbour's sorted
+// index (bits 5..31) and its periodic-shift index (bits 0..4: 0..26 for
+// -1/0/+1 per axis); the owner lane is stored beside it.
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
 k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
             int reach, double L, double rcut, double alpha, int *__restrict__ counter,
             double *__restrict__ fr, double *__restrict__ energy) {
-    __shared__ unsigned int queue[kQueue][kWave];
+    __shared__ unsigned int qe[kQueue];
+    __shared__ unsigned char qo[kQueue];
+    __shared__ double4 home_pos[kWave];
+    __shared__ double facc[3][kWave];
     const int lane = threadIdx.x;
     const int total = istart[ncell];
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
@@ -208,7 +217,6 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
     // prefilter thresholds; the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
     const float rc2f = (float)(rcut * rcut) * (1.0f + 1e-5f);
-    const int nslab = SMALL_BOX ? 1 : 2 * reach + 1;
     const int lo = SMALL_BOX ? 0 : -reach, hi = SMALL_BOX ? nc - 1 : reach;
     double en_tot = 0.0;
 
@@ -229,8 +237,7 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                 chi = mid;
         }
         const int home = (int)clo;
-        const int local = item - istart[home];
-        const int chunk = local / nslab, slab = local - chunk * nslab;
+        const int chunk = item - istart[home];
         const int hb = start[home], he = start[home + 1];
         const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
         const int i = hb + chunk * kWave + lane;
@@ -239,25 +246,67 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         // home-cell-relative single-precision coordinates for the prefilter
         const double ox = hx * side, oy = hy * side, oz = hz * side;
         const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
-        PairAcc acc = {0.0, 0.0, 0.0, 0.0};
-        int tail = 0;
+        home_pos[lane] = pi;
+        facc[0][lane] = 0.0;
+        facc[1][lane] = 0.0;
+        facc[2][lane] = 0.0;
+        int count = 0; // wave-uniform queue length
+        __syncthreads();
 
-        auto drain = [&]() {
-            const int m = wave_max(tail);
-            for (int k = 0; k < m; k++) {
-                if (k < tail) {
-                    const unsigned int e = queue[k][lane];
-                    const int j = (int)(e >> 5), s = (int)(e & 31u);
-                    const double sx = (double)(s / 9 - 1) * L, sy = (double)((s / 3) % 3 - 1) * L,
-                                 sz = (double)(s % 3 - 1) * L;
-                    pair<FP32>(pi, sorted[j], sx, sy, sz, rcut, alpha, two_a_spi, acc);
+        // one round: lane l takes entry l (if l < m), evaluates it and adds
+        // the force to the owner's accumulator
+        auto drain_round = [&](int m) {
+            if (lane < m) {
+                const unsigned int e = qe[lane];
+                const int o = qo[lane];
+                const int j = (int)(e >> 5), sh = (int)(e & 31u);
+                const double sx = (double)(sh / 9 - 1) * L, sy = (double)((sh / 3) % 3 - 1) * L,
+                             sz = (double)(sh % 3 - 1) * L;
+                PairAcc pa = {0.0, 0.0, 0.0, 0.0};
+                pair<FP32>(home_pos[o], sorted[j], sx, sy, sz, rcut, alpha, two_a_spi, pa);
+                if (pa.f0 != 0.0 || pa.f1 != 0.0 || pa.f2 != 0.0) {
+                    atomicAdd(&facc[0][o], pa.f0);
+                    atomicAdd(&facc[1][o], pa.f1);
+                    atomicAdd(&facc[2][o], pa.f2);
                 }
+                en_tot += pa.en;
             }
-            tail = 0;
+            __syncthreads();
+            // move the rest (< 64 entries) to the front
+            const int rest = count - m;
+            unsigned int e2 = 0;
+            unsigned char o2 = 0;
+            if (lane < rest) {
+                e2 = qe[m + lane];
+                o2 = qo[m + lane];
+            }
+            __syncthreads();
+            if (lane < rest) {
+                qe[lane] = e2;
+                qo[lane] = o2;
+            }
+            count = rest;
+            __syncthreads();
         };
 
-        const int a_lo = SMALL_BOX ? lo : lo + slab, a_hi = SMALL_BOX ? hi : lo + slab;
-        for (int a = a_lo; a <= a_hi; a++) {
+        auto append = [&](bool ok, unsigned int entry) {
+            const unsigned long long mask = __ballot(ok);
+            if (mask) {
+                const unsigned int rank = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+                if (ok) {
+                    qe[count + rank] = entry;
+                    qo[count + rank] = (unsigned char)lane;
+                }
+                count += __popcll(mask);
+                if (count >= kWave) {
+                    __syncthreads();
+                    drain_round(kWave);
+                }
+            }
+        };
+
+        for (int a = lo; a <= hi; a++) {
             int cx = SMALL_BOX ? a : hx + a, wx = 0;
             if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
             for (int b = lo; b <= hi; b++) {
@@ -287,14 +336,10 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                                 dx -= rx * L;
                                 dy -= ry * L;
                                 dz -= rz * L;
-                                const unsigned int s = (unsigned)(((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1));
+                                const unsigned int sh =
+                                    (unsigned)(((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1));
                                 const double r2 = dx * dx + dy * dy + dz * dz;
-                                if (active && j != i && r2 <= rc2) {
-                                    queue[tail][lane] = ((unsigned)j << 5) | s;
-                                    tail++;
-                                }
-                                if (__ballot(tail == kQueue))
-                                    drain();
+                                append(active && j != i && r2 <= rc2, ((unsigned)j << 5) | sh);
                             }
                         } else {
                             // neighbour in the home cell's frame, image shift included
@@ -306,26 +351,23 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                                 const float dy = pyf - bcastf(myf, k);
                                 const float dz = pzf - bcastf(mzf, k);
                                 const float r2 = dx * dx + dy * dy + dz * dz;
-                                if (active && j != i && r2 <= rc2f) {
-                                    queue[tail][lane] = ((unsigned)j << 5) | sidx;
-                                    tail++;
-                                }
-                                if (__ballot(tail == kQueue))
-                                    drain();
+                                append(active && j != i && r2 <= rc2f, ((unsigned)j << 5) | sidx);
                             }
                         }
                     }
                 }
             }
         }
-        drain();
+        __syncthreads();
+        if (count > 0)
+            drain_round(count);
         if (active) {
             const int p = perm[i];
-            atomicAdd(&fr[p], acc.f0);
-            atomicAdd(&fr[(long)n + p], acc.f1);
-            atomicAdd(&fr[2L * n + p], acc.f2);
+            atomicAdd(&fr[p], facc[0][lane]);
+            atomicAdd(&fr[(long)n + p], facc[1][lane]);
+            atomicAdd(&fr[2L * n + p], facc[2][lane]);
         }
-        en_tot += acc.en;
+        __syncthreads();
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
@@ -436,7 +478,7 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
                        c->start, c->cursor, c->perm, c->sorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
-    const int nslab = all_cells ? 1 : 2 * c->reach + 1;
+    const int nslab = 1; // one item walks the whole neighbourhood
     hipLaunchKernelGGL(k_cell_items, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, s,
                        c->count, c->ncell, nslab, c->items);
     P3MK_TRY(p3mk_launch_check("cell items"));
