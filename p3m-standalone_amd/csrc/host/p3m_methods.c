@@ -769,8 +769,13 @@ int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, do
 /* The method table                                                        */
 /* ---------------------------------------------------------------------- */
 
+/* The method-table entry points have no status channel (types.h:306-316):
+ * a failure is reported and ends the program, as a failed allocation in the
+ * reference's Init does, instead of returning wrong forces. */
 static void report(const char *where) {
     fprintf(stderr, "p3m_hip: %s: %s\n", where, g_err);
+    fflush(stderr);
+    exit(EXIT_FAILURE);
 }
 
 #define DEFINE_METHOD(NAME, ID)                                                                \

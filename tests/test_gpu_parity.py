@@ -174,6 +174,45 @@ def test_error_estimates(pkg, orc, method):
     assert abs(total - np.hypot(kref, rref)) <= 1e-8 * np.hypot(kref, rref)
 
 
+@pytest.mark.parametrize("name,method_id", [("method_p3m_ik", 0), ("method_p3m_ik_i", 1),
+                                            ("method_p3m_ad", 2), ("method_p3m_ad_i", 3),
+                                            ("method_p3m_ik_r", 6), ("method_p3m_ad_r", 7)])
+def test_method_table_drives_like_main(pkg, orc, name, method_id):
+    """The exported method_t objects called through their function pointers
+    in the order of the reference driver: Init (main.c:361), then per alpha
+    Influence_function (main.c:378) and Kspace_force (common.c:305)
+    accumulating into f_k, then Error / Error_k (main.c:432-433)."""
+    m = pkg.exported_method(name)
+    assert m.method_id == method_id
+    s = random_system(400, 10.0, seed=40 + method_id)
+    system = pkg.HostSystem.from_dict(s)
+    M, cao, rcut = 16, 6, 3.0
+    params = pkg.parameters(M, cao, 0.9, rcut)
+    pptr = ctypes.pointer(params)
+    d = m.Init(system.ptr, pptr)
+    assert d
+    forces = pkg.HostForces(400)
+    fk_sum, f_sum = np.zeros((3, 400)), np.zeros((3, 400))
+    for alpha in (0.9, 1.3):
+        params.alpha = alpha
+        m.Influence_function(system.ptr, pptr, d)
+        m.Kspace_force(system.ptr, pptr, d, forces.ptr)
+        fk, f = orc.kspace_force(method_id, s, M, cao, alpha)
+        fk_sum += fk
+        f_sum += f
+    assert rel_err(forces.kspace(), fk_sum) < FP64_TOL
+    if method_id in (2, 7):
+        assert rel_err(forces.total(), f_sum) < 1e-11
+    else:
+        assert np.all(forces.total() == 0.0)
+    q2 = float(np.sum(s["q"] ** 2))
+    kref = orc.kspace_error(method_id, 400, 10.0, q2, M, cao, params.alpha)
+    rref = orc.realspace_error(400, 10.0, q2, rcut, params.alpha)
+    assert abs(m.Error_k(system.ptr, pptr) - kref) <= 1e-8 * abs(kref)
+    assert abs(m.Error(system.ptr, pptr) - np.hypot(kref, rref)) <= 1e-8 * np.hypot(kref, rref)
+    pkg.check(pkg.lib().p3m_hip_free(d))
+
+
 @pytest.mark.parametrize("M", [16, 18, 20, 22, 24, 26, 28, 30, 36, 42, 44, 48, 64, 128])
 def test_fft3d_vs_numpy(pkg, M):
     """The Stockham transform (radices 2..13) against numpy's FFT."""
