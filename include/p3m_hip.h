@@ -278,6 +278,15 @@ int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101)
  * [2] back-interpolation, [3] whole Kspace_force, [4] last real-space call
  * (0 if none).  Waits for the stream. */
 int p3m_hip_stage_times(data_t *d, double out[5]);
+/* Per-step stage times without synchronising between steps: after
+ * p3m_hip_stage_ring_start(d, n), each of the next n calls of
+ * p3m_hip_calculate_forces_device records its stage events into a set of its
+ * own (n = 0 stops recording).  p3m_hip_stage_ring_read waits for the stream
+ * and writes, for each recorded step s < *nsteps <= max_steps, the five
+ * times of p3m_hip_stage_times to out[5 s .. 5 s + 4].  Fills the role of the
+ * reference's runtime_t timers (p3m-common.h:48-57) for a sequence of steps. */
+int p3m_hip_stage_ring_start(data_t *d, int nsteps);
+int p3m_hip_stage_ring_read(data_t *d, double *out, int max_steps, int *nsteps);
 /* Copy the influence function (M^3 doubles, r_ind order) to the host. */
 int p3m_hip_get_influence_function(data_t *d, double *G);
 /* Copy the self-force coefficients (27 x 3 doubles) to the host. */

@@ -110,6 +110,8 @@ def lib():
             "p3m_hip_error": (i, [i, _SP, _PP, _dp, _dp]),
             "p3m_hip_set_error_table": (i, [i, i, i, _dp]),
             "p3m_hip_stage_times": (i, [_DPT, _dp]),
+            "p3m_hip_stage_ring_start": (i, [_DPT, i]),
+            "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
             "p3m_hip_get_influence_function": (i, [_DPT, _dp]),
             "p3m_hip_get_self_force_coefficients": (i, [_DPT, _dp]),
         }
@@ -265,6 +267,18 @@ class Solver:
         t = np.zeros(5)
         check(lib().p3m_hip_stage_times(self.d, t.ctypes.data_as(_dp)))
         return t
+
+    def stage_ring_start(self, nsteps):
+        """Record the stage events of the next `nsteps` device force calls."""
+        check(lib().p3m_hip_stage_ring_start(self.d, int(nsteps)))
+
+    def stage_ring_read(self, max_steps):
+        """(steps, 5) stage times in ms of the recorded calls."""
+        out = np.zeros((max_steps, 5))
+        k = ctypes.c_int()
+        check(lib().p3m_hip_stage_ring_read(self.d, out.ctypes.data_as(_dp), int(max_steps),
+                                            ctypes.byref(k)))
+        return out[:k.value]
 
     # device-resident entry points (pointers to device memory, e.g. torch tensors)
     def kspace_force_device(self, n, length, x, y, z, q, fk, f):

@@ -87,6 +87,8 @@ typedef struct {
     int have_ks_events;
     int have_rs_events;
     float stage_ms[5];
+    unsigned char *ring_rs; /* per event-ring set: real space ran */
+    int ring_sets;
 } p3m_ctx;
 
 static p3m_ctx *ctx_of(data_t *d) { return d ? (p3m_ctx *)d->method_data : NULL; }
@@ -117,6 +119,7 @@ static void ctx_free(p3m_ctx *c) {
     p3mk_cells_destroy(c->cells);
     p3mk_psort_destroy(c->psort);
     p3mk_queue_destroy(c->q);
+    free(c->ring_rs);
     free(c);
 }
 
@@ -543,6 +546,61 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
     /* common.c:300-304: f += f_k + f_r */
     TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fk, f, c->q), "sum");
     TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fr, f, c->q), "sum");
+    if (c->ring_rs != NULL) {
+        int k = p3mk_ring_filled(c->q);
+        if (k < c->ring_sets)
+            c->ring_rs[k] = rcut != 0.0;
+        p3mk_ring_advance(c->q);
+    }
+    return P3M_HIP_OK;
+}
+
+int p3m_hip_stage_ring_start(data_t *d, int nsteps) {
+    p3m_ctx *c = ctx_of(d);
+    if (c == NULL)
+        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+    if (nsteps < 0)
+        return fail(P3M_HIP_EINVAL, "negative step count");
+    TRYK(p3mk_ring_start(c->q, nsteps), "event ring");
+    free(c->ring_rs);
+    c->ring_rs = NULL;
+    c->ring_sets = 0;
+    if (nsteps > 0) {
+        c->ring_rs = (unsigned char *)calloc((size_t)nsteps, 1);
+        if (c->ring_rs == NULL) {
+            p3mk_ring_start(c->q, 0);
+            return fail(P3M_HIP_ENOMEM, "host allocation failed");
+        }
+        c->ring_sets = nsteps;
+    }
+    return P3M_HIP_OK;
+}
+
+int p3m_hip_stage_ring_read(data_t *d, double *out, int max_steps, int *nsteps) {
+    p3m_ctx *c = ctx_of(d);
+    int k, s, filled;
+    float t;
+    if (c == NULL || out == NULL || nsteps == NULL)
+        return fail(P3M_HIP_EINVAL, "bad arguments");
+    TRYK(p3mk_queue_sync(c->q), "sync");
+    filled = p3mk_ring_filled(c->q);
+    if (filled > max_steps)
+        filled = max_steps;
+    for (s = 0; s < filled; s++) {
+        double *o = out + 5 * (size_t)s;
+        for (k = 0; k < 3; k++) {
+            TRYK(p3mk_ring_elapsed_ms(c->q, s, k, k + 1, &t), "event timing");
+            o[k] = t;
+        }
+        TRYK(p3mk_ring_elapsed_ms(c->q, s, 0, 3, &t), "event timing");
+        o[3] = t;
+        o[4] = 0.0;
+        if (c->ring_rs[s]) {
+            TRYK(p3mk_ring_elapsed_ms(c->q, s, 4, 5, &t), "event timing");
+            o[4] = t;
+        }
+    }
+    *nsteps = filled;
     return P3M_HIP_OK;
 }
 

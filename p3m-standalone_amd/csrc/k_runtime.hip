@@ -31,7 +31,23 @@ int p3mk_launch_check(const char *what) {
 struct p3mk_queue {
     hipStream_t stream;
     hipEvent_t ev[P3MK_NUM_EVENTS];
+    // optional ring of event sets: set k receives the events of the k-th
+    // step after p3mk_ring_start, so the stage times of several steps can be
+    // read after they all ran
+    hipEvent_t *ring = nullptr;
+    int ring_sets = 0;
+    int ring_pos = 0;
 };
+
+static void ring_free(p3mk_queue *q) {
+    if (q->ring) {
+        for (int i = 0; i < q->ring_sets * P3MK_NUM_EVENTS; i++)
+            (void)hipEventDestroy(q->ring[i]);
+        delete[] q->ring;
+    }
+    q->ring = nullptr;
+    q->ring_sets = q->ring_pos = 0;
+}
 
 hipStream_t p3mk_stream(p3mk_queue *q) { return q ? q->stream : (hipStream_t)0; }
 
@@ -73,6 +89,7 @@ extern "C" void p3mk_queue_destroy(p3mk_queue *q) {
     if (!q)
         return;
     (void)hipStreamSynchronize(q->stream);
+    ring_free(q);
     for (int i = 0; i < P3MK_NUM_EVENTS; i++)
         (void)hipEventDestroy(q->ev[i]);
     (void)hipStreamDestroy(q->stream);
@@ -92,6 +109,56 @@ extern "C" int p3mk_record(p3mk_queue *q, int slot) {
     hipError_t e = hipEventRecord(q->ev[slot], q->stream);
     if (e != hipSuccess)
         return p3mk_fail(P3MK_EDEVICE, "hipEventRecord", (int)e);
+    if (q->ring && q->ring_pos < q->ring_sets) {
+        e = hipEventRecord(q->ring[q->ring_pos * P3MK_NUM_EVENTS + slot], q->stream);
+        if (e != hipSuccess)
+            return p3mk_fail(P3MK_EDEVICE, "hipEventRecord", (int)e);
+    }
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_ring_start(p3mk_queue *q, int nsets) {
+    if (nsets < 0)
+        return p3mk_fail(P3MK_EINVAL, "negative ring size", 0);
+    hipError_t e = hipStreamSynchronize(q->stream);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipStreamSynchronize", (int)e);
+    ring_free(q);
+    if (nsets == 0)
+        return P3MK_OK;
+    q->ring = new (std::nothrow) hipEvent_t[(size_t)nsets * P3MK_NUM_EVENTS];
+    if (!q->ring)
+        return p3mk_fail(P3MK_ENOMEM, "event ring allocation failed", 0);
+    for (int i = 0; i < nsets * P3MK_NUM_EVENTS; i++) {
+        e = hipEventCreate(&q->ring[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; j++)
+                (void)hipEventDestroy(q->ring[j]);
+            delete[] q->ring;
+            q->ring = nullptr;
+            return p3mk_fail(P3MK_EDEVICE, "hipEventCreate", (int)e);
+        }
+    }
+    q->ring_sets = nsets;
+    q->ring_pos = 0;
+    return P3MK_OK;
+}
+
+extern "C" void p3mk_ring_advance(p3mk_queue *q) {
+    if (q->ring && q->ring_pos < q->ring_sets)
+        q->ring_pos++;
+}
+
+extern "C" int p3mk_ring_filled(p3mk_queue *q) { return q->ring ? q->ring_pos : 0; }
+
+extern "C" int p3mk_ring_elapsed_ms(p3mk_queue *q, int set, int a, int b, float *ms) {
+    if (!q->ring || set < 0 || set >= q->ring_pos || a < 0 || b < 0 || a >= P3MK_NUM_EVENTS ||
+        b >= P3MK_NUM_EVENTS)
+        return p3mk_fail(P3MK_EINVAL, "event ring slot out of range", 0);
+    hipError_t e = hipEventElapsedTime(ms, q->ring[set * P3MK_NUM_EVENTS + a],
+                                       q->ring[set * P3MK_NUM_EVENTS + b]);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipEventElapsedTime", (int)e);
     return P3MK_OK;
 }
 

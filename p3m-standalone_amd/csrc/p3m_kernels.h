@@ -36,6 +36,13 @@ void p3mk_queue_destroy(p3mk_queue *q);
 int p3mk_queue_sync(p3mk_queue *q);
 int p3mk_record(p3mk_queue *q, int slot);
 int p3mk_elapsed_ms(p3mk_queue *q, int a, int b, float *ms);
+/* Event ring: after p3mk_ring_start(q, n), every p3mk_record also records
+ * into set k (k = number of p3mk_ring_advance calls so far, while k < n);
+ * n = 0 removes the ring.  Sets are read with p3mk_ring_elapsed_ms. */
+int p3mk_ring_start(p3mk_queue *q, int nsets);
+void p3mk_ring_advance(p3mk_queue *q);
+int p3mk_ring_filled(p3mk_queue *q);
+int p3mk_ring_elapsed_ms(p3mk_queue *q, int set, int a, int b, float *ms);
 int p3mk_alloc(void **p, size_t bytes);
 void p3mk_release(void *p);
 int p3mk_alloc_host(void **p, size_t bytes);
