@@ -263,6 +263,23 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
 int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, data_t *d,
                              forces_t *f);
 
+/* Adjoint (vector-Jacobian product) of Kspace_force; no counterpart in the
+ * reference, whose path has no derivatives (the differentiable form of the
+ * k-space force asked of this library).  For a cotangent dF of the k-space
+ * force -- f_k plus, for ad / ad-r, the self-force term Kspace_force
+ * subtracts from f (p3m-ad-self-forces.c:114-139) -- adds
+ * sum_{i,d} dF_{i,d} dF_{i,d}/dx_j to dpos (3 arrays) and dF/dq_j to dq.
+ * The field meshes are recomputed from the positions (nothing of a forward
+ * call is stored).  Position derivatives are those of the B-spline weights at
+ * the quantised offsets the forward force uses (charge-assign.c:94-97). */
+int p3m_hip_kspace_force_vjp(system_t *s, parameters_t *p, data_t *d, const vector_array_t *dF,
+                             vector_array_t *dpos, double *dq);
+/* Device-resident adjoint: gF (3n, x/y/z blocks) is the cotangent; dpos
+ * (3n) and dq (n) are accumulated (+=).  Enqueued on the library's stream. */
+int p3m_hip_kspace_force_vjp_device(data_t *d, int n, double length, const double *x,
+                                    const double *y, const double *z, const double *q,
+                                    const double *gF, double *dpos, double *dq);
+
 /* Error estimates with a status (Error / Error_k of the method table):
  * realpart.c:343-346 with p3m-ik.c:234-286, p3m-ik-i.c:280-305,394-399,
  * p3m-ad.c:475-505, p3m-ad-i.c:237-279.  The k-space sums run on the GPU. */

@@ -91,6 +91,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libp3m_hip.so is not built; run `make -C %s`" % _HERE)
+        try:
+            # torch ships its own HIP runtime under the same soname
+            # (libamdhip64.so.7); whichever is loaded first serves the whole
+            # process, and torch fails ("No HIP GPUs") on another one, so
+            # torch is loaded first when present
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         i, d, vp, ip = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)
         sigs = {
@@ -110,6 +118,9 @@ def lib():
             "p3m_hip_error": (i, [i, _SP, _PP, _dp, _dp]),
             "p3m_hip_set_error_table": (i, [i, i, i, _dp]),
             "p3m_hip_stage_times": (i, [_DPT, _dp]),
+            "p3m_hip_kspace_force_vjp": (i, [_SP, _PP, _DPT, ctypes.POINTER(VectorArray),
+                                             ctypes.POINTER(VectorArray), _dp]),
+            "p3m_hip_kspace_force_vjp_device": (i, [_DPT, i, d, vp, vp, vp, vp, vp, vp, vp]),
             "p3m_hip_stage_ring_start": (i, [_DPT, i]),
             "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
             "p3m_hip_get_influence_function": (i, [_DPT, _dp]),
@@ -267,6 +278,21 @@ class Solver:
         t = np.zeros(5)
         check(lib().p3m_hip_stage_times(self.d, t.ctypes.data_as(_dp)))
         return t
+
+    def kspace_force_vjp(self, dF):
+        """Gradient of sum(dF * F_k) w.r.t. positions (3, n) and charges (n)
+        (host arrays; F_k includes the self-force term of methods 2 and 7)."""
+        n = self.system.n
+        gin = _Vec(*np.asarray(dF, dtype=np.float64).reshape(3, n))
+        z = np.zeros(n)
+        dpos = _Vec(z, z, z)
+        dq = np.zeros(n)
+        check(lib().p3m_hip_kspace_force_vjp(self.system.ptr, self.pptr, self.d, ctypes.byref(gin.c),
+                                             ctypes.byref(dpos.c), dq.ctypes.data_as(_dp)))
+        return dpos.numpy(), dq
+
+    def kspace_force_vjp_device(self, n, length, x, y, z, q, gF, dpos, dq):
+        check(lib().p3m_hip_kspace_force_vjp_device(self.d, n, length, x, y, z, q, gF, dpos, dq))
 
     def stage_ring_start(self, nsteps):
         """Record the stage events of the next `nsteps` device force calls."""

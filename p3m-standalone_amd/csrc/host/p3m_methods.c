@@ -89,6 +89,7 @@ typedef struct {
     float stage_ms[5];
     unsigned char *ring_rs; /* per event-ring set: real space ran */
     int ring_sets;
+    void *adj[3]; /* adjoint meshes, made by the first VJP call */
 } p3m_ctx;
 
 static p3m_ctx *ctx_of(data_t *d) { return d ? (p3m_ctx *)d->method_data : NULL; }
@@ -107,6 +108,8 @@ static void ctx_free(p3m_ctx *c) {
         if (c->F[k] != c->mesh)
             p3mk_release(c->F[k]);
     }
+    for (k = 0; k < 3; k++)
+        p3mk_release(c->adj[k]);
     p3mk_release(c->G);
     p3mk_release(c->Dn);
     p3mk_release(c->selfb);
@@ -294,44 +297,58 @@ int p3m_hip_set_precision(data_t *d, int precision) {
 /* Kspace_force                                                            */
 /* ---------------------------------------------------------------------- */
 
-static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, const double *z,
-                          const double *q, double *fk, double *f) {
+/* Assignment, forward FFT, influence-function multiply (ik: and gradient),
+ * backward FFT: leaves the field meshes c->F[0..2] (ik) or the potential
+ * mesh c->mesh (ad).  Records events 1 and 2. */
+static int field_enqueue(p3m_ctx *c, int n, const double *x, const double *y, const double *z,
+                         const double *q) {
     p3mk_fft *fft = ctx_fft(c);
     const p3mk_geom *g = &c->geom;
-    const double L = g->L;
-    TRYK(p3mk_record(c->q, 0), "event");
+    void *spec = is_r2c(c->method) ? c->spec[0] : c->mesh;
+    void *outs[3];
+    int k;
     TRYK(p3mk_assign(g, n, x, y, z, q, c->psort, c->mesh, c->q), "charge assignment");
     TRYK(p3mk_record(c->q, 1), "event");
+    TRYK(p3mk_fft_forward_zy(fft, c->mesh, spec, c->q), "forward FFT");
     if (!is_ad(c->method)) {
         /* ik: F_d = i (2 pi/L) Dn_d G Q^ (p3m-ik.c:196-216, p3m-ik-i.c:187-212,
          * p3m-ik-real.c:98-121), three backward transforms */
-        void *spec = is_r2c(c->method) ? c->spec[0] : c->mesh;
-        void *outs[3];
-        int k;
         for (k = 0; k < 3; k++)
             outs[k] = is_r2c(c->method) ? c->spec[k] : c->F[k];
-        TRYK(p3mk_fft_forward_zy(fft, c->mesh, spec, c->q), "forward FFT");
         TRYK(p3mk_convolve(fft, 3, c->G, c->Dn, spec, outs, c->q), "convolution");
         for (k = 0; k < 3; k++)
             TRYK(p3mk_fft_backward_yz(fft, outs[k], c->F[k], c->q), "backward FFT");
-        TRYK(p3mk_record(c->q, 2), "event");
-        /* assign_forces* with prefactor 1/(2 L^3) (p3m-ik.c:227) */
-        TRYK(p3mk_interp_ik(g, n, c->psort, c->F, 1.0 / (2.0 * L * L * L), fk, c->q),
-             "back-interpolation");
     } else {
         /* ad: phi = FFT^-1[G Q^] (p3m-ad.c:167-181, p3m-ad-i.c:161-175,
          * p3m-ad-real.c:75-89) */
-        void *spec = is_r2c(c->method) ? c->spec[0] : c->mesh;
-        void *outs[3];
         outs[0] = spec;
         outs[1] = outs[2] = NULL;
-        TRYK(p3mk_fft_forward_zy(fft, c->mesh, spec, c->q), "forward FFT");
         TRYK(p3mk_convolve(fft, 1, c->G, c->Dn, spec, outs, c->q), "convolution");
         TRYK(p3mk_fft_backward_yz(fft, spec, c->mesh, c->q), "backward FFT");
-        TRYK(p3mk_record(c->q, 2), "event");
-        /* assign_forces_ad* with prefactor M/L^3 (p3m-ad.c:187) and the
-         * self-force subtraction of methods 2 and 7 (p3m-ad.c:189-191) */
-        TRYK(p3mk_interp_ad(g, n, c->psort, c->mesh, (double)g->M / (L * L * L),
+    }
+    TRYK(p3mk_record(c->q, 2), "event");
+    return P3M_HIP_OK;
+}
+
+/* prefactor of the back-interpolation sums: 1/(2 L^3) for ik (p3m-ik.c:227),
+ * M/L^3 for ad (p3m-ad.c:187) */
+static double interp_prefactor(const p3m_ctx *c) {
+    const double L = c->geom.L;
+    return is_ad(c->method) ? (double)c->geom.M / (L * L * L) : 1.0 / (2.0 * L * L * L);
+}
+
+static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, const double *z,
+                          const double *q, double *fk, double *f) {
+    const p3mk_geom *g = &c->geom;
+    TRYK(p3mk_record(c->q, 0), "event");
+    TRY(field_enqueue(c, n, x, y, z, q));
+    if (!is_ad(c->method)) {
+        TRYK(p3mk_interp_ik(g, n, c->psort, c->F, interp_prefactor(c), fk, c->q),
+             "back-interpolation");
+    } else {
+        /* assign_forces_ad* and the self-force subtraction of methods 2 and 7
+         * (p3m-ad.c:189-191) */
+        TRYK(p3mk_interp_ad(g, n, c->psort, c->mesh, interp_prefactor(c),
                             has_self_force(c->method) ? c->selfb : NULL, fk, f, c->q),
              "back-interpolation");
     }
@@ -820,6 +837,118 @@ int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, do
         *kspace = k;
     if (total)
         *total = sqrt(r * r + k * k);
+    return P3M_HIP_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Adjoint of Kspace_force                                                 */
+/* ---------------------------------------------------------------------- */
+
+static int ensure_adjoint_buffers(p3m_ctx *c) {
+    p3mk_geom g64 = c->geom;
+    int k, nb = is_ad(c->method) ? 1 : 3;
+    g64.fp32 = 0;
+    for (k = 0; k < nb; k++)
+        if (c->adj[k] == NULL)
+            TRYK(p3mk_alloc(&c->adj[k], p3mk_mesh_bytes(&g64)), "adjoint mesh");
+    return P3M_HIP_OK;
+}
+
+/* The field meshes are recomputed from the positions (no forward state is
+ * kept), then the cotangent is assigned, transformed with the transposed
+ * k-space operator and gathered together with the fields. */
+int p3m_hip_kspace_force_vjp_device(data_t *d, int n, double length, const double *x,
+                                    const double *y, const double *z, const double *q,
+                                    const double *gF, double *dpos, double *dq) {
+    p3m_ctx *c = ctx_of(d);
+    p3mk_fft *fft;
+    const p3mk_geom *g;
+    const double L = length;
+    void *in[3];
+    int k;
+    if (c == NULL)
+        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+    if (n != c->n)
+        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
+    if (length != c->geom.L)
+        return fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
+    if (!c->have_G)
+        return fail(P3M_HIP_EINVAL, "influence function not computed");
+    if (gF == NULL || dpos == NULL || dq == NULL)
+        return fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
+    if (n == 0)
+        return P3M_HIP_OK;
+    TRY(ensure_adjoint_buffers(c));
+    fft = ctx_fft(c);
+    g = &c->geom;
+    TRYK(p3mk_record(c->q, 0), "event");
+    TRY(field_enqueue(c, n, x, y, z, q));
+    if (!is_ad(c->method)) {
+        /* psi_d = sum_i q_i g_{i,d} W_i; lambda = sum_d K_d^T psi_d */
+        for (k = 0; k < 3; k++) {
+            in[k] = is_r2c(c->method) ? c->spec[k] : c->adj[k];
+            TRYK(p3mk_assign_weighted(g, n, c->psort, 1, gF + (size_t)k * n, c->adj[k], c->q),
+                 "cotangent assignment");
+            TRYK(p3mk_fft_forward_zy(fft, c->adj[k], in[k], c->q), "forward FFT");
+        }
+        TRYK(p3mk_divergence(fft, c->G, c->Dn, in, in[0], c->q), "adjoint multiply");
+        TRYK(p3mk_fft_backward_yz(fft, in[0], c->adj[0], c->q), "backward FFT");
+        TRYK(p3mk_adjoint_gather(g, 0, n, c->psort, q, gF, c->F, c->adj[0], interp_prefactor(c),
+                                 NULL, dpos, dq, c->q),
+             "adjoint gather");
+    } else {
+        /* psi = sum_i q_i sum_d g_{i,d} W^(d)_i; lambda = K psi */
+        void *F[3];
+        void *outs[3];
+        in[0] = is_r2c(c->method) ? c->spec[0] : c->adj[0];
+        outs[0] = in[0];
+        outs[1] = outs[2] = NULL;
+        TRYK(p3mk_assign_weighted(g, n, c->psort, 2, gF, c->adj[0], c->q), "cotangent assignment");
+        TRYK(p3mk_fft_forward_zy(fft, c->adj[0], in[0], c->q), "forward FFT");
+        TRYK(p3mk_convolve(fft, 1, c->G, c->Dn, in[0], outs, c->q), "adjoint multiply");
+        TRYK(p3mk_fft_backward_yz(fft, in[0], c->adj[0], c->q), "backward FFT");
+        F[0] = c->mesh;
+        F[1] = F[2] = NULL;
+        /* the ad derivative weights carry 1/L (p3m-ad.c:187 with
+         * charge-assign.c:1123-1125) */
+        TRYK(p3mk_adjoint_gather(g, 1, n, c->psort, q, gF, F, c->adj[0], interp_prefactor(c) / L,
+                                 has_self_force(c->method) ? c->selfb : NULL, dpos, dq, c->q),
+             "adjoint gather");
+    }
+    TRYK(p3mk_record(c->q, 3), "event");
+    c->have_ks_events = 1;
+    return P3M_HIP_OK;
+}
+
+int p3m_hip_kspace_force_vjp(system_t *s, parameters_t *p, data_t *d, const vector_array_t *dF,
+                             vector_array_t *dpos, double *dq) {
+    p3m_ctx *c = ctx_of(d);
+    size_t n, b;
+    int dim;
+    if (c == NULL)
+        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+    TRY(check_params(c->method, s, p));
+    TRY(check_host_system(s));
+    if (dF == NULL || dpos == NULL || dq == NULL)
+        return fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
+    if (s->nparticles != c->n)
+        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
+    n = (size_t)s->nparticles;
+    if (n == 0)
+        return P3M_HIP_OK;
+    b = sizeof(double) * n;
+    TRY(upload_positions(c, s));
+    for (dim = 0; dim < 3; dim++)
+        TRYK(p3mk_copy_h2d(c->force + dim * n, field((vector_array_t *)dF, dim), b, c->q), "upload");
+    TRYK(p3mk_fill_zero(c->force + 3 * n, sizeof(double) * 4 * n, c->q), "zero");
+    TRY(p3m_hip_kspace_force_vjp_device(d, (int)n, s->length, c->pos, c->pos + n, c->pos + 2 * n,
+                                        c->pos + 3 * n, c->force, c->force + 3 * n,
+                                        c->force + 6 * n));
+    TRYK(p3mk_copy_d2h(c->hforce, c->force + 3 * n, sizeof(double) * 4 * n, c->q), "download");
+    TRYK(p3mk_queue_sync(c->q), "sync");
+    for (dim = 0; dim < 3; dim++)
+        add_host(field(dpos, dim), c->hforce + dim * n, n);
+    add_host(dq, c->hforce + 3 * n, n);
     return P3M_HIP_OK;
 }
 

@@ -370,6 +370,50 @@ k_convolve_x(const c2_t<T> *in, c2_t<T> *out0, c2_t<T> *out1, c2_t<T> *out2, Lin
     }
 }
 
+// Adjoint of the ik gradient step: x-axis forward pass of three spectra,
+// out = sum_d (-i (2 pi / L) Dn[n_d] G) X_d, x-axis backward pass.  The
+// multiplier of the forward gradient, i (2 pi / L) Dn G, is odd and purely
+// imaginary, so its transpose as a real operator is its negative.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_divergence_x(const c2_t<T> *in0, const c2_t<T> *in1, const c2_t<T> *in2, c2_t<T> *out, LineMap m,
+               int W, Radices rp, const c2_t<T> *__restrict__ tw, const double *__restrict__ G,
+               const double *__restrict__ Dn, double twopi_L, int lt) {
+    using V = c2_t<T>;
+    extern __shared__ unsigned char smem[];
+    const int n = m.n, pitch = n + 1;
+    V *A = reinterpret_cast<V *>(smem);
+    V *B = A + lt * pitch;
+    V *Cb = B + lt * pitch;
+    const long L0 = (long)blockIdx.x * lt;
+    const V *ins[3] = {in0, in1, in2};
+    const int M = n;
+    for (int d = 0; d < 3; d++) {
+        load_tile(ins[d], A, m, L0, lt, pitch, false);
+        __syncthreads();
+        const V *X = fft_tile(A, B, lt, n, pitch, rp, tw, 0);
+        for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
+            const int e = idx / lt, l = idx - e * lt;
+            const long L = L0 + l;
+            if (L < m.nlines) {
+                const int j = (int)(L / W), k = (int)(L - (long)j * W);
+                const int nd = (d == 0) ? e : ((d == 1) ? j : k);
+                const double f = twopi_L * Dn[nd] * G[((long)e * M + j) * M + k];
+                const V v = X[l * pitch + e];
+                V w;
+                w.x = (T)(f * (double)v.y);
+                w.y = (T)(-f * (double)v.x);
+                if (d > 0)
+                    w = cadd(w, Cb[l * pitch + e]);
+                Cb[l * pitch + e] = w;
+            }
+        }
+        __syncthreads();
+    }
+    V *r = fft_tile(Cb, A, lt, n, pitch, rp, tw, 1);
+    store_tile(out, r, m, L0, lt, pitch, false);
+}
+
 bool make_radices(int n, Radices &rp) {
     rp.n = 0;
     static const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
@@ -553,6 +597,24 @@ int convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spe
     return p3mk_launch_check("convolve");
 }
 
+template <typename T>
+int divergence(p3mk_fft *f, const double *G, const double *Dn, void *const in[3], void *out,
+               p3mk_queue *q) {
+    using V = c2_t<T>;
+    const int M = f->M;
+    const int W = f->g.r2c ? f->H : M;
+    const long nl = (long)M * W;
+    LineMap x = {M, nl, nl, 0, nl};
+    const int lt = tile_lines<T>(M, 3);
+    const size_t lds = (size_t)3 * lt * (M + 1) * sizeof(V);
+    const c2_t<T> *tw = sizeof(T) == 8 ? (const V *)f->tw64 : (const V *)f->tw32;
+    const double twopi_L = 2.0 * 3.14159265358979323846 / f->g.L;
+    hipLaunchKernelGGL(k_divergence_x<T>, dim3((unsigned)((nl + lt - 1) / lt)), dim3(kThreads), lds,
+                       p3mk_stream(q), (const V *)in[0], (const V *)in[1], (const V *)in[2],
+                       (V *)out, x, W, f->rp, tw, G, Dn, twopi_L, lt);
+    return p3mk_launch_check("divergence");
+}
+
 }  // namespace
 
 extern "C" int p3mk_fft_forward_zy(p3mk_fft *f, void *mesh, void *spec, p3mk_queue *q) {
@@ -569,6 +631,12 @@ extern "C" int p3mk_convolve(p3mk_fft *f, int nout, const double *G, const doubl
         return p3mk_fail(P3MK_EINVAL, "convolve: nout must be 1 or 3", 0);
     return f->g.fp32 ? convolve<float>(f, nout, G, Dn, spec, out, q)
                      : convolve<double>(f, nout, G, Dn, spec, out, q);
+}
+
+extern "C" int p3mk_divergence(p3mk_fft *f, const double *G, const double *Dn, void *const in[3],
+                               void *out, p3mk_queue *q) {
+    return f->g.fp32 ? divergence<float>(f, G, Dn, in, out, q)
+                     : divergence<double>(f, G, Dn, in, out, q);
 }
 
 template <typename T> static int fft3d(p3mk_fft *f, void *data, int sign, p3mk_queue *q) {

@@ -84,11 +84,18 @@ __global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *_
 // returned value to wait for), and the FP32 form's products are summed
 // without rounding error growth; the mesh itself is stored in the form's
 // precision.
-template <int P, typename T, bool R2C, bool IL>
+// What a particle deposits (MODE): kCharge q W; kWeighted q g_i W with a
+// per-particle factor g (the adjoint's q g_d); kDipole
+// q sum_d g_{i,d} W^(d), W^(d) = W with the d-axis weight replaced by its
+// derivative (the adjoint of ad back-interpolation).
+enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
+
+template <int P, typename T, bool R2C, bool IL, int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
               const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
-              T *__restrict__ mesh) {
+              T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
+              int n) {
     extern __shared__ unsigned char smem_raw[];
     double *acc = reinterpret_cast<double *>(smem_raw);
     constexpr int NM = IL ? 2 : 1;
@@ -119,12 +126,46 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
     __syncthreads();
     for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         const double4 r = sorted[p];
-        const T qv = (T)r.w;
+        T qv = (T)r.w;
+        T g0 = T(0), g1 = T(0), g2 = T(0);
+        if (MODE == kWeighted) {
+            qv = (T)(r.w * gv[perm[p]]);
+        } else if (MODE == kDipole) {
+            const int i = perm[p];
+            g0 = (T)gv[i];
+            g1 = (T)gv[(long)n + i];
+            g2 = (T)gv[2L * n + i];
+        }
 #pragma unroll
         for (int ii = 0; ii < NM; ii++) {
             int b0, b1, b2;
             T w0[P], w1[P], w2[P];
             const double half = 0.5 * ii;
+            if (MODE == kDipole) {
+                // W^(d) summed with g_d: fold g_d into the derivative weights
+                // of axis d, then deposit (d0 w1 w2 + w0 d1 w2 + w0 w1 d2)
+                T d0[P], d1[P], d2[P];
+                axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
+                axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
+                axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
+                int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+                o0 += (o0 < 0) ? M : 0;
+                o1 += (o1 < 0) ? M : 0;
+                o2 += (o2 < 0) ? M : 0;
+                double *a = acc + ii * npts;
+                for (int a0 = 0; a0 < P; a0++) {
+#pragma unroll
+                    for (int a1 = 0; a1 < P; a1++) {
+                        const T u = qv * (g0 * d0[a0] * w1[a1] + g1 * w0[a0] * d1[a1]);
+                        const T v = qv * (g2 * w0[a0] * w1[a1]);
+                        const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
+#pragma unroll
+                        for (int a2 = 0; a2 < P; a2++)
+                            atomicAdd(&a[row + a2], (double)(u * w2[a2] + v * d2[a2]));
+                    }
+                }
+                continue;
+            }
             axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
             axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
             axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
@@ -303,6 +344,173 @@ k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
     }
 }
 
+// ---- adjoint (vector-Jacobian product) of the k-space force ---------------
+//
+// ik: f_{i,d} = -s q_i sum_m W_i(m) phi_d(m), phi_d = K_d Q, Q = sum_j q_j W_j,
+// s = prefac (x 0.5 per mesh when interlaced).  For a cotangent g_{i,d}:
+//   dq_j     = -s      sum_m W_j(m)    H_j(m)
+//   dx_{j,e} = -s q_j  sum_m d_e W_j(m) H_j(m),
+//   H_j(m)   = sum_d g_{j,d} phi_d(m) + lambda(m),
+//   lambda   = sum_d K_d^T psi_d,  psi_d = sum_i q_i g_{i,d} W_i.
+// ad: f_{i,d} = -s q_i sum_m W^(d)_i(m) phi(m), phi = K Q (K symmetric):
+//   dq_j     = -s     sum_m [sum_d g_{j,d} W^(d)_j phi + W_j lambda]
+//   dx_{j,e} = -s q_j sum_m [sum_d g_{j,d} d_e W^(d)_j phi + d_e W_j lambda],
+//   lambda   = K psi,  psi = sum_i q_i sum_d g_{i,d} W^(d)_i.
+// Derivatives are those of the B-spline weights at the quantised offsets
+// (d/dx = Hi d/du); outputs accumulate (+=) in the caller's order.
+template <int P, typename T, bool R2C, bool IL>
+__global__ void __launch_bounds__(kThreads)
+k_adjoint_ik(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
+             const double *__restrict__ q, const double *__restrict__ gv, double Hi, int M,
+             const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
+             const T *__restrict__ lam, double prefac, double *__restrict__ dpos,
+             double *__restrict__ dq) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    constexpr int NM = IL ? 2 : 1;
+    const int i = perm[t];
+    const double4 r = sorted[t];
+    const double qi = q[i];
+    const T g0 = (T)gv[i], g1 = (T)gv[(long)n + i], g2 = (T)gv[2L * n + i];
+    Acc<T> aq, ax, ay, az;
+    for (int ii = 0; ii < NM; ii++) {
+        int b0, b1, b2;
+        T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
+        const double half = 0.5 * ii;
+        axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
+        axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
+        axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
+        int kk[P];
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            kk[c] = wrap1(b2 + c, M);
+        for (int a = 0; a < P; a++) {
+            const int ia = wrap1(b0 + a, M);
+#pragma unroll
+            for (int b = 0; b < P; b++) {
+                const int jb = wrap1(b1 + b, M);
+                const long row = ((long)ia * M + jb) * M;
+                const T wab = w0[a] * w1[b], dab = d0[a] * w1[b], adb = w0[a] * d1[b];
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const long idx = row + kk[c];
+                    const T H = g0 * mesh_val<T, R2C>(F0, idx, ii) + g1 * mesh_val<T, R2C>(F1, idx, ii) +
+                                g2 * mesh_val<T, R2C>(F2, idx, ii) + mesh_val<T, R2C>(lam, idx, ii);
+                    aq.add(wab * w2[c] * H);
+                    ax.add(dab * w2[c] * H);
+                    ay.add(adb * w2[c] * H);
+                    az.add(wab * d2[c] * H);
+                }
+            }
+        }
+    }
+    const double s = IL ? 0.5 * prefac : prefac;
+    dq[i] += -s * aq.value();
+    dpos[i] += -s * qi * Hi * ax.value();
+    dpos[(long)n + i] += -s * qi * Hi * ay.value();
+    dpos[2L * n + i] += -s * qi * Hi * az.value();
+}
+
+template <int P, typename T, bool R2C, bool IL>
+__global__ void __launch_bounds__(kThreads)
+k_adjoint_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
+             const double *__restrict__ q, const double *__restrict__ gv, double Hi, int M,
+             const T *__restrict__ phi, const T *__restrict__ lam, double prefac,
+             const double *__restrict__ selfb, double h, double *__restrict__ dpos,
+             double *__restrict__ dq) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    constexpr int NM = IL ? 2 : 1;
+    const int i = perm[t];
+    const double4 r = sorted[t];
+    const double qi = q[i];
+    const double gd0 = gv[i], gd1 = gv[(long)n + i], gd2 = gv[2L * n + i];
+    const T g0 = (T)gd0, g1 = (T)gd1, g2 = (T)gd2;
+    Acc<T> aq, ax, ay, az;
+    for (int ii = 0; ii < NM; ii++) {
+        int b0, b1, b2;
+        T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P], e0[P], e1[P], e2[P];
+        double xq;
+        const double half = 0.5 * ii;
+        axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
+        axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
+        axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
+        // second derivatives at the same quantised offsets
+        locate(r.x, Hi, (double)((P - 1) / 2), half, M, b0, xq);
+        Bspline<P, T>::d2w((T)xq, e0);
+        locate(r.y, Hi, (double)((P - 1) / 2), half, M, b1, xq);
+        Bspline<P, T>::d2w((T)xq, e1);
+        locate(r.z, Hi, (double)((P - 1) / 2), half, M, b2, xq);
+        Bspline<P, T>::d2w((T)xq, e2);
+        int kk[P];
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            kk[c] = wrap1(b2 + c, M);
+        for (int a = 0; a < P; a++) {
+            const int ia = wrap1(b0 + a, M);
+#pragma unroll
+            for (int b = 0; b < P; b++) {
+                const int jb = wrap1(b1 + b, M);
+                const long row = ((long)ia * M + jb) * M;
+                // x-y factors of the stencil terms (z factor applied per c)
+                const T ww = w0[a] * w1[b], dw_ = d0[a] * w1[b], wd = w0[a] * d1[b];
+                const T ew = e0[a] * w1[b], dd = d0[a] * d1[b], we = w0[a] * e1[b];
+                // sum_d g_d W^(d) = (g0 dw_ + g1 wd) w2 + g2 ww d2
+                const T s_w = g0 * dw_ + g1 * wd, s_d = g2 * ww;
+                // d_x: (g0 ew + g1 dd) w2 + g2 dw_ d2 ; d_y: (g0 dd + g1 we) w2 + g2 wd d2
+                // d_z: (g0 dw_ + g1 wd) d2 + g2 ww e2
+                const T xw = g0 * ew + g1 * dd, xd = g2 * dw_;
+                const T yw = g0 * dd + g1 * we, yd = g2 * wd;
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const long idx = row + kk[c];
+                    const T ph = mesh_val<T, R2C>(phi, idx, ii);
+                    const T la = mesh_val<T, R2C>(lam, idx, ii);
+                    aq.add((s_w * w2[c] + s_d * d2[c]) * ph + ww * w2[c] * la);
+                    ax.add((xw * w2[c] + xd * d2[c]) * ph + dw_ * w2[c] * la);
+                    ay.add((yw * w2[c] + yd * d2[c]) * ph + wd * w2[c] * la);
+                    az.add((s_w * d2[c] + s_d * e2[c]) * ph + ww * d2[c] * la);
+                }
+            }
+        }
+    }
+    const double s = IL ? 0.5 * prefac : prefac;
+    double dqi = -s * aq.value();
+    double dx = -s * qi * Hi * ax.value(), dy = -s * qi * Hi * ay.value(),
+           dz = -s * qi * Hi * az.value();
+    if (selfb != nullptr) {
+        // f_{i,d} -= q_i^2 sum_m' b_d(m') sin(theta), theta = 2 pi m'.r / h
+        const double c0 = 2 * 3.14159265358979323846 / h;
+        const double q2 = qi * qi;
+        double ss = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        int ind = 0;
+        for (int m0 = -1; m0 <= 1; m0++)
+            for (int m1 = -1; m1 <= 1; m1++)
+                for (int m2 = -1; m2 <= 1; m2++) {
+                    const double th = 2 * 3.14159265358979323846 *
+                                      (m0 * r.x / h + m1 * r.y / h + m2 * r.z / h);
+                    const double gb = gd0 * selfb[ind] + gd1 * selfb[ind + 1] + gd2 * selfb[ind + 2];
+                    double sn, cs;
+                    sincos(th, &sn, &cs);
+                    ss += gb * sn;
+                    cx += gb * cs * m0;
+                    cy += gb * cs * m1;
+                    cz += gb * cs * m2;
+                    ind += 3;
+                }
+        dqi -= 2.0 * qi * ss;
+        dx -= q2 * c0 * cx;
+        dy -= q2 * c0 * cy;
+        dz -= q2 * c0 * cz;
+    }
+    dq[i] += dqi;
+    dpos[i] += dx;
+    dpos[(long)n + i] += dy;
+    dpos[2L * n + i] += dz;
+}
+
 // tile side: 8 mesh points (LDS region (8 + P)^3 per mesh)
 int tile_side(int M) { return M < 8 ? M : 8; }
 
@@ -364,8 +572,9 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
 
 namespace {
 
-template <typename T, bool R2C>
-int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, p3mk_queue *qu) {
+template <typename T, bool R2C, int MODE>
+int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const double *gv,
+                  p3mk_queue *qu) {
     const double Hi = (double)g->M / g->L;
     const int NM = g->interlaced ? 2 : 1;
     const int R = s->T + g->cao - 1 + (g->interlaced ? 1 : 0);
@@ -382,13 +591,13 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, p3mk_que
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, true>), grid, dim3(kThreads), lds, st,    \
-                               s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M, Hi,   \
-                               m);                                                                 \
+            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
+                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               Hi, m, s->perm, gv, n);                                             \
         else                                                                                       \
-            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, false>), grid, dim3(kThreads), lds, st,   \
-                               s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M, Hi,   \
-                               m);                                                                 \
+            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
+                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               Hi, m, s->perm, gv, n);                                             \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -453,6 +662,67 @@ int launch_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void 
     return p3mk_launch_check("interp_ad");
 }
 
+template <typename T, bool R2C>
+int assign_mode_t(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, int mode, const double *gv,
+                  p3mk_queue *qu) {
+    switch (mode) {
+    case kCharge: return launch_assign<T, R2C, kCharge>(g, n, s, mesh, gv, qu);
+    case kWeighted: return launch_assign<T, R2C, kWeighted>(g, n, s, mesh, gv, qu);
+    default: return launch_assign<T, R2C, kDipole>(g, n, s, mesh, gv, qu);
+    }
+}
+
+int assign_mode(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, int mode, const double *gv,
+                p3mk_queue *qu) {
+    if (g->fp32)
+        return g->r2c ? assign_mode_t<float, true>(g, n, s, mesh, mode, gv, qu)
+                      : assign_mode_t<float, false>(g, n, s, mesh, mode, gv, qu);
+    return g->r2c ? assign_mode_t<double, true>(g, n, s, mesh, mode, gv, qu)
+                  : assign_mode_t<double, false>(g, n, s, mesh, mode, gv, qu);
+}
+
+template <typename T, bool R2C>
+int launch_adjoint(const p3mk_geom *g, int ad, int n, const p3mk_psort *s, const double *q,
+                   const double *gv, void *const F[3], const void *lam, double prefac,
+                   const double *selfb, double *dpos, double *dq, p3mk_queue *qu) {
+    const double Hi = (double)g->M / g->L;
+    const double h = g->L / g->M;
+    dim3 grid((n + kThreads - 1) / kThreads);
+    const T *F0 = (const T *)F[0], *F1 = (const T *)F[1], *F2 = (const T *)F[2];
+    const T *la = (const T *)lam;
+    hipStream_t st = p3mk_stream(qu);
+    const bool il = g->interlaced != 0;
+    switch (g->cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        if (ad) {                                                                                  \
+            if (il)                                                                                \
+                hipLaunchKernelGGL((k_adjoint_ad<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n, \
+                                   s->sorted, s->perm, q, gv, Hi, g->M, F0, la, prefac, selfb, h,  \
+                                   dpos, dq);                                                      \
+            else                                                                                   \
+                hipLaunchKernelGGL((k_adjoint_ad<P, T, R2C, false>), grid, dim3(kThreads), 0, st,  \
+                                   n, s->sorted, s->perm, q, gv, Hi, g->M, F0, la, prefac, selfb,  \
+                                   h, dpos, dq);                                                   \
+        } else {                                                                                   \
+            if (il)                                                                                \
+                hipLaunchKernelGGL((k_adjoint_ik<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n, \
+                                   s->sorted, s->perm, q, gv, Hi, g->M, F0, F1, F2, la, prefac,    \
+                                   dpos, dq);                                                      \
+            else                                                                                   \
+                hipLaunchKernelGGL((k_adjoint_ik<P, T, R2C, false>), grid, dim3(kThreads), 0, st,  \
+                                   n, s->sorted, s->perm, q, gv, Hi, g->M, F0, F1, F2, la, prefac, \
+                                   dpos, dq);                                                      \
+        }                                                                                          \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+    default:
+        return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
+    }
+    return p3mk_launch_check("adjoint");
+}
+
 }  // namespace
 
 extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const double *y,
@@ -481,9 +751,19 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
                        s->count, s->ntiles, s->chunks);
     P3MK_TRY(p3mk_launch_check("tile chunks"));
     P3MK_TRY(p3mk_exclusive_scan(s->chunks, s->ntiles, s->cstart, s->scan_scratch, qu));
-    if (g->fp32)
-        return g->r2c ? launch_assign<float, true>(g, n, s, mesh, qu) : launch_assign<float, false>(g, n, s, mesh, qu);
-    return g->r2c ? launch_assign<double, true>(g, n, s, mesh, qu) : launch_assign<double, false>(g, n, s, mesh, qu);
+    return assign_mode(g, n, s, mesh, kCharge, nullptr, qu);
+}
+
+extern "C" int p3mk_assign_weighted(const p3mk_geom *g, int n, p3mk_psort *s, int mode,
+                                    const double *gv, void *mesh, p3mk_queue *qu) {
+    if (n != s->n || g->M != s->M)
+        return p3mk_fail(P3MK_EINVAL, "weighted assignment needs the sort of the same particles", 0);
+    if (mode != 1 && mode != 2)
+        return p3mk_fail(P3MK_EINVAL, "weighted assignment mode must be 1 or 2", 0);
+    P3MK_TRY(p3mk_fill_zero(mesh, p3mk_mesh_bytes(g), qu));
+    if (n <= 0)
+        return P3MK_OK;
+    return assign_mode(g, n, s, mesh, mode, gv, qu);
 }
 
 extern "C" int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
@@ -513,4 +793,19 @@ extern "C" int p3mk_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, co
                       : launch_interp_ad<float, false>(g, n, s, phi, prefac, selfb, fk, f, qu);
     return g->r2c ? launch_interp_ad<double, true>(g, n, s, phi, prefac, selfb, fk, f, qu)
                   : launch_interp_ad<double, false>(g, n, s, phi, prefac, selfb, fk, f, qu);
+}
+
+extern "C" int p3mk_adjoint_gather(const p3mk_geom *g, int ad, int n, const p3mk_psort *s,
+                                   const double *q, const double *gv, void *const F[3],
+                                   const void *lam, double prefac, const double *selfb,
+                                   double *dpos, double *dq, p3mk_queue *qu) {
+    if (n <= 0)
+        return P3MK_OK;
+    if (n != s->n)
+        return p3mk_fail(P3MK_EINVAL, "adjoint needs the sort of the same particles", 0);
+    if (g->fp32)
+        return g->r2c ? launch_adjoint<float, true>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu)
+                      : launch_adjoint<float, false>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu);
+    return g->r2c ? launch_adjoint<double, true>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu)
+                  : launch_adjoint<double, false>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu);
 }

@@ -85,6 +85,11 @@ int p3mk_fft_forward_zy(p3mk_fft *f, void *mesh, void *spec, p3mk_queue *q);
  * ik (nout = 3): out[d] <- i (2 pi / L) Dn[n_d] G * spec, d = 0, 1, 2 */
 int p3mk_convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spec,
                   void *const out[3], p3mk_queue *q);
+/* transpose of the ik gradient: x-axis forward pass of the three spectra
+ * in[d] (after p3mk_fft_forward_zy), out <- sum_d -i (2 pi / L) Dn[n_d] G in[d],
+ * x-axis backward pass (out may equal in[0]) */
+int p3mk_divergence(p3mk_fft *f, const double *G, const double *Dn, void *const in[3], void *out,
+                    p3mk_queue *q);
 /* backward transform along y and z: c2c in place (mesh == spec);
  * r2c spec -> real mesh */
 int p3mk_fft_backward_yz(p3mk_fft *f, void *spec, void *mesh, p3mk_queue *q);
@@ -103,6 +108,20 @@ void p3mk_psort_destroy(p3mk_psort *s);
 int p3mk_assign(const p3mk_geom *g, int n, const double *x, const double *y, const double *z,
                 const double *q, p3mk_psort *s, void *mesh, p3mk_queue *qu);
 /* ik back-interpolation: fk[d*n + i] += -prefac * sum W F_d (meshes F[0..2]) */
+/* assignment with the sort of the last p3mk_assign (same particles, n):
+ * mode 1: particle i deposits q_i gv[i] W_i; mode 2: q_i sum_d gv[d n + i]
+ * W^(d)_i, W^(d) = W with axis d's weight replaced by its derivative in mesh
+ * units.  Zero-fills the mesh first. */
+int p3mk_assign_weighted(const p3mk_geom *g, int n, p3mk_psort *s, int mode, const double *gv,
+                         void *mesh, p3mk_queue *q);
+/* adjoint gather of the k-space force for the cotangent gv (3n, caller's
+ * order): dq (n) and dpos (3n) += (-prefac') sums over the stencil of the
+ * field meshes F (ik: 3 meshes; ad: F[0] = phi) and lambda; for ad, prefac
+ * includes the 1/L of the derivative weights and selfb (may be NULL) adds
+ * the self-force term. */
+int p3mk_adjoint_gather(const p3mk_geom *g, int ad, int n, const p3mk_psort *s, const double *q,
+                        const double *gv, void *const F[3], const void *lam, double prefac,
+                        const double *selfb, double *dpos, double *dq, p3mk_queue *q_);
 int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
                    double prefac, double *fk, p3mk_queue *qu);
 /* ad back-interpolation: fk -= prefac * sum phi dQ; with selfb (81 FP64 on

@@ -70,7 +70,7 @@ def test_weight_identities(orc, cao):
 def test_generated_device_polynomials_match_oracle(orc, cao):
     """The integer-coefficient Horner polynomials compiled into the kernels
     equal the oracle's weights at every quantised table offset."""
-    D, w, dw = gen_bspline.integer_tables(cao)
+    D, w, dw, d2w = gen_bspline.integer_tables(cao)
     args = np.arange(0, 131073, 997)
     xs = (args - 65536) / 131072.0
     for j in range(cao):
@@ -81,6 +81,15 @@ def test_generated_device_polynomials_match_oracle(orc, cao):
             dv = np.polynomial.polynomial.polyval(xs, np.array(dw[j], dtype=float)) / D
             dref = np.array([orc.weight_d(j, x, cao) for x in xs])
             assert np.max(np.abs(dv - dref)) < 1e-13
+        if cao >= 3:
+            # second derivative (the ad adjoint's) vs central differences of
+            # the oracle's first derivative inside the piece
+            h = 1e-5
+            xi = xs[(xs > -0.5 + h) & (xs < 0.5 - h)]
+            d2v = np.polynomial.polynomial.polyval(xi, np.array(d2w[j], dtype=float)) / D
+            fd = np.array([(orc.weight_d(j, x + h, cao) - orc.weight_d(j, x - h, cao)) / (2 * h)
+                           for x in xi])
+            assert np.max(np.abs(d2v - fd)) < 1e-6
 
 
 def test_locate_quantisation(orc):
