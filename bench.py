@@ -37,6 +37,11 @@ WORKLOADS = {
                                gen="normal", seed=5),
 }
 DEFAULT = "gaussian-1M-ad-i-m128"
+# many independent systems in one batch (evaluated with one launch per stage)
+BATCH_WORKLOAD = "batch-256-ad-i"
+# forward k-space force plus its adjoint (vector-Jacobian product)
+ADJOINT_WORKLOAD = "adjoint-32k-ik-m32"
+SMOOTH_EVEN_16_48 = [16, 18, 20, 22, 24, 28, 30, 32, 36, 40, 42, 44, 48]
 
 
 def make_system(w):
@@ -110,6 +115,7 @@ def run_form(pkg, torch, w, system, dev, precision, steps, warmup):
     solver.stage_ring_start(0)
     st = np.median(ring, axis=0)
     forces = f.cpu().numpy().reshape(3, N)
+    kforces = fk.cpu().numpy().reshape(3, N)
     solver.close()
     return {
         "ms_per_step": 1e3 * elapsed / steps,
@@ -117,7 +123,41 @@ def run_form(pkg, torch, w, system, dev, precision, steps, warmup):
                      "realspace": st[4]},
         "stage_samples": len(ring),
         "forces": forces,
+        "kforces": kforces,
     }
+
+
+def ewald_gates(pkg, torch, w, dev, results, nsample=1000, precision=1e-6, rcut_e=12.0):
+    """Accuracy gates against the FP64 Ewald sum on a seeded sample of
+    particles (rms over the sample): (i) total force, (ii) k-space force vs
+    the Ewald k-space force at the P3M alpha."""
+    N, L = w["N"], float(w["L"])
+    q = dev["q"].cpu().numpy()
+    q2 = float(np.sum(q * q))
+    rcut_e = min(rcut_e, 0.5 * L)
+    alpha_e, kmax = pkg.ewald_parameters(N, L, q2, precision, rcut_e)
+    idx = np.sort(np.random.default_rng(2024).choice(N, min(nsample, N), replace=False))
+    ns = len(idx)
+    di = torch.tensor(idx.astype(np.int32), device="cuda")
+    ft = torch.zeros(3 * ns, dtype=torch.float64, device="cuda")
+    fk = torch.zeros_like(ft)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pkg.ewald_sample_device(N, L, *[dev[k].data_ptr() for k in "xyzq"], ns, di.data_ptr(), alpha_e,
+                            rcut_e, kmax, ft.data_ptr(), w["alpha"], fk.data_ptr())
+    t_ewald = time.perf_counter() - t0
+    ft = ft.cpu().numpy().reshape(3, ns)
+    fk = fk.cpu().numpy().reshape(3, ns)
+
+    def rms(a, b):
+        return float(np.sqrt(np.sum((a - b) ** 2) / ns))
+
+    out = {"sample": ns, "ewald_alpha": round(alpha_e, 6), "ewald_kmax": kmax, "ewald_rcut": rcut_e,
+           "ewald_precision_target": precision, "ewald_seconds": round(t_ewald, 2)}
+    for name, r in results.items():
+        out[name] = {"rms_total_vs_ewald": rms(r["forces"][:, idx], ft),
+                     "rms_kspace_vs_ewald": rms(r["kforces"][:, idx], fk)}
+    return out
 
 
 # Algorithmic flops of one ordered real-space pair within rcut (Realteil,
@@ -215,12 +255,131 @@ def cpu_baseline(w, sysd):
                       "%d of %d particles on a cell list, scaled (%.1f s)" % (tk, sample, N, tr)}
 
 
+def batch_systems(nsys=256, seed=4, rcut=3.0, eps=1e-4, density=0.5):
+    """256 systems: N even, uniform in [500, 20000]; L = (N / 0.5)^(1/3); mesh
+    from the even 11-smooth sizes in [16, 48]; cao 3..7; alpha from the
+    tuner's real-space formula at 1e-4; uniform positions, +-1 neutral."""
+    rng = np.random.default_rng(seed)
+    systems, data = [], []
+    for _ in range(nsys):
+        n = 2 * int(rng.integers(250, 10001))
+        L = (n / density) ** (1.0 / 3.0)
+        M = int(rng.choice(SMOOTH_EVEN_16_48))
+        cao = int(rng.integers(3, 8))
+        x, y, z = (rng.uniform(0.0, L, n) for _ in range(3))
+        q = rng.permutation(np.repeat([1.0, -1.0], n // 2))
+        q2 = float(n)
+        alpha = float(np.sqrt(-np.log(eps * np.sqrt(n * rcut * L ** 3) / (2 * np.sqrt(2) * q2))) / rcut)
+        systems.append(dict(n=n, L=L, M=M, cao=cao, alpha=alpha, rcut=rcut))
+        data.append(dict(x=x, y=y, z=z, q=q))
+    return systems, data
+
+
+def bench_batch(pkg, torch, args):
+    """One step = Calculate_forces of all 256 systems (real + k space), as
+    one replay of the captured graph of batched launches."""
+    method = 3
+    systems, data = batch_systems()
+    ntot = sum(s["n"] for s in systems)
+    dev = {k: torch.tensor(np.concatenate([d[k] for d in data]), dtype=torch.float64, device="cuda")
+           for k in "xyzq"}
+    ptr = [dev[k].data_ptr() for k in "xyzq"]
+    res = {}
+    for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
+        batch = pkg.Batch(method, systems, prec)
+        f = torch.zeros(3 * ntot, dtype=torch.float64, device="cuda")
+        fk, fr = torch.zeros_like(f), torch.zeros_like(f)
+        torch.cuda.synchronize()
+        tile, grid = batch.autotune(*ptr, f.data_ptr(), fk.data_ptr(), fr.data_ptr())
+        batch.capture(*ptr, f.data_ptr(), fk.data_ptr(), fr.data_ptr(), None)
+        for _ in range(args.warmup):
+            batch.replay()
+        batch.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.replay()
+        batch.synchronize()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": round(ms, 4), "value": round(ntot / (ms * 1e-3), 1),
+                     "tile_side": tile, "rs_grid": grid, "forces": f.cpu().numpy()}
+        batch.close()
+    rel = float(np.linalg.norm(res["mixed"]["forces"] - res["fp64"]["forces"]) /
+                np.linalg.norm(res["fp64"]["forces"]))
+    for r in res.values():
+        r.pop("forces")
+    return {
+        "metric": "particle-force evals/sec at rms err <=1e-4 (mixed vs FP64 GPU), HBM GB/s",
+        "value": res["mixed"]["value"], "unit": "particle-force evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 mesh/FFT/interp/pairs, f64 influence+accumulation",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": BATCH_WORKLOAD, "method": method, "systems": len(systems),
+                   "N_total": ntot, "N_range": [500, 20000], "mesh": "even 11-smooth 16..48",
+                   "cao": "3..7", "rcut": 3.0, "alpha": "tuning.c:293 at 1e-4",
+                   "parallelism": "single GPU, one launch per stage over all systems, HIP graph"},
+        "roofline": None,
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "accuracy": {"rel_mixed_vs_fp64": rel},
+    }
+
+
+def bench_adjoint(pkg, torch, args):
+    """One step = k-space force of 32768 charges (ik, mesh 32, cao 5) plus
+    the adjoint of loss = sum |F - F_target|^2 w.r.t. positions and charges."""
+    rng = np.random.default_rng(5)
+    N, L, M, cao, alpha = 32768, 32.0, 32, 5, 1.0
+    x, y, z = (rng.uniform(0.0, L, N) for _ in range(3))
+    q = rng.normal(0.0, 0.5, N)
+    q -= q.mean()
+    sysd = {"x": x, "y": y, "z": z, "q": q, "L": L}
+    dev = {k: torch.tensor(sysd[k], dtype=torch.float64, device="cuda") for k in "xyzq"}
+    ptr = [dev[k].data_ptr() for k in "xyzq"]
+    res = {}
+    for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
+        solver = pkg.Solver(0, pkg.HostSystem.from_dict(sysd), pkg.parameters(M, cao, alpha), prec)
+        fk = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+        f = torch.zeros_like(fk)
+        g = torch.tensor(rng.normal(size=3 * N), dtype=torch.float64, device="cuda")
+        dpos = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+        dq = torch.zeros(N, dtype=torch.float64, device="cuda")
+
+        def step():
+            solver.kspace_force_device(N, L, *ptr, fk.data_ptr(), f.data_ptr())
+            solver.kspace_force_vjp_device(N, L, *ptr, g.data_ptr(), dpos.data_ptr(), dq.data_ptr())
+
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            step()
+        solver.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        solver.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": round(ms, 4), "value": round(N / (ms * 1e-3), 1)}
+        solver.close()
+    return {
+        "metric": "particle-force evals/sec (forward + adjoint)", "value": res["mixed"]["value"],
+        "unit": "particle-force evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 mesh/FFT/interp, f64 influence+accumulation",
+        "data": "synthetic (seeded numpy generator)",
+        "config": {"workload": ADJOINT_WORKLOAD, "method": 0, "N": N, "box": L, "mesh": M, "cao": cao,
+                   "alpha": alpha, "parallelism": "single GPU"},
+        "roofline": None, "fp64": res["fp64"], "mixed": res["mixed"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT,
+                    choices=sorted(WORKLOADS) + [BATCH_WORKLOAD, ADJOINT_WORKLOAD])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -233,6 +392,12 @@ def main():
     import __graft_entry__ as entry
 
     pkg = entry.load_package()
+    if args.workload == BATCH_WORKLOAD:
+        print(json.dumps(bench_batch(pkg, torch, args)))
+        return
+    if args.workload == ADJOINT_WORKLOAD:
+        print(json.dumps(bench_adjoint(pkg, torch, args)))
+        return
     w = WORKLOADS[args.workload]
     sysd = make_system(w)
     system = pkg.HostSystem.from_dict(sysd)
@@ -247,6 +412,7 @@ def main():
     rms_force = float(np.sqrt(np.sum(f64 ** 2) / N))
     params = pkg.parameters(w["M"], w["cao"], w["alpha"], w["rcut"])
     est_total, est_k = pkg.error_estimate(w["method"], system, params)
+    gates = ewald_gates(pkg, torch, w, dev, {"fp64": r64, "mixed": rmx})
 
     pairs = pair_count(w, sysd) if w["rcut"] > 0 else 0.0
     dom, per = roofline(w, rmx, 4, pairs)
@@ -275,7 +441,7 @@ def main():
                  "stages": per64},
         "speedup_mixed_vs_fp64": round(r64["ms_per_step"] / rmx["ms_per_step"], 3),
         "accuracy": {"rms_mixed_vs_fp64": rms_mixed_vs_fp64, "rms_force": rms_force,
-                     "estimate_total": est_total, "estimate_kspace": est_k},
+                     "estimate_total": est_total, "estimate_kspace": est_k, "ewald": gates},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, sysd)

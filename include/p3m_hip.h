@@ -290,6 +290,76 @@ int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, do
  * the estimate interpolates it as find_error.c:84-107 does. */
 int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101);
 
+/* FP64 Ewald sum on sampled particles: the reference's Ewald method
+ * (ewald.c:127-172 k space, realpart.c:124-167 real space) evaluated only for
+ * the particles idx[0..nsample) of a system in device memory, as the accuracy
+ * oracle of large systems.  f_total (device, 3 nsample, x/y/z blocks) += the
+ * Ewald force (lattice vectors n^2 <= kmax^2 at alpha, minimum-image pairs
+ * within rcut <= L/2); f_kspace (may be NULL) += that force minus the
+ * real-space sum at alpha_split within rcut, i.e. the k-space force at
+ * alpha_split when rcut covers the erfc range of alpha_split.  Synchronous. */
+int p3m_hip_ewald_sample_device(int n, double length, const double *x, const double *y,
+                                const double *z, const double *q, int nsample, const int *idx,
+                                double alpha, double rcut, int kmax, double alpha_split,
+                                double *f_total, double *f_kspace);
+
+/* ---------------------------------------------------------------------- */
+/* Batches of independent systems                                          */
+/* ---------------------------------------------------------------------- */
+
+/* No counterpart in the reference, whose driver evaluates one system at a
+ * time (main.c:361-386); for callers with many small systems (ensembles,
+ * replicas, parameter scans) that cannot fill a GPU one by one.  All systems
+ * share one method id and precision; each has its own size, box, mesh, order,
+ * alpha and cutoff.  Particles are concatenated system by system: system s
+ * holds slots [n_0 + ... + n_{s-1}, + n_s) of x, y, z, q (device, FP64), and
+ * force arrays use the same slots in three blocks (x, y, z) of the total
+ * count.  Positions must lie in [0, L_s).  Every stage is one launch over all
+ * systems, on the batch's own stream. */
+typedef struct {
+    int nparticles;
+    double length;
+    int mesh; /* even, factors 2, 3, 5, 7, 11, 13 */
+    int cao;  /* 1..7 */
+    double alpha;
+    double rcut; /* 0: no real-space part; else <= length / 2 */
+} p3m_hip_batch_system;
+typedef struct p3m_hip_batch p3m_hip_batch;
+
+/* Make a batch; computes every system's influence function (and, for ad /
+ * ad-r, its self-force coefficients) as Influence_function does. */
+int p3m_hip_batch_create(int method_id, int precision, int nsys, const p3m_hip_batch_system *sys,
+                         p3m_hip_batch **out);
+int p3m_hip_batch_free(p3m_hip_batch *b);
+/* New alpha for system s (recomputes its influence function). */
+int p3m_hip_batch_set_alpha(p3m_hip_batch *b, int s, double alpha);
+/* Kspace_force of every system: fk += k-space force, f -= self force (ad,
+ * ad-r).  Enqueued on the batch's stream. */
+int p3m_hip_batch_kspace_force_device(p3m_hip_batch *b, const double *x, const double *y,
+                                      const double *z, const double *q, double *fk, double *f);
+/* Calculate_forces (common.c:253-312) of every system: zero f, fk, fr;
+ * fr += real space (systems with rcut > 0); fk += k space; f += fk + fr;
+ * energy[s] (device, nsys doubles, may be NULL) receives system s's
+ * real-space energy.  Enqueued on the batch's stream. */
+int p3m_hip_batch_calculate_forces_device(p3m_hip_batch *b, const double *x, const double *y,
+                                          const double *z, const double *q, double *f, double *fk,
+                                          double *fr, double *energy);
+/* Capture p3m_hip_batch_calculate_forces_device with these arrays into a HIP
+ * graph; p3m_hip_batch_replay launches it (same arrays, new contents). */
+int p3m_hip_batch_capture(p3m_hip_batch *b, const double *x, const double *y, const double *z,
+                          const double *q, double *f, double *fk, double *fr, double *energy);
+int p3m_hip_batch_replay(p3m_hip_batch *b);
+int p3m_hip_batch_synchronize(p3m_hip_batch *b);
+/* Choose the launch shapes (assignment tile side, real-space grid) by timing
+ * force evaluations of the given input; drops a captured graph.  Results are
+ * unchanged by the choice (to rounding of the summation order). */
+int p3m_hip_batch_autotune(p3m_hip_batch *b, const double *x, const double *y, const double *z,
+                           const double *q, double *f, double *fk, double *fr, int *tile_side,
+                           int *rs_grid);
+/* alpha for a real-space rms force error `precision` at cutoff rcut, the
+ * formula of the reference's tuner (tuning.c:293). */
+double p3m_hip_tune_alpha(int n, double length, double q2, double rcut, double precision);
+
 /* Stage times (ms, HIP events on the library's stream) of the last calls:
  * [0] charge assignment, [1] FFTs + influence-function multiply,
  * [2] back-interpolation, [3] whole Kspace_force, [4] last real-space call

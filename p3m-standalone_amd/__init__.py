@@ -67,6 +67,11 @@ class Data(ctypes.Structure):
                 ("runtime", Runtime)]
 
 
+class BatchSystem(ctypes.Structure):
+    _fields_ = [("nparticles", ctypes.c_int), ("length", ctypes.c_double), ("mesh", ctypes.c_int),
+                ("cao", ctypes.c_int), ("alpha", ctypes.c_double), ("rcut", ctypes.c_double)]
+
+
 _SP, _PP, _DPT, _FP = (ctypes.POINTER(System), ctypes.POINTER(Parameters), ctypes.POINTER(Data),
                        ctypes.POINTER(Forces))
 InitFn = ctypes.CFUNCTYPE(_DPT, _SP, _PP)
@@ -121,6 +126,18 @@ def lib():
             "p3m_hip_kspace_force_vjp": (i, [_SP, _PP, _DPT, ctypes.POINTER(VectorArray),
                                              ctypes.POINTER(VectorArray), _dp]),
             "p3m_hip_kspace_force_vjp_device": (i, [_DPT, i, d, vp, vp, vp, vp, vp, vp, vp]),
+            "p3m_hip_batch_create": (i, [i, i, i, ctypes.POINTER(BatchSystem),
+                                         ctypes.POINTER(vp)]),
+            "p3m_hip_batch_free": (i, [vp]),
+            "p3m_hip_batch_set_alpha": (i, [vp, i, d]),
+            "p3m_hip_batch_kspace_force_device": (i, [vp, vp, vp, vp, vp, vp, vp]),
+            "p3m_hip_batch_calculate_forces_device": (i, [vp] + [vp] * 8),
+            "p3m_hip_batch_capture": (i, [vp] + [vp] * 8),
+            "p3m_hip_batch_replay": (i, [vp]),
+            "p3m_hip_batch_synchronize": (i, [vp]),
+            "p3m_hip_batch_autotune": (i, [vp] + [vp] * 7 + [ip, ip]),
+            "p3m_hip_tune_alpha": (d, [i, d, d, d, d]),
+            "p3m_hip_ewald_sample_device": (i, [i, d, vp, vp, vp, vp, i, vp, d, d, i, d, vp, vp]),
             "p3m_hip_stage_ring_start": (i, [_DPT, i]),
             "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
             "p3m_hip_get_influence_function": (i, [_DPT, _dp]),
@@ -341,3 +358,99 @@ def set_error_table(kind, mesh, cao, table):
     t = _c(table)
     assert t.shape == (101,)
     check(lib().p3m_hip_set_error_table(kind, mesh, cao, t.ctypes.data_as(_dp)))
+
+
+class Batch:
+    """Many independent systems evaluated together (p3m_hip_batch_*).
+
+    `systems` is a list of dicts with keys n, L, M, cao, alpha, rcut.  Device
+    arrays (pointers) are concatenated system by system; force arrays hold
+    three blocks (x, y, z) of the total particle count."""
+
+    def __init__(self, method_id, systems, precision=FP64):
+        arr = (BatchSystem * len(systems))(*[
+            BatchSystem(int(s["n"]), float(s["L"]), int(s["M"]), int(s["cao"]), float(s["alpha"]),
+                        float(s.get("rcut", 0.0))) for s in systems])
+        self.method_id = method_id
+        self.systems = [dict(s) for s in systems]
+        self.ntot = sum(int(s["n"]) for s in systems)
+        self.offsets = np.concatenate([[0], np.cumsum([int(s["n"]) for s in systems])]).astype(int)
+        self.b = ctypes.c_void_p()
+        check(lib().p3m_hip_batch_create(method_id, precision, len(systems), arr, ctypes.byref(self.b)))
+
+    def close(self):
+        if self.b:
+            check(lib().p3m_hip_batch_free(self.b))
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_alpha(self, s, alpha):
+        check(lib().p3m_hip_batch_set_alpha(self.b, int(s), float(alpha)))
+        self.systems[s]["alpha"] = float(alpha)
+
+    def kspace_force_device(self, x, y, z, q, fk, f):
+        check(lib().p3m_hip_batch_kspace_force_device(self.b, x, y, z, q, fk, f))
+
+    def calculate_forces_device(self, x, y, z, q, f, fk, fr, energy=None):
+        check(lib().p3m_hip_batch_calculate_forces_device(self.b, x, y, z, q, f, fk, fr, energy))
+
+    def capture(self, x, y, z, q, f, fk, fr, energy=None):
+        check(lib().p3m_hip_batch_capture(self.b, x, y, z, q, f, fk, fr, energy))
+
+    def replay(self):
+        check(lib().p3m_hip_batch_replay(self.b))
+
+    def synchronize(self):
+        check(lib().p3m_hip_batch_synchronize(self.b))
+
+    def autotune(self, x, y, z, q, f, fk, fr):
+        t, g = ctypes.c_int(), ctypes.c_int()
+        check(lib().p3m_hip_batch_autotune(self.b, x, y, z, q, f, fk, fr, ctypes.byref(t),
+                                           ctypes.byref(g)))
+        return t.value, g.value
+
+
+def tune_alpha(n, length, q2, rcut, precision):
+    """alpha of the reference's tuner for a real-space error target (tuning.c:293)."""
+    return lib().p3m_hip_tune_alpha(int(n), float(length), float(q2), float(rcut), float(precision))
+
+
+def ewald_error_estimates(n, length, q2, alpha, rcut, kmax):
+    """Real- and k-space rms force error estimates of the Ewald sum
+    (Kolafa-Perram eq. 16, Petersen eq. 11; ewald.c:174-199)."""
+    import math
+    r = q2 * math.sqrt(rcut / (2.0 * length ** 3)) * math.exp(-(alpha * rcut) ** 2) / ((alpha * rcut) ** 2)
+    k = (2.0 * q2 * alpha / length * math.sqrt(1.0 / (math.pi * kmax * n)) *
+         math.exp(-(math.pi * kmax / (alpha * length)) ** 2))
+    return r, k
+
+
+def ewald_parameters(n, length, q2, precision, rcut):
+    """(alpha, kmax) of an Ewald sum with real-space cutoff rcut whose
+    estimated rms force error is <= precision: alpha from the real-space
+    estimate, then the smallest kmax that meets it in k space."""
+    lo, hi = 1e-3, 50.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if ewald_error_estimates(n, length, q2, mid, rcut, 1)[0] > 0.5 * precision:
+            lo = mid
+        else:
+            hi = mid
+    alpha = hi
+    kmax = 1
+    while ewald_error_estimates(n, length, q2, alpha, rcut, kmax)[1] > 0.5 * precision:
+        kmax += 1
+    return alpha, kmax
+
+
+def ewald_sample_device(n, length, x, y, z, q, nsample, idx, alpha, rcut, kmax, f_total,
+                        alpha_split=0.0, f_kspace=None):
+    """Sampled FP64 Ewald forces (device pointers); see p3m_hip_ewald_sample_device."""
+    check(lib().p3m_hip_ewald_sample_device(int(n), float(length), x, y, z, q, int(nsample), idx,
+                                            float(alpha), float(rcut), int(kmax), float(alpha_split),
+                                            f_total, f_kspace))

@@ -1,0 +1,77 @@
+// Descriptors of a batch of independent systems and the batched launchers of
+// the FFT, particle and real-space kernels (C++ linkage, internal to the
+// library).  Every batched launch covers all systems: a block (or a particle
+// slot, or a work item) finds its system by binary search in a prefix array,
+// then runs the same per-tile / per-particle / per-item body as the
+// single-system kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_tiles.hpp"
+#include "k_internal.hpp"
+
+namespace p3m {
+
+// buffer slots of a system's mesh arena
+enum BufSlot { kBufMesh = 0, kBufSpec0 = 1, kBufF0 = 4, kNumBuf = 7 };
+
+struct BSys {
+    // mesh geometry
+    int M, cao, T, nt;  // mesh, order, tile side, tiles per side
+    double L, Hi, h;
+    long tile0;         // first global tile index
+    int p0, n;          // particle slots [p0, p0 + n) in the concatenated arrays
+    fft::Radices rp;
+    void *buf[kNumBuf]; // mesh, spec0..2, F0..2 (element type of the batch)
+    const double *G, *Dn, *selfb;
+    const void *tw;     // twiddles exp(-2 pi i k / M), batch precision
+    double prefac;      // back-interpolation prefactor (ad: includes nothing else)
+    // real space
+    int cell0, nc, reach, small;
+    double rcut, alpha;
+};
+
+// forms shared by all systems of a batch
+struct BForm {
+    int r2c, interlaced, fp32, ad;
+};
+
+// ---- launchers (defined next to the single-system kernels) ----
+// FFT: blk = block prefix over systems for the given pass
+int bfft_pass(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks,
+              size_t lds, int axis,
+              int slot, int inv, p3mk_queue *q);
+int bfft_r2c_z(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks,
+              size_t lds, int in_slot,
+               int out_slot, p3mk_queue *q);
+int bfft_c2r_z(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks,
+              size_t lds, int in_slot,
+               int out_slot, p3mk_queue *q);
+int bfft_convolve_x(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks,
+              size_t lds,
+                    int nout, int in_slot, int out_slot, p3mk_queue *q);
+// lines per tile and LDS buffers of each batched pass kind (host side)
+int bfft_lines_per_tile(const BForm &f, int M, int kind);
+size_t bfft_lds_bytes(const BForm &f, int M, int kind);
+enum { kPassZ = 0, kPassY = 1, kPassX = 2 };
+
+// particles: sort by global tile, assign, back-interpolate
+int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, int ntot,
+               const double *x, const double *y, const double *z, const double *q, int *key,
+               int *count, int *start, int *cursor, int *perm, int *chunks, int *cstart,
+               int *scan_scratch, double4 *sorted, p3mk_queue *qu);
+int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
+                 long max_items, size_t lds, const int *start, const int *cstart,
+                 const double4 *sorted, p3mk_queue *qu);
+int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int ntot,
+                 const double4 *sorted, const int *perm, double *fk, double *ftot, p3mk_queue *qu);
+
+// real space: cells per system, one persistent launch over all items
+int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
+               long ncell, int ntot, const double *x, const double *y, const double *z,
+               const double *q, int *cell_id, int *count, int *start, int *cursor, int *perm,
+               int *items, int *istart, int *scan_scratch, int *counter, double4 *sorted,
+               int grid, double *fr, double *energy, p3mk_queue *qu);
+
+}  // namespace p3m

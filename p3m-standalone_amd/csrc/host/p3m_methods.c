@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include "../p3m_kernels.h"
+#include "host_common.h"
 
 #define PI_D 3.14159265358979323846
 
@@ -27,7 +28,7 @@ static _Thread_local char g_err[512];
 
 const char *p3m_hip_last_error(void) { return g_err; }
 
-static int fail(int code, const char *fmt, ...) {
+int p3mh_fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -35,33 +36,19 @@ static int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-static int kfail(int code, const char *what) {
-    return fail(code, "%s: %s", what, p3mk_error());
+int p3mh_kfail(int code, const char *what) {
+    return p3mh_fail(code, "%s: %s", what, p3mk_error());
 }
-
-#define TRYK(expr, what)                          \
-    do {                                          \
-        int rc_ = (expr);                         \
-        if (rc_ != P3MK_OK)                       \
-            return kfail(rc_, what);              \
-    } while (0)
-
-#define TRY(expr)                                 \
-    do {                                          \
-        int rc_ = (expr);                         \
-        if (rc_ != P3M_HIP_OK)                    \
-            return rc_;                           \
-    } while (0)
 
 /* ---------------------------------------------------------------------- */
 /* method descriptions                                                     */
 /* ---------------------------------------------------------------------- */
 
-static int is_ad(int id) { return id == 2 || id == 3 || id == 7; }
-static int is_interlaced(int id) { return id == 1 || id == 3; }
-static int is_r2c(int id) { return id == 6 || id == 7; }
-static int has_self_force(int id) { return id == 2 || id == 7; }
-static int known_method(int id) { return id == 0 || id == 1 || id == 2 || id == 3 || id == 6 || id == 7; }
+#define is_ad p3mh_is_ad
+#define is_interlaced p3mh_is_interlaced
+#define is_r2c p3mh_is_r2c
+#define has_self_force p3mh_has_self_force
+#define known_method p3mh_known_method
 
 /* device state hanging off data_t.method_data */
 typedef struct {
@@ -128,19 +115,19 @@ static void ctx_free(p3m_ctx *c) {
 
 static int check_params(int id, const system_t *s, const parameters_t *p) {
     if (!known_method(id))
-        return fail(P3M_HIP_EINVAL, "method id %d is not a P3M method of this library", id);
+        return p3mh_fail(P3M_HIP_EINVAL, "method id %d is not a P3M method of this library", id);
     if (s == NULL || p == NULL)
-        return fail(P3M_HIP_EINVAL, "system and parameters must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "system and parameters must not be NULL");
     if (p->cao < 1 || p->cao > 7)
-        return fail(P3M_HIP_EINVAL, "charge assignment order %d not in 1..7", p->cao);
+        return p3mh_fail(P3M_HIP_EINVAL, "charge assignment order %d not in 1..7", p->cao);
     if (p->mesh < 2 || (p->mesh % 2) != 0)
-        return fail(P3M_HIP_EINVAL, "mesh %d must be even and >= 2", p->mesh);
+        return p3mh_fail(P3M_HIP_EINVAL, "mesh %d must be even and >= 2", p->mesh);
     if (p->mesh < p->cao)
-        return fail(P3M_HIP_EINVAL, "mesh %d smaller than cao %d", p->mesh, p->cao);
+        return p3mh_fail(P3M_HIP_EINVAL, "mesh %d smaller than cao %d", p->mesh, p->cao);
     if (s->nparticles < 0)
-        return fail(P3M_HIP_EINVAL, "negative particle count");
+        return p3mh_fail(P3M_HIP_EINVAL, "negative particle count");
     if (!(s->length > 0.0) || !isfinite(s->length))
-        return fail(P3M_HIP_EINVAL, "box length must be positive");
+        return p3mh_fail(P3M_HIP_EINVAL, "box length must be positive");
     return P3M_HIP_OK;
 }
 
@@ -199,10 +186,10 @@ static p3mk_fft *ctx_fft(p3m_ctx *c) { return c->fft[c->geom.fp32 ? 1 : 0]; }
 int p3m_hip_influence_function(system_t *s, parameters_t *p, data_t *d) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     TRY(check_params(c->method, s, p));
     if (p->mesh != c->geom.M || p->cao != c->geom.cao)
-        return fail(P3M_HIP_EINVAL, "mesh/cao differ from the values given to Init");
+        return p3mh_fail(P3M_HIP_EINVAL, "mesh/cao differ from the values given to Init");
     c->geom.L = s->length;
     TRYK(p3mk_influence(c->method, &c->geom, p->alpha, 1, c->G, c->Dn, c->q), "influence function");
     /* ad / ad-r also recompute the self-force coefficients, except at
@@ -227,19 +214,19 @@ int p3m_hip_init(int method_id, system_t *s, parameters_t *p, int precision, dat
     p3m_ctx *c;
     int rc;
     if (out == NULL)
-        return fail(P3M_HIP_EINVAL, "out must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "out must not be NULL");
     *out = NULL;
     TRY(check_params(method_id, s, p));
     if (precision != P3M_HIP_FP64 && precision != P3M_HIP_MIXED)
-        return fail(P3M_HIP_EINVAL, "unknown precision %d", precision);
+        return p3mh_fail(P3M_HIP_EINVAL, "unknown precision %d", precision);
     if (p3mk_device_check() != P3MK_OK)
-        return kfail(P3M_HIP_ENODEVICE, "device");
+        return p3mh_kfail(P3M_HIP_ENODEVICE, "device");
     d = (data_t *)calloc(1, sizeof(data_t));
     c = (p3m_ctx *)calloc(1, sizeof(p3m_ctx));
     if (d == NULL || c == NULL) {
         free(d);
         free(c);
-        return fail(P3M_HIP_ENOMEM, "host allocation");
+        return p3mh_fail(P3M_HIP_ENOMEM, "host allocation");
     }
     d->mesh = p->mesh;
     d->method_data = c;
@@ -256,7 +243,7 @@ int p3m_hip_init(int method_id, system_t *s, parameters_t *p, int precision, dat
         d->self_force_corrections = (double *)calloc(81, sizeof(double));
         if (d->self_force_corrections == NULL) {
             p3m_hip_free(d);
-            return fail(P3M_HIP_ENOMEM, "host allocation");
+            return p3mh_fail(P3M_HIP_ENOMEM, "host allocation");
         }
     }
     rc = ctx_alloc(c);
@@ -285,9 +272,9 @@ int p3m_hip_free(data_t *d) {
 int p3m_hip_set_precision(data_t *d, int precision) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (precision != P3M_HIP_FP64 && precision != P3M_HIP_MIXED)
-        return fail(P3M_HIP_EINVAL, "unknown precision %d", precision);
+        return p3mh_fail(P3M_HIP_EINVAL, "unknown precision %d", precision);
     c->precision = precision;
     c->geom.fp32 = precision == P3M_HIP_MIXED;
     return P3M_HIP_OK;
@@ -383,29 +370,29 @@ int p3m_hip_kspace_force_device(data_t *d, int n, double length, const double *x
                                 double *f) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (n != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
     if (length != c->geom.L)
-        return fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
+        return p3mh_fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
     if (!c->have_G)
-        return fail(P3M_HIP_EINVAL, "influence function not computed");
+        return p3mh_fail(P3M_HIP_EINVAL, "influence function not computed");
     if (has_self_force(c->method) && f == NULL)
-        return fail(P3M_HIP_EINVAL, "method %d needs the total-force array f", c->method);
+        return p3mh_fail(P3M_HIP_EINVAL, "method %d needs the total-force array f", c->method);
     return kspace_enqueue(c, n, x, y, z, q, fk, f);
 }
 
 int p3m_hip_synchronize(data_t *d) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     TRYK(p3mk_queue_sync(c->q), "sync");
     return P3M_HIP_OK;
 }
 
 static int check_host_system(const system_t *s) {
     if (s->p == NULL || s->p->x == NULL || s->p->y == NULL || s->p->z == NULL || s->q == NULL)
-        return fail(P3M_HIP_EINVAL, "system arrays must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "system arrays must not be NULL");
     return P3M_HIP_OK;
 }
 
@@ -420,7 +407,7 @@ static int upload_positions(p3m_ctx *c, const system_t *s) {
                               c->iscratch, c->q),
          "position check");
     if (bad)
-        return fail(P3M_HIP_ERANGE, "%d particles outside [0, L) or not finite", bad);
+        return p3mh_fail(P3M_HIP_ERANGE, "%d particles outside [0, L) or not finite", bad);
     return P3M_HIP_OK;
 }
 
@@ -429,15 +416,15 @@ int p3m_hip_kspace_force(system_t *s, parameters_t *p, data_t *d, forces_t *f) {
     size_t n, i;
     int dim;
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     TRY(check_params(c->method, s, p));
     TRY(check_host_system(s));
     if (f == NULL || f->f_k == NULL || (has_self_force(c->method) && f->f == NULL))
-        return fail(P3M_HIP_EINVAL, "forces must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "forces must not be NULL");
     if (s->nparticles != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
     if (s->length != c->geom.L)
-        return fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
+        return p3mh_fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
     n = (size_t)s->nparticles;
     if (n == 0)
         return P3M_HIP_OK;
@@ -477,7 +464,7 @@ int p3m_hip_stage_times(data_t *d, double out[5]) {
     p3m_ctx *c = ctx_of(d);
     int k;
     if (c == NULL || out == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     TRYK(p3mk_queue_sync(c->q), "sync");
     TRY(collect_stage_times(c));
     for (k = 0; k < 5; k++)
@@ -489,7 +476,7 @@ int p3m_hip_get_influence_function(data_t *d, double *G) {
     p3m_ctx *c = ctx_of(d);
     size_t M3;
     if (c == NULL || G == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     M3 = (size_t)c->geom.M * c->geom.M * c->geom.M;
     TRYK(p3mk_copy_d2h(G, c->G, sizeof(double) * M3, c->q), "download");
     TRYK(p3mk_queue_sync(c->q), "sync");
@@ -499,7 +486,7 @@ int p3m_hip_get_influence_function(data_t *d, double *G) {
 int p3m_hip_get_self_force_coefficients(data_t *d, double *b) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL || b == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     TRYK(p3mk_copy_d2h(b, c->selfb, sizeof(double) * 81, c->q), "download");
     TRYK(p3mk_queue_sync(c->q), "sync");
     return P3M_HIP_OK;
@@ -525,11 +512,11 @@ int p3m_hip_realteil_device(data_t *d, int n, double length, const double *x, co
                             double *fr, double *energy) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (n > c->n)
-        return fail(P3M_HIP_EINVAL, "more particles than Init's %d", c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "more particles than Init's %d", c->n);
     if (!(rcut > 0.0) || rcut > 0.5 * length)
-        return fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
+        return p3mh_fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
     TRY(ensure_cells(c, c->n, length, rcut));
     TRYK(p3mk_record(c->q, 4), "event");
     TRYK(p3mk_realspace(c->cells, n, length, x, y, z, q, rcut, alpha, c->geom.fp32, fr, energy,
@@ -546,11 +533,11 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
                                     double *energy) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (f == NULL || fk == NULL || fr == NULL)
-        return fail(P3M_HIP_EINVAL, "force arrays must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "force arrays must not be NULL");
     if (n != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
     /* common.c:259-263 */
     TRYK(p3mk_fill_zero(f, sizeof(double) * 3 * (size_t)n, c->q), "zero");
     TRYK(p3mk_fill_zero(fk, sizeof(double) * 3 * (size_t)n, c->q), "zero");
@@ -575,9 +562,9 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
 int p3m_hip_stage_ring_start(data_t *d, int nsteps) {
     p3m_ctx *c = ctx_of(d);
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (nsteps < 0)
-        return fail(P3M_HIP_EINVAL, "negative step count");
+        return p3mh_fail(P3M_HIP_EINVAL, "negative step count");
     TRYK(p3mk_ring_start(c->q, nsteps), "event ring");
     free(c->ring_rs);
     c->ring_rs = NULL;
@@ -586,7 +573,7 @@ int p3m_hip_stage_ring_start(data_t *d, int nsteps) {
         c->ring_rs = (unsigned char *)calloc((size_t)nsteps, 1);
         if (c->ring_rs == NULL) {
             p3mk_ring_start(c->q, 0);
-            return fail(P3M_HIP_ENOMEM, "host allocation failed");
+            return p3mh_fail(P3M_HIP_ENOMEM, "host allocation failed");
         }
         c->ring_sets = nsteps;
     }
@@ -598,7 +585,7 @@ int p3m_hip_stage_ring_read(data_t *d, double *out, int max_steps, int *nsteps) 
     int k, s, filled;
     float t;
     if (c == NULL || out == NULL || nsteps == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     TRYK(p3mk_queue_sync(c->q), "sync");
     filled = p3mk_ring_filled(c->q);
     if (filled > max_steps)
@@ -641,20 +628,20 @@ int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f) {
     int rc = P3M_HIP_OK, dim;
     double energy = 0.0;
     if (s == NULL || p == NULL || f == NULL || f->f_r == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     TRY(check_host_system(s));
     n = (size_t)s->nparticles;
     if (n == 0)
         return P3M_HIP_OK;
     if (!(p->rcut > 0.0) || p->rcut > 0.5 * s->length)
-        return fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
+        return p3mh_fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
     if (p3mk_queue_create(&q) != P3MK_OK)
-        return kfail(P3M_HIP_ENODEVICE, "stream");
+        return p3mh_kfail(P3M_HIP_ENODEVICE, "stream");
 #define CHK(expr, what)                     \
     if (rc == P3M_HIP_OK) {                 \
         int r_ = (expr);                    \
         if (r_ != P3MK_OK)                  \
-            rc = kfail(r_, what);           \
+            rc = p3mh_kfail(r_, what);           \
     }
     CHK(p3mk_cells_create(&cells, (int)n, s->length, p->rcut), "cell list");
     CHK(p3mk_alloc((void **)&pos, sizeof(double) * 4 * n), "positions");
@@ -695,13 +682,13 @@ int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, da
     int dim;
     double energy = 0.0;
     if (c == NULL || m == NULL)
-        return fail(P3M_HIP_EINVAL, "bad arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
     if (m->method_id != c->method)
-        return fail(P3M_HIP_EINVAL, "data_t belongs to method %d, not %d", c->method, m->method_id);
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t belongs to method %d, not %d", c->method, m->method_id);
     TRY(check_params(c->method, s, p));
     TRY(check_host_system(s));
     if (f == NULL || f->f == NULL || f->f_k == NULL || f->f_r == NULL)
-        return fail(P3M_HIP_EINVAL, "forces must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "forces must not be NULL");
     n = (size_t)s->nparticles;
     /* common.c:259-263: all three force arrays start from zero */
     for (dim = 0; dim < 3; dim++) {
@@ -712,9 +699,9 @@ int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, da
     if (n == 0)
         return P3M_HIP_OK;
     if (s->nparticles != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
     if (s->length != c->geom.L)
-        return fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
+        return p3mh_fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
     TRY(upload_positions(c, s));
     TRYK(p3mk_fill_zero(c->force, sizeof(double) * 9 * n, c->q), "zero");
     TRYK(p3mk_fill_zero(c->energy, sizeof(double), c->q), "zero");
@@ -754,13 +741,13 @@ static int g_ntables;
 int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101) {
     int k;
     if ((kind != 0 && kind != 1) || table101 == NULL || cao < 1 || cao > 7 || mesh < 2)
-        return fail(P3M_HIP_EINVAL, "bad error-table arguments");
+        return p3mh_fail(P3M_HIP_EINVAL, "bad error-table arguments");
     for (k = 0; k < g_ntables; k++)
         if (g_tables[k].kind == kind && g_tables[k].mesh == mesh && g_tables[k].cao == cao)
             break;
     if (k == g_ntables) {
         if (g_ntables == MAX_TABLES)
-            return fail(P3M_HIP_ENOMEM, "too many error tables");
+            return p3mh_fail(P3M_HIP_ENOMEM, "too many error tables");
         g_ntables++;
     }
     g_tables[k].kind = kind;
@@ -803,7 +790,7 @@ static int kspace_error(int id, const system_t *s, const parameters_t *p, double
         he_q = (id == 1) ? 0.0 : t; /* ik keeps the lookup's -1 (p3m-ik.c:255-274) */
     }
     if (p3mk_queue_create(&q) != P3MK_OK)
-        return kfail(P3M_HIP_ENODEVICE, "stream");
+        return p3mh_kfail(P3M_HIP_ENODEVICE, "stream");
     kr = p3mk_alloc((void **)&scratch, sizeof(double) * p3mk_reduce_scratch_doubles(M));
     if (kr == P3MK_OK) {
         double sum = 0.0;
@@ -811,7 +798,7 @@ static int kspace_error(int id, const system_t *s, const parameters_t *p, double
         he_q += sum;
     }
     if (kr != P3MK_OK)
-        rc = kfail(kr, "error estimate");
+        rc = p3mh_kfail(kr, "error estimate");
     p3mk_release(scratch);
     p3mk_queue_destroy(q);
     if (rc != P3M_HIP_OK)
@@ -867,15 +854,15 @@ int p3m_hip_kspace_force_vjp_device(data_t *d, int n, double length, const doubl
     void *in[3];
     int k;
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     if (n != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
     if (length != c->geom.L)
-        return fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
+        return p3mh_fail(P3M_HIP_EINVAL, "box length differs from the influence function's");
     if (!c->have_G)
-        return fail(P3M_HIP_EINVAL, "influence function not computed");
+        return p3mh_fail(P3M_HIP_EINVAL, "influence function not computed");
     if (gF == NULL || dpos == NULL || dq == NULL)
-        return fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
     if (n == 0)
         return P3M_HIP_OK;
     TRY(ensure_adjoint_buffers(c));
@@ -926,13 +913,13 @@ int p3m_hip_kspace_force_vjp(system_t *s, parameters_t *p, data_t *d, const vect
     size_t n, b;
     int dim;
     if (c == NULL)
-        return fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
     TRY(check_params(c->method, s, p));
     TRY(check_host_system(s));
     if (dF == NULL || dpos == NULL || dq == NULL)
-        return fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
+        return p3mh_fail(P3M_HIP_EINVAL, "cotangent and gradient arrays must not be NULL");
     if (s->nparticles != c->n)
-        return fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
+        return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", s->nparticles, c->n);
     n = (size_t)s->nparticles;
     if (n == 0)
         return P3M_HIP_OK;

@@ -16,416 +16,58 @@
 #include <cmath>
 #include <new>
 
+#include "batch_desc.hpp"
 #include "device_common.hpp"
+#include "fft_tiles.hpp"
 #include "k_internal.hpp"
 
 using namespace p3m;
+using namespace p3m::fft;
 
 namespace {
-
-constexpr int kMaxPasses = 12;
-constexpr int kThreads = 256;
-constexpr size_t kLdsBudget = 64 * 1024;
-
-struct Radices {
-    int n;
-    int r[kMaxPasses];
-};
-
-// Geometry of a batch of lines: line L = a*C + c starts at a*sa + c,
-// element e sits at start + e*es (units: complex elements).
-struct LineMap {
-    int n;       // transform length
-    long es;     // element stride
-    long C;      // lines with unit stride between neighbours
-    long sa;     // stride between groups of C lines
-    long nlines; // number of lines
-};
-
-__device__ __forceinline__ long line_start(const LineMap &m, long L) {
-    const long a = L / m.C;
-    return a * m.sa + (L - a * m.C);
-}
-
-template <typename V> __device__ __forceinline__ V twid(const V *tw, int idx, int inv) {
-    V w = tw[idx];
-    if (inv)
-        w.y = -w.y;
-    return w;
-}
-
-template <typename V> __device__ __forceinline__ V mul_mi(V a, int inv) {
-    // multiply by -i (forward) or +i (backward)
-    V r;
-    if (inv) {
-        r.x = -a.y;
-        r.y = a.x;
-    } else {
-        r.x = a.y;
-        r.y = -a.x;
-    }
-    return r;
-}
-
-template <int R, typename V> struct Dft;
-
-template <typename V> struct Dft<2, V> {
-    __device__ static __forceinline__ void run(V *v, const V *, int, int) {
-        V a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    }
-};
-
-template <typename V> struct Dft<4, V> {
-    __device__ static __forceinline__ void run(V *v, const V *, int, int inv) {
-        V t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        V t2 = cadd(v[1], v[3]), t3 = mul_mi(csub(v[1], v[3]), inv);
-        v[0] = cadd(t0, t2);
-        v[2] = csub(t0, t2);
-        v[1] = cadd(t1, t3);
-        v[3] = csub(t1, t3);
-    }
-};
-
-template <typename V> struct Dft<8, V> {
-    __device__ static __forceinline__ void run(V *v, const V *tw, int n, int inv) {
-        V e[4] = {v[0], v[2], v[4], v[6]};
-        V o[4] = {v[1], v[3], v[5], v[7]};
-        Dft<4, V>::run(e, tw, n, inv);
-        Dft<4, V>::run(o, tw, n, inv);
-        const int s = n / 8;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            V t = (k == 0) ? o[0] : cmul(o[k], twid(tw, k * s, inv));
-            v[k] = cadd(e[k], t);
-            v[k + 4] = csub(e[k], t);
-        }
-    }
-};
-
-// generic odd radix: X_m = sum_r v_r W^(rm), W = exp(-+2 pi i / R)
-template <int R, typename V> struct Dft {
-    __device__ static __forceinline__ void run(V *v, const V *tw, int n, int inv) {
-        const int s = n / R;
-        V w[R];
-#pragma unroll
-        for (int q = 0; q < R; q++)
-            w[q] = twid(tw, q * s, inv);
-        V out[R];
-#pragma unroll
-        for (int m = 0; m < R; m++) {
-            V acc = v[0];
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-                const V p = cmul(v[r], w[(r * m) % R]);
-                acc = cadd(acc, p);
-            }
-            out[m] = acc;
-        }
-#pragma unroll
-        for (int m = 0; m < R; m++)
-            v[m] = out[m];
-    }
-};
-
-template <int R, typename V>
-__device__ __forceinline__ void butterfly(const V *__restrict__ src, V *__restrict__ dst, int n,
-                                          int Ns, int j, const V *__restrict__ tw, int inv) {
-    const int k = j % Ns;
-    const int str = n / R;
-    V v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-        v[r] = src[j + r * str];
-    if (Ns > 1) {
-        const int step = k * (n / (Ns * R));
-#pragma unroll
-        for (int r = 1; r < R; r++)
-            v[r] = cmul(v[r], twid(tw, r * step, inv));
-    }
-    Dft<R, V>::run(v, tw, n, inv);
-    const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; r++)
-        dst[base + r * Ns] = v[r];
-}
-
-// All radix passes over `nl` lines held in LDS (line l at src + l*pitch).
-// Returns the buffer that holds the result.
-template <typename V>
-__device__ V *fft_tile(V *src, V *dst, int nl, int n, int pitch, const Radices &rp,
-                       const V *__restrict__ tw, int inv) {
-    int Ns = 1;
-    for (int p = 0; p < rp.n; p++) {
-        const int R = rp.r[p];
-        const int nb = n / R;
-        const int total = nl * nb;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int l = t / nb, j = t - l * nb;
-            const V *s = src + l * pitch;
-            V *d = dst + l * pitch;
-            switch (R) {
-            case 2: butterfly<2>(s, d, n, Ns, j, tw, inv); break;
-            case 3: butterfly<3>(s, d, n, Ns, j, tw, inv); break;
-            case 4: butterfly<4>(s, d, n, Ns, j, tw, inv); break;
-            case 5: butterfly<5>(s, d, n, Ns, j, tw, inv); break;
-            case 7: butterfly<7>(s, d, n, Ns, j, tw, inv); break;
-            case 8: butterfly<8>(s, d, n, Ns, j, tw, inv); break;
-            case 11: butterfly<11>(s, d, n, Ns, j, tw, inv); break;
-            case 13: butterfly<13>(s, d, n, Ns, j, tw, inv); break;
-            default: break;
-            }
-        }
-        __syncthreads();
-        V *t = src;
-        src = dst;
-        dst = t;
-        Ns *= R;
-    }
-    return src;
-}
-
-// Tile load / store.  Strided lines: consecutive threads take consecutive
-// lines (adjacent in memory).  Contiguous lines: consecutive threads take
-// consecutive elements.
-template <typename V>
-__device__ __forceinline__ void load_tile(const V *__restrict__ g, V *t, const LineMap &m, long L0,
-                                          int lt, int pitch, bool contig) {
-    const int n = m.n, total = lt * n;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int l, e;
-        if (contig) {
-            l = idx / n;
-            e = idx - l * n;
-        } else {
-            e = idx / lt;
-            l = idx - e * lt;
-        }
-        const long L = L0 + l;
-        if (L < m.nlines)
-            t[l * pitch + e] = g[line_start(m, L) + (long)e * m.es];
-    }
-}
-
-template <typename V>
-__device__ __forceinline__ void store_tile(V *__restrict__ g, const V *t, const LineMap &m, long L0,
-                                           int lt, int pitch, bool contig) {
-    const int n = m.n, total = lt * n;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        int l, e;
-        if (contig) {
-            l = idx / n;
-            e = idx - l * n;
-        } else {
-            e = idx / lt;
-            l = idx - e * lt;
-        }
-        const long L = L0 + l;
-        if (L < m.nlines)
-            g[line_start(m, L) + (long)e * m.es] = t[l * pitch + e];
-    }
-}
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_fft_pass(c2_t<T> *data, LineMap m, Radices rp, const c2_t<T> *__restrict__ tw, int inv, int lt,
            int contig) {
-    using V = c2_t<T>;
     extern __shared__ unsigned char smem[];
-    const int pitch = m.n + 1;
-    V *A = reinterpret_cast<V *>(smem);
-    V *B = A + lt * pitch;
-    const long L0 = (long)blockIdx.x * lt;
-    load_tile(data, A, m, L0, lt, pitch, contig != 0);
-    __syncthreads();
-    V *r = fft_tile(A, B, lt, m.n, pitch, rp, tw, inv);
-    store_tile(data, r, m, L0, lt, pitch, contig != 0);
+    fft_pass_tile<T>(data, m, rp, tw, inv, lt, contig, (long)blockIdx.x * lt, smem);
 }
 
-// r2c along z: real lines of length n (dense, line L at L*n) -> first n/2+1
-// coefficients (line L at L*H).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_fft_r2c_z(const T *__restrict__ in, c2_t<T> *__restrict__ out, long nlines, int n, Radices rp,
             const c2_t<T> *__restrict__ tw, int lt) {
-    using V = c2_t<T>;
     extern __shared__ unsigned char smem[];
-    const int pitch = n + 1, H = n / 2 + 1;
-    V *A = reinterpret_cast<V *>(smem);
-    V *B = A + lt * pitch;
-    const long L0 = (long)blockIdx.x * lt;
-    for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-        const int l = idx / n, e = idx - l * n;
-        if (L0 + l < nlines) {
-            V v;
-            v.x = in[(L0 + l) * n + e];
-            v.y = T(0);
-            A[l * pitch + e] = v;
-        }
-    }
-    __syncthreads();
-    V *r = fft_tile(A, B, lt, n, pitch, rp, tw, 0);
-    for (int idx = threadIdx.x; idx < lt * H; idx += blockDim.x) {
-        const int l = idx / H, e = idx - l * H;
-        if (L0 + l < nlines)
-            out[(L0 + l) * H + e] = r[l * pitch + e];
-    }
+    fft_r2c_z_tile<T>(in, out, nlines, n, rp, tw, lt, (long)blockIdx.x * lt, smem);
 }
 
-// c2r along z: n/2+1 Hermitian coefficients per line -> n real values.  The
-// full line is rebuilt as X[n-k] = conj(X[k]) and the real part of the
-// backward transform kept, which is FFTW's c2r on Hermitian input.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_fft_c2r_z(const c2_t<T> *__restrict__ in, T *__restrict__ out, long nlines, int n, Radices rp,
             const c2_t<T> *__restrict__ tw, int lt) {
-    using V = c2_t<T>;
     extern __shared__ unsigned char smem[];
-    const int pitch = n + 1, H = n / 2 + 1;
-    V *A = reinterpret_cast<V *>(smem);
-    V *B = A + lt * pitch;
-    const long L0 = (long)blockIdx.x * lt;
-    for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-        const int l = idx / n, e = idx - l * n;
-        if (L0 + l < nlines) {
-            V v;
-            if (e < H) {
-                v = in[(L0 + l) * H + e];
-            } else {
-                v = in[(L0 + l) * H + (n - e)];
-                v.y = -v.y;
-            }
-            A[l * pitch + e] = v;
-        }
-    }
-    __syncthreads();
-    V *r = fft_tile(A, B, lt, n, pitch, rp, tw, 1);
-    for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-        const int l = idx / n, e = idx - l * n;
-        if (L0 + l < nlines)
-            out[(L0 + l) * n + e] = r[l * pitch + e].x;
-    }
+    fft_c2r_z_tile<T>(in, out, nlines, n, rp, tw, lt, (long)blockIdx.x * lt, smem);
 }
 
-// Fused x pass: forward along x, multiply, backward along x.
-// Lines are c = j*W + k (W = M for c2c, M/2+1 for r2c), element e = n_x.
-// nout == 1: out0 = G * X.  nout == 3: outd = i (2 pi/L) Dn[n_d] G * X.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_convolve_x(const c2_t<T> *in, c2_t<T> *out0, c2_t<T> *out1, c2_t<T> *out2, LineMap m, int W,
              Radices rp, const c2_t<T> *__restrict__ tw, const double *__restrict__ G,
              const double *__restrict__ Dn, double twopi_L, int nout, int lt) {
-    using V = c2_t<T>;
     extern __shared__ unsigned char smem[];
-    const int n = m.n, pitch = n + 1;
-    V *A = reinterpret_cast<V *>(smem);
-    V *B = A + lt * pitch;
-    V *Cb = B + lt * pitch;
-    const long L0 = (long)blockIdx.x * lt;
-    load_tile(in, A, m, L0, lt, pitch, false);
-    __syncthreads();
-    V *X = fft_tile(A, B, lt, n, pitch, rp, tw, 0);
-    V *other = (X == A) ? B : A;
-    const int M = n;
-    // X <- G * X (FP64 product, rounded to T)
-    for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-        const int e = idx / lt, l = idx - e * lt;
-        const long L = L0 + l;
-        if (L < m.nlines) {
-            const int j = (int)(L / W), k = (int)(L - (long)j * W);
-            const double g = G[((long)e * M + j) * M + k];
-            V v = X[l * pitch + e];
-            v.x = (T)((double)v.x * g);
-            v.y = (T)((double)v.y * g);
-            X[l * pitch + e] = v;
-        }
-    }
-    __syncthreads();
-    if (nout == 1) {
-        V *r = fft_tile(X, other, lt, n, pitch, rp, tw, 1);
-        store_tile(out0, r, m, L0, lt, pitch, false);
-        return;
-    }
-    V *outs[3] = {out0, out1, out2};
-    for (int d = 0; d < 3; d++) {
-        for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-            const int e = idx / lt, l = idx - e * lt;
-            const long L = L0 + l;
-            if (L < m.nlines) {
-                const int j = (int)(L / W), k = (int)(L - (long)j * W);
-                const int nd = (d == 0) ? e : ((d == 1) ? j : k);
-                const double f = twopi_L * Dn[nd];
-                const V v = X[l * pitch + e];
-                V w;
-                w.x = (T)(-f * (double)v.y);
-                w.y = (T)(f * (double)v.x);
-                Cb[l * pitch + e] = w;
-            }
-        }
-        __syncthreads();
-        V *r = fft_tile(Cb, other, lt, n, pitch, rp, tw, 1);
-        store_tile(outs[d], r, m, L0, lt, pitch, false);
-        __syncthreads();
-    }
+    convolve_x_tile<T>(in, out0, out1, out2, m, W, rp, tw, G, Dn, twopi_L, nout, lt,
+                       (long)blockIdx.x * lt, smem);
 }
 
-// Adjoint of the ik gradient step: x-axis forward pass of three spectra,
-// out = sum_d (-i (2 pi / L) Dn[n_d] G) X_d, x-axis backward pass.  The
-// multiplier of the forward gradient, i (2 pi / L) Dn G, is odd and purely
-// imaginary, so its transpose as a real operator is its negative.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_divergence_x(const c2_t<T> *in0, const c2_t<T> *in1, const c2_t<T> *in2, c2_t<T> *out, LineMap m,
                int W, Radices rp, const c2_t<T> *__restrict__ tw, const double *__restrict__ G,
                const double *__restrict__ Dn, double twopi_L, int lt) {
-    using V = c2_t<T>;
     extern __shared__ unsigned char smem[];
-    const int n = m.n, pitch = n + 1;
-    V *A = reinterpret_cast<V *>(smem);
-    V *B = A + lt * pitch;
-    V *Cb = B + lt * pitch;
-    const long L0 = (long)blockIdx.x * lt;
-    const V *ins[3] = {in0, in1, in2};
-    const int M = n;
-    for (int d = 0; d < 3; d++) {
-        load_tile(ins[d], A, m, L0, lt, pitch, false);
-        __syncthreads();
-        const V *X = fft_tile(A, B, lt, n, pitch, rp, tw, 0);
-        for (int idx = threadIdx.x; idx < lt * n; idx += blockDim.x) {
-            const int e = idx / lt, l = idx - e * lt;
-            const long L = L0 + l;
-            if (L < m.nlines) {
-                const int j = (int)(L / W), k = (int)(L - (long)j * W);
-                const int nd = (d == 0) ? e : ((d == 1) ? j : k);
-                const double f = twopi_L * Dn[nd] * G[((long)e * M + j) * M + k];
-                const V v = X[l * pitch + e];
-                V w;
-                w.x = (T)(f * (double)v.y);
-                w.y = (T)(-f * (double)v.x);
-                if (d > 0)
-                    w = cadd(w, Cb[l * pitch + e]);
-                Cb[l * pitch + e] = w;
-            }
-        }
-        __syncthreads();
-    }
-    V *r = fft_tile(Cb, A, lt, n, pitch, rp, tw, 1);
-    store_tile(out, r, m, L0, lt, pitch, false);
-}
-
-bool make_radices(int n, Radices &rp) {
-    rp.n = 0;
-    static const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
-    for (int r : order) {
-        while (n % r == 0) {
-            if (rp.n >= kMaxPasses)
-                return false;
-            rp.r[rp.n++] = r;
-            n /= r;
-        }
-    }
-    return n == 1;
+    divergence_x_tile<T>(in0, in1, in2, out, m, W, rp, tw, G, Dn, twopi_L, lt,
+                         (long)blockIdx.x * lt, smem);
 }
 
 }  // namespace
@@ -508,19 +150,6 @@ extern "C" void p3mk_fft_destroy(p3mk_fft *f) {
 }
 
 namespace {
-
-// lines per workgroup: as many as fit nbuf LDS buffers, capped so that a
-// strided row of the tile spans at most 256 bytes
-template <typename T> int tile_lines(int n, int nbuf) {
-    const size_t line = (size_t)(n + 1) * sizeof(c2_t<T>);
-    int lt = (int)(kLdsBudget / (nbuf * line));
-    const int cap = 256 / (int)sizeof(c2_t<T>);
-    if (lt > cap)
-        lt = cap;
-    if (lt < 1)
-        lt = 1;
-    return lt;
-}
 
 template <typename T>
 int pass(p3mk_fft *f, c2_t<T> *data, const LineMap &m, int inv, bool contig, p3mk_queue *q) {
@@ -616,6 +245,148 @@ int divergence(p3mk_fft *f, const double *G, const double *Dn, void *const in[3]
 }
 
 }  // namespace
+
+// ---- batched passes over many systems ------------------------------------
+//
+// Block b belongs to the system s with blk[s] <= b < blk[s+1]; its tile
+// starts at line (b - blk[s]) * lt(M_s).  Buffers are the systems' own.
+
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_bfft_pass(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys, int axis, int slot,
+            int inv, int r2c) {
+    extern __shared__ unsigned char smem[];
+    using V = c2_t<T>;
+    const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
+    const BSys &S = sys[s];
+    const int M = S.M, W = r2c ? M / 2 + 1 : M;
+    LineMap m;
+    if (axis == kPassZ)
+        m = {M, 1, 1, M, (long)M * M};
+    else
+        m = {M, W, W, (long)M * W, (long)M * W};
+    const int lt = tile_lines<T>(M, 2);
+    fft_pass_tile<T>((V *)S.buf[slot], m, S.rp, (const V *)S.tw, inv, lt, axis == kPassZ ? 1 : 0,
+                     (long)(blockIdx.x - blk[s]) * lt, smem);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_bfft_r2c_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys, int in_slot,
+             int out_slot) {
+    extern __shared__ unsigned char smem[];
+    using V = c2_t<T>;
+    const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
+    const BSys &S = sys[s];
+    const int M = S.M, lt = tile_lines<T>(M, 2);
+    fft_r2c_z_tile<T>((const T *)S.buf[in_slot], (V *)S.buf[out_slot], (long)M * M, M, S.rp,
+                      (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_bfft_c2r_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys, int in_slot,
+             int out_slot) {
+    extern __shared__ unsigned char smem[];
+    using V = c2_t<T>;
+    const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
+    const BSys &S = sys[s];
+    const int M = S.M, lt = tile_lines<T>(M, 2);
+    fft_c2r_z_tile<T>((const V *)S.buf[in_slot], (T *)S.buf[out_slot], (long)M * M, M, S.rp,
+                      (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_bconvolve_x(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys, int nout,
+              int in_slot, int out_slot, int r2c) {
+    extern __shared__ unsigned char smem[];
+    using V = c2_t<T>;
+    const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
+    const BSys &S = sys[s];
+    const int M = S.M, W = r2c ? M / 2 + 1 : M;
+    const long nl = (long)M * W;
+    const LineMap m = {M, nl, nl, 0, nl};
+    const int lt = tile_lines<T>(M, nout == 3 ? 3 : 2);
+    V *o0 = (V *)S.buf[out_slot];
+    V *o1 = nout == 3 ? (V *)S.buf[out_slot + 1] : nullptr;
+    V *o2 = nout == 3 ? (V *)S.buf[out_slot + 2] : nullptr;
+    convolve_x_tile<T>((const V *)S.buf[in_slot], o0, o1, o2, m, W, S.rp, (const V *)S.tw, S.G, S.Dn,
+                       2.0 * 3.14159265358979323846 / S.L, nout, lt,
+                       (long)(blockIdx.x - blk[s]) * lt, smem);
+}
+
+}  // namespace
+
+namespace p3m {
+
+int bfft_lines_per_tile(const BForm &f, int M, int kind) {
+    const int nbuf = (kind == kPassX && !f.ad) ? 3 : 2;
+    return f.fp32 ? tile_lines<float>(M, nbuf) : tile_lines<double>(M, nbuf);
+}
+
+size_t bfft_lds_bytes(const BForm &f, int M, int kind) {
+    const int nbuf = (kind == kPassX && !f.ad) ? 3 : 2;
+    const size_t v = f.fp32 ? sizeof(float2) : sizeof(double2);
+    return (size_t)nbuf * bfft_lines_per_tile(f, M, kind) * (M + 1) * v;
+}
+
+int bfft_pass(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks, size_t lds, int axis,
+              int slot, int inv, p3mk_queue *q) {
+    if (nblocks <= 0)
+        return P3MK_OK;
+    if (f.fp32)
+        hipLaunchKernelGGL(k_bfft_pass<float>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, axis, slot, inv, f.r2c);
+    else
+        hipLaunchKernelGGL(k_bfft_pass<double>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, axis, slot, inv, f.r2c);
+    return p3mk_launch_check("batched fft pass");
+}
+
+int bfft_r2c_z(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks, size_t lds, int in_slot,
+               int out_slot, p3mk_queue *q) {
+    if (nblocks <= 0)
+        return P3MK_OK;
+    if (f.fp32)
+        hipLaunchKernelGGL(k_bfft_r2c_z<float>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, in_slot, out_slot);
+    else
+        hipLaunchKernelGGL(k_bfft_r2c_z<double>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, in_slot, out_slot);
+    return p3mk_launch_check("batched fft r2c");
+}
+
+int bfft_c2r_z(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks, size_t lds, int in_slot,
+               int out_slot, p3mk_queue *q) {
+    if (nblocks <= 0)
+        return P3MK_OK;
+    if (f.fp32)
+        hipLaunchKernelGGL(k_bfft_c2r_z<float>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, in_slot, out_slot);
+    else
+        hipLaunchKernelGGL(k_bfft_c2r_z<double>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, in_slot, out_slot);
+    return p3mk_launch_check("batched fft c2r");
+}
+
+int bfft_convolve_x(const BForm &f, const BSys *sys, const long *blk, int nsys, long nblocks, size_t lds,
+                    int nout, int in_slot, int out_slot, p3mk_queue *q) {
+    if (nblocks <= 0)
+        return P3MK_OK;
+    if (f.fp32)
+        hipLaunchKernelGGL(k_bconvolve_x<float>, dim3((unsigned)nblocks), dim3(kThreads), lds,
+                           p3mk_stream(q), sys, blk, nsys, nout, in_slot, out_slot, f.r2c);
+    else
+        hipLaunchKernelGGL(k_bconvolve_x<double>, dim3((unsigned)nblocks), dim3(kThreads),
+                           lds, p3mk_stream(q), sys, blk, nsys, nout, in_slot, out_slot,
+                           f.r2c);
+    return p3mk_launch_check("batched convolve");
+}
+
+}  // namespace p3m
 
 extern "C" int p3mk_fft_forward_zy(p3mk_fft *f, void *mesh, void *spec, p3mk_queue *q) {
     return f->g.fp32 ? forward_zy<float>(f, mesh, spec, q) : forward_zy<double>(f, mesh, spec, q);

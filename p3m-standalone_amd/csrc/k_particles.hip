@@ -24,18 +24,18 @@
 
 #include <new>
 
+#include "batch_desc.hpp"
 #include "device_common.hpp"
 #include "k_internal.hpp"
+#include "particle_bodies.hpp"
 
 using namespace p3m;
 
 namespace {
 
-constexpr int kThreads = 256;
+using namespace p3m::part;
 
 // ---- sort by mesh tile ------------------------------------------------------
-
-constexpr int kChunk = 1024;
 
 __device__ __forceinline__ int axis_base(double r, double Hi, int P, int M) {
     int b;
@@ -78,18 +78,6 @@ __global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *_
         chunks[t] = (count[t] + kChunk - 1) / kChunk;
 }
 
-// ---- halo-tile assignment ------------------------------------------------------
-
-// LDS accumulation is FP64 in both forms: fire-and-forget ds_add_f64 (no
-// returned value to wait for), and the FP32 form's products are summed
-// without rounding error growth; the mesh itself is stored in the form's
-// precision.
-// What a particle deposits (MODE): kCharge q W; kWeighted q g_i W with a
-// per-particle factor g (the adjoint's q g_d); kDipole
-// q sum_d g_{i,d} W^(d), W^(d) = W with the d-axis weight replaced by its
-// derivative (the adjoint of ad back-interpolation).
-enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
-
 template <int P, typename T, bool R2C, bool IL, int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
@@ -97,135 +85,17 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
               T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
               int n) {
     extern __shared__ unsigned char smem_raw[];
-    double *acc = reinterpret_cast<double *>(smem_raw);
-    constexpr int NM = IL ? 2 : 1;
-    const int R = Tt + P - 1 + (IL ? 1 : 0); // halo region side
-    const int npts = R * R * R;
-    const int nall = NM * npts;
     // work item -> (tile, chunk): binary search in the chunk prefix sums
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    long lo = 0, hi = ntiles;
-    while (hi - lo > 1) {
-        const long mid = (lo + hi) / 2;
-        if (cstart[mid] <= item)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const long tile = lo;
+    const long tile = item_tile(cstart, ntiles, item);
     const int chunk = item - cstart[tile];
     const int pb = tstart[tile] + chunk * kChunk;
     const int pe = min(tstart[tile + 1], pb + kChunk);
-    const int tz = (int)(tile % nt), ty = (int)((tile / nt) % nt), tx = (int)(tile / ((long)nt * nt));
-    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
-
-    for (int k = threadIdx.x; k < nall; k += blockDim.x)
-        acc[k] = 0.0;
-    __syncthreads();
-    for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        const double4 r = sorted[p];
-        T qv = (T)r.w;
-        T g0 = T(0), g1 = T(0), g2 = T(0);
-        if (MODE == kWeighted) {
-            qv = (T)(r.w * gv[perm[p]]);
-        } else if (MODE == kDipole) {
-            const int i = perm[p];
-            g0 = (T)gv[i];
-            g1 = (T)gv[(long)n + i];
-            g2 = (T)gv[2L * n + i];
-        }
-#pragma unroll
-        for (int ii = 0; ii < NM; ii++) {
-            int b0, b1, b2;
-            T w0[P], w1[P], w2[P];
-            const double half = 0.5 * ii;
-            if (MODE == kDipole) {
-                // W^(d) summed with g_d: fold g_d into the derivative weights
-                // of axis d, then deposit (d0 w1 w2 + w0 d1 w2 + w0 w1 d2)
-                T d0[P], d1[P], d2[P];
-                axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
-                axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
-                axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
-                int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
-                o0 += (o0 < 0) ? M : 0;
-                o1 += (o1 < 0) ? M : 0;
-                o2 += (o2 < 0) ? M : 0;
-                double *a = acc + ii * npts;
-                for (int a0 = 0; a0 < P; a0++) {
-#pragma unroll
-                    for (int a1 = 0; a1 < P; a1++) {
-                        const T u = qv * (g0 * d0[a0] * w1[a1] + g1 * w0[a0] * d1[a1]);
-                        const T v = qv * (g2 * w0[a0] * w1[a1]);
-                        const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
-#pragma unroll
-                        for (int a2 = 0; a2 < P; a2++)
-                            atomicAdd(&a[row + a2], (double)(u * w2[a2] + v * d2[a2]));
-                    }
-                }
-                continue;
-            }
-            axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
-            axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
-            axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
-            // local offsets inside the halo region (base is within T + 1 of
-            // the tile start, modulo the periodic wrap)
-            int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
-            o0 += (o0 < 0) ? M : 0;
-            o1 += (o1 < 0) ? M : 0;
-            o2 += (o2 < 0) ? M : 0;
-            double *a = acc + ii * npts;
-            for (int a0 = 0; a0 < P; a0++) {
-                const T t0 = qv * w0[a0];
-#pragma unroll
-                for (int a1 = 0; a1 < P; a1++) {
-                    const T t1 = t0 * w1[a1];
-                    const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
-#pragma unroll
-                    for (int a2 = 0; a2 < P; a2++)
-                        atomicAdd(&a[row + a2], (double)(t1 * w2[a2]));
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // add the region to the mesh (periodic wrap of the halo)
-    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
-        const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
-        const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
-        const long g = ((long)gx * M + gy) * M + gz;
-#pragma unroll
-        for (int ii = 0; ii < NM; ii++) {
-            const T v = (T)acc[ii * npts + k];
-            if (v != T(0)) {
-                if (R2C)
-                    atomicAdd(&mesh[g], v);
-                else
-                    atomicAdd(&mesh[2 * g + ii], v);
-            }
-        }
-    }
+    assign_item<P, T, R2C, IL, MODE>(sorted, pb, pe, tile, nt, Tt, M, Hi, mesh, perm, gv, n,
+                                     reinterpret_cast<double *>(smem_raw));
 }
-
-// ---- back-interpolation -----------------------------------------------------
-
-template <typename T, bool R2C>
-__device__ __forceinline__ T mesh_val(const T *__restrict__ m, long idx, int ii) {
-    return R2C ? m[idx] : m[2 * idx + ii];
-}
-
-template <typename T> struct Acc;
-template <> struct Acc<double> {
-    double s = 0.0;
-    __device__ __forceinline__ void add(double v) { s += v; }
-    __device__ __forceinline__ double value() const { return s; }
-};
-template <> struct Acc<float> {
-    CompSum<float> s;
-    __device__ __forceinline__ void add(float v) { s.add(v); }
-    __device__ __forceinline__ double value() const { return s.value(); }
-};
 
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
@@ -233,46 +103,8 @@ k_interp_ik(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
             int M, const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
             double prefac, double *__restrict__ fk) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    constexpr int NM = IL ? 2 : 1;
-    Acc<T> acc[3];
-    const double4 r = sorted[t];
-    // interlaced: each mesh contributes with 0.5 (charge-assign.c:674-675)
-    const T scale = (T)(IL ? 0.5 * prefac : prefac) * (T)r.w;
-    for (int ii = 0; ii < NM; ii++) {
-        int b0, b1, b2;
-        T w0[P], w1[P], w2[P];
-        const double half = 0.5 * ii;
-        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
-        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
-        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
-        int kk[P];
-#pragma unroll
-        for (int c = 0; c < P; c++)
-            kk[c] = wrap1(b2 + c, M);
-        for (int a = 0; a < P; a++) {
-            const int ia = wrap1(b0 + a, M);
-#pragma unroll
-            for (int b = 0; b < P; b++) {
-                const int jb = wrap1(b1 + b, M);
-                const T t1 = w0[a] * w1[b];
-                const long row = ((long)ia * M + jb) * M;
-#pragma unroll
-                for (int c = 0; c < P; c++) {
-                    const long idx = row + kk[c];
-                    const T B = scale * (t1 * w2[c]);
-                    acc[0].add(-mesh_val<T, R2C>(F0, idx, ii) * B);
-                    acc[1].add(-mesh_val<T, R2C>(F1, idx, ii) * B);
-                    acc[2].add(-mesh_val<T, R2C>(F2, idx, ii) * B);
-                }
-            }
-        }
-    }
-    const int i = perm[t];
-    fk[i] += acc[0].value();
-    fk[(long)n + i] += acc[1].value();
-    fk[2L * n + i] += acc[2].value();
+    if (t < n)
+        interp_ik_particle<P, T, R2C, IL>(t, n, sorted, perm, Hi, M, F0, F1, F2, prefac, fk, n);
 }
 
 template <int P, typename T, bool R2C, bool IL>
@@ -282,66 +114,9 @@ k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
             const double *__restrict__ selfb, double h, double *__restrict__ fk,
             double *__restrict__ f) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    constexpr int NM = IL ? 2 : 1;
-    Acc<T> acc[3];
-    const double4 r = sorted[t];
-    const double xi = r.x, yi = r.y, zi = r.z, qi = r.w;
-    // dQ_d = W'.W.W q/L; force_d -= prefac * phi * dQ_d, interlaced x 0.5
-    // (charge-assign.c:1123-1125, 1374-1377, 758-760)
-    const T scale = (T)((IL ? 0.5 * prefac : prefac) * (qi * Leni));
-    for (int ii = 0; ii < NM; ii++) {
-        int b0, b1, b2;
-        T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
-        const double half = 0.5 * ii;
-        axis_weights_d<P, T>(xi, Hi, half, M, b0, w0, d0);
-        axis_weights_d<P, T>(yi, Hi, half, M, b1, w1, d1);
-        axis_weights_d<P, T>(zi, Hi, half, M, b2, w2, d2);
-        int kk[P];
-#pragma unroll
-        for (int c = 0; c < P; c++)
-            kk[c] = wrap1(b2 + c, M);
-        for (int a = 0; a < P; a++) {
-            const int ia = wrap1(b0 + a, M);
-#pragma unroll
-            for (int b = 0; b < P; b++) {
-                const int jb = wrap1(b1 + b, M);
-                const long row = ((long)ia * M + jb) * M;
-                const T wx = d0[a] * w1[b], wy = w0[a] * d1[b], wz = w0[a] * w1[b];
-#pragma unroll
-                for (int c = 0; c < P; c++) {
-                    const T B = scale * mesh_val<T, R2C>(phi, row + kk[c], ii);
-                    acc[0].add(-B * (wx * w2[c]));
-                    acc[1].add(-B * (wy * w2[c]));
-                    acc[2].add(-B * (wz * d2[c]));
-                }
-            }
-        }
-    }
-    const int i = perm[t];
-    fk[i] += acc[0].value();
-    fk[(long)n + i] += acc[1].value();
-    fk[2L * n + i] += acc[2].value();
-    if (selfb != nullptr) {
-        // f -= sum_m' b(m') q^2 sin(2 pi m'.r / h)  (p3m-ad-self-forces.c:122-135)
-        const double q2 = qi * qi;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        int ind = 0;
-        for (int m0 = -1; m0 <= 1; m0++)
-            for (int m1 = -1; m1 <= 1; m1++)
-                for (int m2 = -1; m2 <= 1; m2++) {
-                    const double st =
-                        q2 * sin(2 * 3.14159265358979323846 * (m0 * xi / h + m1 * yi / h + m2 * zi / h));
-                    s0 += selfb[ind] * st;
-                    s1 += selfb[ind + 1] * st;
-                    s2 += selfb[ind + 2] * st;
-                    ind += 3;
-                }
-        f[i] -= s0;
-        f[(long)n + i] -= s1;
-        f[2L * n + i] -= s2;
-    }
+    if (t < n)
+        interp_ad_particle<P, T, R2C, IL>(t, n, sorted, perm, Hi, Leni, M, phi, prefac, selfb, h,
+                                          fk, f, n);
 }
 
 // ---- adjoint (vector-Jacobian product) of the k-space force ---------------
@@ -809,3 +584,182 @@ extern "C" int p3mk_adjoint_gather(const p3mk_geom *g, int ad, int n, const p3mk
     return g->r2c ? launch_adjoint<double, true>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu)
                   : launch_adjoint<double, false>(g, ad, n, s, q, gv, F, lam, prefac, selfb, dpos, dq, qu);
 }
+
+// ---- batched kernels over many systems ----------------------------------------
+//
+// Particles of system s occupy slots [p0_s, p0_s + n_s) of the concatenated
+// arrays; tiles of system s are global tiles [tile0_s, tile0_s + nt_s^3), so
+// after the sort system s's particles are again the slots [p0_s, p0_s + n_s).
+
+namespace {
+
+__global__ void k_btile_count(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
+                              const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, int *__restrict__ key,
+                              int *__restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot)
+        return;
+    const BSys &S = sys[psys[i]];
+    const int tx = axis_base(x[i], S.Hi, S.cao, S.M) / S.T, ty = axis_base(y[i], S.Hi, S.cao, S.M) / S.T,
+              tz = axis_base(z[i], S.Hi, S.cao, S.M) / S.T;
+    const int k = (int)(S.tile0 + (tx * S.nt + ty) * S.nt + tz);
+    key[i] = k;
+    atomicAdd(&count[k], 1);
+}
+
+template <typename T, bool R2C, bool IL>
+__global__ void __launch_bounds__(kThreads)
+k_bassign(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsys, long ntiles,
+          const int *__restrict__ start, const int *__restrict__ cstart,
+          const double4 *__restrict__ sorted) {
+    extern __shared__ unsigned char smem_raw[];
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
+        return;
+    const long tile = item_tile(cstart, ntiles, item);
+    const int s = fft::prefix_slot(tile0s, nsys, tile);
+    const BSys &S = sys[s];
+    const int chunk = item - cstart[tile];
+    const int pb = start[tile] + chunk * kChunk;
+    const int pe = min(start[tile + 1], pb + kChunk);
+    T *mesh = (T *)S.buf[kBufMesh];
+    double *acc = reinterpret_cast<double *>(smem_raw);
+    const long tl = tile - S.tile0;
+    switch (S.cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        assign_item<P, T, R2C, IL, kCharge>(sorted, pb, pe, tl, S.nt, S.T, S.M, S.Hi, mesh,         \
+                                            nullptr, nullptr, 0, acc);                            \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+    default: break;
+    }
+}
+
+template <typename T, bool R2C, bool IL, bool AD>
+__global__ void __launch_bounds__(kThreads)
+k_binterp(const BSys *__restrict__ sys, const int *__restrict__ p0s, int nsys, int ntot,
+          const double4 *__restrict__ sorted, const int *__restrict__ perm, double *__restrict__ fk,
+          double *__restrict__ ftot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot)
+        return;
+    const BSys &S = sys[fft::prefix_slot(p0s, nsys, t)];
+    if (AD) {
+        const T *phi = (const T *)S.buf[kBufMesh];
+        switch (S.cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        interp_ad_particle<P, T, R2C, IL>(t, ntot, sorted, perm, S.Hi, 1.0 / S.L, S.M, phi,         \
+                                          S.prefac, S.selfb, S.h, fk, ftot, ntot);                \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+        default: break;
+        }
+    } else {
+        const T *F0 = (const T *)S.buf[kBufF0], *F1 = (const T *)S.buf[kBufF0 + 1],
+                *F2 = (const T *)S.buf[kBufF0 + 2];
+        switch (S.cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        interp_ik_particle<P, T, R2C, IL>(t, ntot, sorted, perm, S.Hi, S.M, F0, F1, F2, S.prefac,   \
+                                          fk, ntot);                                              \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+        default: break;
+        }
+    }
+}
+
+template <typename T, bool R2C, bool IL>
+void launch_bassign(const BSys *sys, const long *tile0s, int nsys, long ntiles, long max_items,
+                    size_t lds, const int *start, const int *cstart, const double4 *sorted,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_bassign<T, R2C, IL>), dim3((unsigned)max_items), dim3(kThreads), lds, st,
+                       sys, tile0s, nsys, ntiles, start, cstart, sorted);
+}
+
+template <typename T, bool R2C, bool IL>
+void launch_binterp(bool ad, const BSys *sys, const int *p0s, int nsys, int ntot,
+                    const double4 *sorted, const int *perm, double *fk, double *ftot,
+                    hipStream_t st) {
+    const dim3 grid((ntot + kThreads - 1) / kThreads);
+    if (ad)
+        hipLaunchKernelGGL((k_binterp<T, R2C, IL, true>), grid, dim3(kThreads), 0, st, sys, p0s,
+                           nsys, ntot, sorted, perm, fk, ftot);
+    else
+        hipLaunchKernelGGL((k_binterp<T, R2C, IL, false>), grid, dim3(kThreads), 0, st, sys, p0s,
+                           nsys, ntot, sorted, perm, fk, ftot);
+}
+
+}  // namespace
+
+namespace p3m {
+
+int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, int ntot,
+               const double *x, const double *y, const double *z, const double *q, int *key,
+               int *count, int *start, int *cursor, int *perm, int *chunks, int *cstart,
+               int *scan_scratch, double4 *sorted, p3mk_queue *qu) {
+    (void)f;
+    if (ntot <= 0)
+        return P3MK_OK;
+    hipStream_t st = p3mk_stream(qu);
+    const int nb = (ntot + 255) / 256;
+    P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * (ntiles + 1), qu));
+    P3MK_TRY(p3mk_fill_zero(cursor, sizeof(int) * (ntiles + 1), qu));
+    hipLaunchKernelGGL(k_btile_count, dim3(nb), dim3(256), 0, st, ntot, sys, psys, x, y, z, key,
+                       count);
+    P3MK_TRY(p3mk_launch_check("batched tile count"));
+    P3MK_TRY(p3mk_exclusive_scan(count, ntiles, start, scan_scratch, qu));
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, ntot, x, y, z, q, key, start,
+                       cursor, perm, sorted);
+    P3MK_TRY(p3mk_launch_check("batched tile scatter"));
+    hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, st,
+                       count, ntiles, chunks);
+    P3MK_TRY(p3mk_launch_check("batched tile chunks"));
+    return p3mk_exclusive_scan(chunks, ntiles, cstart, scan_scratch, qu);
+}
+
+int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
+                 long max_items, size_t lds, const int *start, const int *cstart,
+                 const double4 *sorted, p3mk_queue *qu) {
+    if (max_items <= 0)
+        return P3MK_OK;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+    hipStream_t st = p3mk_stream(qu);
+#define GO(T, R, I) launch_bassign<T, R, I>(sys, tile0s, nsys, ntiles, max_items, lds, start, cstart, sorted, st)
+    if (f.fp32) {
+        if (f.r2c) { if (f.interlaced) GO(float, true, true); else GO(float, true, false); }
+        else { if (f.interlaced) GO(float, false, true); else GO(float, false, false); }
+    } else {
+        if (f.r2c) { if (f.interlaced) GO(double, true, true); else GO(double, true, false); }
+        else { if (f.interlaced) GO(double, false, true); else GO(double, false, false); }
+    }
+#undef GO
+    return p3mk_launch_check("batched assignment");
+}
+
+int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int ntot,
+                 const double4 *sorted, const int *perm, double *fk, double *ftot, p3mk_queue *qu) {
+    if (ntot <= 0)
+        return P3MK_OK;
+    hipStream_t st = p3mk_stream(qu);
+    const bool ad = f.ad != 0;
+#define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, p0s, nsys, ntot, sorted, perm, fk, ftot, st)
+    if (f.fp32) {
+        if (f.r2c) { if (f.interlaced) GO(float, true, true); else GO(float, true, false); }
+        else { if (f.interlaced) GO(float, false, true); else GO(float, false, false); }
+    } else {
+        if (f.r2c) { if (f.interlaced) GO(double, true, true); else GO(double, true, false); }
+        else { if (f.interlaced) GO(double, false, true); else GO(double, false, false); }
+    }
+#undef GO
+    return p3mk_launch_check("batched back-interpolation");
+}
+
+}  // namespace p3m

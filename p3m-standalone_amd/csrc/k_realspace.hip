@@ -31,6 +31,7 @@
 #include <cmath>
 #include <new>
 
+#include "batch_desc.hpp"
 #include "device_common.hpp"
 #include "erfcx_coeffs.hpp"
 #include "k_internal.hpp"
@@ -113,14 +114,17 @@ struct PairAcc {
     double f0, f1, f2, en;
 };
 
-// erfcx(x) = exp(x^2) erfc(x), x >= 0, from the generated piecewise tables
-__device__ __forceinline__ double erfcx_d(double x) {
+constexpr int kErfcxTab = p3m::kErfcxDoublePieces * (p3m::kErfcxDoubleDegree + 1);
+
+// erfcx(x) = exp(x^2) erfc(x), x >= 0, from the generated piecewise FP64
+// tables (a copy in LDS: the piece index differs from lane to lane)
+__device__ __forceinline__ double erfcx_d(double x, const double *tab) {
     const double y = 1.0 / (1.0 + 0.5 * x);
     const int K = p3m::kErfcxDoublePieces;
     int k = (int)(y * K);
     k = k > K - 1 ? K - 1 : k;
     const double t = (y * K - (double)k) - 0.5;
-    const double *c = p3m::kErfcxDoubleCoef[k];
+    const double *c = tab + k * (p3m::kErfcxDoubleDegree + 1);
     double acc = c[p3m::kErfcxDoubleDegree];
 #pragma unroll
     for (int d = p3m::kErfcxDoubleDegree - 1; d >= 0; d--)
@@ -128,28 +132,28 @@ __device__ __forceinline__ double erfcx_d(double x) {
     return acc;
 }
 
+// FP32 erfcx: one generated polynomial on [0, 6] (x clamped beyond, where
+// erfc < 3e-17 and the pair's contribution vanishes)
 __device__ __forceinline__ float erfcx_f(float x) {
-    const float y = 1.0f / (1.0f + 0.5f * x);
-    const int K = p3m::kErfcxFloatPieces;
-    int k = (int)(y * K);
-    k = k > K - 1 ? K - 1 : k;
-    const float t = (y * K - (float)k) - 0.5f;
-    const float *c = p3m::kErfcxFloatCoef[k];
-    float acc = c[p3m::kErfcxFloatDegree];
+    x = fminf(x, p3m::kErfcxSingleX1);
+    const float y = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
+    const float t = fmaf(y, p3m::kErfcxSingleA, p3m::kErfcxSingleB);
+    float acc = p3m::kErfcxSingleCoef[p3m::kErfcxSingleDegree];
 #pragma unroll
-    for (int d = p3m::kErfcxFloatDegree - 1; d >= 0; d--)
-        acc = fmaf(acc, t, c[d]);
+    for (int d = p3m::kErfcxSingleDegree - 1; d >= 0; d--)
+        acc = fmaf(acc, t, p3m::kErfcxSingleCoef[d]);
     return acc;
 }
 
-// Full pair evaluation with the reference's inclusion test r <= rcut
-// (realpart.c:134-148).  erfc(x) = exp(-x^2) erfcx(x) shares the one
-// exponential with the exp(-x^2) term of the force; x^2 is split exactly
-// (hi + lo by FMA) so exp(-x^2) keeps its accuracy for large x.
+// Pair evaluation with the reference's inclusion test r <= rcut
+// (realpart.c:134-148); dx is the reference's minimum-image difference
+// bit for bit.  erfc(x) = exp(-x^2) erfcx(x) shares the one exponential with
+// the exp(-x^2) term of the force; in FP64 x^2 is split exactly (hi + lo by
+// FMA) so exp(-x^2) keeps its accuracy for large x.
 template <bool FP32>
 __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, double sx, double sy,
                                      double sz, double rcut, double alpha, double two_a_spi,
-                                     PairAcc &acc) {
+                                     const double *tab, PairAcc &acc) {
     double dx, dy, dz, r2;
     {
 #pragma clang fp contract(off)
@@ -161,21 +165,23 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     // r <= rcut decided exactly (correctly rounded sqrt) only near the edge
     const double rc2 = rcut * rcut;
     bool inside = r2 < rc2 * (1.0 - 1e-12);
-    if (!inside && r2 <= rc2 * (1.0 + 1e-12))
+    const bool edge = !inside && r2 <= rc2 * (1.0 + 1e-12);
+    if (__builtin_expect(edge, 0))
         inside = sqrt(r2) <= rcut;
     if (!inside)
         return;
     const double qq = pi.w * pj.w;
     if (FP32) {
-        const float rinv = rsqrtf((float)r2);
-        const float x = (float)alpha * ((float)r2 * rinv);
-        const float e = expf(-x * x);
+        const float r2f = (float)r2;
+        const float rinv = __builtin_amdgcn_rsqf(r2f);
+        const float x = (float)alpha * (r2f * rinv);
+        const float e = __expf(-x * x);
         const float ec = e * erfcx_f(x);
         const float fak = (float)qq * (ec * rinv + (float)two_a_spi * e) * (rinv * rinv);
-        acc.f0 += (double)(fak * (float)dx);
-        acc.f1 += (double)(fak * (float)dy);
-        acc.f2 += (double)(fak * (float)dz);
-        acc.en += 0.5 * qq * (double)(ec * rinv);
+        acc.f0 = (double)(fak * (float)dx);
+        acc.f1 = (double)(fak * (float)dy);
+        acc.f2 = (double)(fak * (float)dz);
+        acc.en = 0.5 * qq * (double)(ec * rinv);
     } else {
         const double rinv = rsqrt(r2);
         const double x = alpha * (r2 * rinv);
@@ -183,196 +189,224 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
         const double x2lo = fma(x, x, -x2);
         const double e0 = exp(-x2);
         const double e = fma(-e0, x2lo, e0); // exp(-(x2 + x2lo))
-        const double ec = e * erfcx_d(x);
+        const double ec = e * erfcx_d(x, tab);
         const double fak = qq * (ec * rinv + two_a_spi * e) * (rinv * rinv);
-        acc.f0 += fak * dx;
-        acc.f1 += fak * dy;
-        acc.f2 += fak * dz;
-        acc.en += 0.5 * qq * ec * rinv;
+        acc.f0 = fak * dx;
+        acc.f1 = fak * dy;
+        acc.f2 = fak * dz;
+        acc.en = 0.5 * qq * ec * rinv;
     }
 }
 
-// Persistent kernel over work items (home cell, chunk of 64 home
-// particles).  Accepted candidates go to one flat queue per wavefront:
-// lanes that accept the same neighbour append at count + (their rank among
-// the accepting lanes), so entries are dense; every drain round hands 64
-// entries to the 64 lanes, whoever owns them, and the pair forces are added
-// to the owners' accumulators in LDS.  An entry packs the neighbour's sorted
-// index (bits 5..31) and its periodic-shift index (bits 0..4: 0..26 for
-// -1/0/+1 per axis); the owner lane is stored beside it.
+// LDS of one wavefront's work item: a ring of queued pairs (neighbour's
+// sorted index; owner lane and periodic shift packed beside it), the home
+// particles, the owners' force accumulators and, for FP64, the erfcx table.
+struct RsShared {
+    uint2 q[kQueue];
+    double4 home_pos[kWave];
+    double facc[3][kWave];
+    double tab[kErfcxTab];
+};
+
+__device__ __forceinline__ void load_erfcx_table(RsShared &sh) {
+    for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
+        sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
+    __syncthreads();
+}
+
+// queue entry: y = owner (bits 0..5) | shift index + 1 per axis (2 bits each
+// at 6, 8, 10: 0, 1, 2 for -L, 0, +L)
+__device__ __forceinline__ unsigned int pack_owner_shift(int lane, int wx, int wy, int wz) {
+    return (unsigned)lane | ((unsigned)(wx + 1) << 6) | ((unsigned)(wy + 1) << 8) |
+           ((unsigned)(wz + 1) << 10);
+}
+
+// One work item (home cell, chunk of 64 home particles).  Lanes own the 64
+// home particles and walk the (2 reach + 1)^3 neighbour cells together;
+// neighbours arrive in chunks of 64 with one coalesced load and are
+// broadcast lane by lane (readlane).  A cheap FP32 distance test in the home
+// cell's frame accepts ~30% of the candidates; the accepted pairs of all
+// lanes go to one ring queue per wavefront (rank by mbcnt over the ballot),
+// and every 64 queued pairs are evaluated by the 64 lanes at once, whoever
+// owns them, with the forces added to the owners' LDS accumulators.  Cells
+// of the item's system are cell0 + (cx nc + cy) nc + cz; forces go to
+// fr[d stride + perm[i]].
 template <bool FP32, bool SMALL_BOX>
-__global__ void __launch_bounds__(kWave)
-k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
-            const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
-            int reach, double L, double rcut, double alpha, int *__restrict__ counter,
-            double *__restrict__ fr, double *__restrict__ energy) {
-    __shared__ unsigned int qe[kQueue];
-    __shared__ unsigned char qo[kQueue];
-    __shared__ double4 home_pos[kWave];
-    __shared__ double facc[3][kWave];
+__device__ __forceinline__ void
+realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
+               const int *__restrict__ perm, const int *__restrict__ start, int nc, int reach,
+               double L, double rcut, double alpha, long stride, double *__restrict__ fr,
+               RsShared &sh, double &en_tot) {
+    constexpr int kMask = kQueue - 1;
     const int lane = threadIdx.x;
-    const int total = istart[ncell];
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
     const double Li = 1.0 / L, side = L / nc;
     // prefilter thresholds; the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
     const float rc2f = (float)(rcut * rcut) * (1.0f + 1e-5f);
     const int lo = SMALL_BOX ? 0 : -reach, hi = SMALL_BOX ? nc - 1 : reach;
-    double en_tot = 0.0;
+    const int hb = start[cell0 + home], he = start[cell0 + home + 1];
+    const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
+    const int i0 = hb + chunk * kWave; // sorted index of lane 0's home particle
+    const int i = i0 + lane;
+    const bool active = i < he;
+    const unsigned long long act = __ballot(active);
+    const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    // home-cell-relative single-precision coordinates for the prefilter
+    const double ox = hx * side, oy = hy * side, oz = hz * side;
+    const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
+    sh.home_pos[lane] = pi;
+    sh.facc[0][lane] = 0.0;
+    sh.facc[1][lane] = 0.0;
+    sh.facc[2][lane] = 0.0;
+    int head = 0, count = 0; // wave-uniform ring state
+    __syncthreads();
 
-    for (;;) {
-        int item = 0;
-        if (lane == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= total)
-            break;
-        // item -> home cell (binary search in the item prefix sums)
-        long clo = 0, chi = ncell;
-        while (chi - clo > 1) {
-            const long mid = (clo + chi) / 2;
-            if (istart[mid] <= item)
-                clo = mid;
-            else
-                chi = mid;
+    // lane l evaluates ring entry head + l (l < m) for its owner
+    auto drain = [&](int m) {
+        if (lane < m) {
+            const uint2 e = sh.q[(head + lane) & kMask];
+            const int o = (int)(e.y & 63u);
+            const double sx = (double)((int)((e.y >> 6) & 3u) - 1) * L;
+            const double sy = (double)((int)((e.y >> 8) & 3u) - 1) * L;
+            const double sz = (double)((int)((e.y >> 10) & 3u) - 1) * L;
+            PairAcc pa = {0.0, 0.0, 0.0, 0.0};
+            pair<FP32>(sh.home_pos[o], sorted[e.x], sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+            atomicAdd(&sh.facc[0][o], pa.f0);
+            atomicAdd(&sh.facc[1][o], pa.f1);
+            atomicAdd(&sh.facc[2][o], pa.f2);
+            en_tot += pa.en;
         }
-        const int home = (int)clo;
-        const int chunk = item - istart[home];
-        const int hb = start[home], he = start[home + 1];
-        const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
-        const int i = hb + chunk * kWave + lane;
-        const bool active = i < he;
-        const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-        // home-cell-relative single-precision coordinates for the prefilter
-        const double ox = hx * side, oy = hy * side, oz = hz * side;
-        const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
-        home_pos[lane] = pi;
-        facc[0][lane] = 0.0;
-        facc[1][lane] = 0.0;
-        facc[2][lane] = 0.0;
-        int count = 0; // wave-uniform queue length
+        head = (head + m) & kMask;
+        count -= m;
         __syncthreads();
+    };
 
-        // one round: lane l takes entry l (if l < m), evaluates it and adds
-        // the force to the owner's accumulator
-        auto drain_round = [&](int m) {
-            if (lane < m) {
-                const unsigned int e = qe[lane];
-                const int o = qo[lane];
-                const int j = (int)(e >> 5), sh = (int)(e & 31u);
-                const double sx = (double)(sh / 9 - 1) * L, sy = (double)((sh / 3) % 3 - 1) * L,
-                             sz = (double)(sh % 3 - 1) * L;
-                PairAcc pa = {0.0, 0.0, 0.0, 0.0};
-                pair<FP32>(home_pos[o], sorted[j], sx, sy, sz, rcut, alpha, two_a_spi, pa);
-                if (pa.f0 != 0.0 || pa.f1 != 0.0 || pa.f2 != 0.0) {
-                    atomicAdd(&facc[0][o], pa.f0);
-                    atomicAdd(&facc[1][o], pa.f1);
-                    atomicAdd(&facc[2][o], pa.f2);
-                }
-                en_tot += pa.en;
-            }
-            __syncthreads();
-            // move the rest (< 64 entries) to the front
-            const int rest = count - m;
-            unsigned int e2 = 0;
-            unsigned char o2 = 0;
-            if (lane < rest) {
-                e2 = qe[m + lane];
-                o2 = qo[m + lane];
-            }
-            __syncthreads();
-            if (lane < rest) {
-                qe[lane] = e2;
-                qo[lane] = o2;
-            }
-            count = rest;
-            __syncthreads();
-        };
-
-        auto append = [&](bool ok, unsigned int entry) {
-            const unsigned long long mask = __ballot(ok);
-            if (mask) {
+    // lanes in `mask` queue the pair (own home particle, neighbour j)
+    auto append = [&](unsigned long long mask, unsigned int j, unsigned int os) {
+        if (mask) {
+            if ((mask >> lane) & 1ull) {
                 const unsigned int rank = __builtin_amdgcn_mbcnt_hi(
                     (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
-                if (ok) {
-                    qe[count + rank] = entry;
-                    qo[count + rank] = (unsigned char)lane;
-                }
-                count += __popcll(mask);
-                if (count >= kWave) {
-                    __syncthreads();
-                    drain_round(kWave);
-                }
+                sh.q[(head + count + (int)rank) & kMask] = make_uint2(j, os);
             }
-        };
+            count += __popcll(mask);
+            if (count >= kWave) {
+                __syncthreads();
+                drain(kWave);
+            }
+        }
+    };
 
-        for (int a = lo; a <= hi; a++) {
-            int cx = SMALL_BOX ? a : hx + a, wx = 0;
-            if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
-            for (int b = lo; b <= hi; b++) {
-                int cy = SMALL_BOX ? b : hy + b, wy = 0;
-                if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
-                for (int c = lo; c <= hi; c++) {
-                    int cz = SMALL_BOX ? c : hz + c, wz = 0;
-                    if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
-                    const int cell = (cx * nc + cy) * nc + cz;
-                    const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
-                    const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-                    const double sx = wx * L, sy = wy * L, sz = wz * L;
-                    const unsigned int sidx = (unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1));
-                    // neighbours in chunks of 64: one coalesced load, then
-                    // lane k's particle is broadcast to all lanes
-                    for (int cb = jb; cb < je; cb += kWave) {
-                        const int jj = cb + lane;
-                        const double4 mine = (jj < je) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
-                        const int cnt = min(kWave, je - cb);
-                        if (SMALL_BOX) {
-                            for (int k = 0; k < cnt; k++) {
-                                const int j = cb + k;
-                                double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
-                                       dz = pi.z - bcast(mine.z, k);
-                                // nearest image per axis (the reference's round(dx/L))
-                                const double rx = round(dx * Li), ry = round(dy * Li), rz = round(dz * Li);
-                                dx -= rx * L;
-                                dy -= ry * L;
-                                dz -= rz * L;
-                                const unsigned int sh =
-                                    (unsigned)(((int)rx + 1) * 9 + ((int)ry + 1) * 3 + ((int)rz + 1));
-                                const double r2 = dx * dx + dy * dy + dz * dz;
-                                append(active && j != i && r2 <= rc2, ((unsigned)j << 5) | sh);
-                            }
-                        } else {
-                            // neighbour in the home cell's frame, image shift included
-                            const float mxf = (float)((mine.x + sx) - ox), myf = (float)((mine.y + sy) - oy),
-                                        mzf = (float)((mine.z + sz) - oz);
-                            for (int k = 0; k < cnt; k++) {
-                                const int j = cb + k;
-                                const float dx = pxf - bcastf(mxf, k);
-                                const float dy = pyf - bcastf(myf, k);
-                                const float dz = pzf - bcastf(mzf, k);
-                                const float r2 = dx * dx + dy * dy + dz * dz;
-                                append(active && j != i && r2 <= rc2f, ((unsigned)j << 5) | sidx);
-                            }
+    for (int a = lo; a <= hi; a++) {
+        int cx = SMALL_BOX ? a : hx + a, wx = 0;
+        if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
+        for (int b = lo; b <= hi; b++) {
+            int cy = SMALL_BOX ? b : hy + b, wy = 0;
+            if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
+            for (int c = lo; c <= hi; c++) {
+                int cz = SMALL_BOX ? c : hz + c, wz = 0;
+                if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
+                const int cell = cell0 + (cx * nc + cy) * nc + cz;
+                const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
+                const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+                const double sx = wx * L, sy = wy * L, sz = wz * L;
+                const unsigned int os_cell = pack_owner_shift(lane, wx, wy, wz);
+                // neighbours in chunks of 64: one coalesced load, then
+                // lane k's particle is broadcast to all lanes
+                for (int cb = jb; cb < je; cb += kWave) {
+                    const int jj = cb + lane;
+                    const double4 mine = (jj < je) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
+                    const int cnt = min(kWave, je - cb);
+                    if (SMALL_BOX) {
+                        for (int k = 0; k < cnt; k++) {
+                            const int j = cb + k;
+                            double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
+                                   dz = pi.z - bcast(mine.z, k);
+                            // nearest image per axis (the reference's round(dx/L))
+                            const double rx = round(dx * Li), ry = round(dy * Li), rz = round(dz * Li);
+                            dx -= rx * L;
+                            dy -= ry * L;
+                            dz -= rz * L;
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            const bool ok = active && j != i && r2 <= rc2;
+                            append(__ballot(ok), (unsigned)j,
+                                   pack_owner_shift(lane, (int)rx, (int)ry, (int)rz));
+                        }
+                    } else {
+                        // neighbour in the home cell's frame, image shift included
+                        const float mxf = (float)((mine.x + sx) - ox), myf = (float)((mine.y + sy) - oy),
+                                    mzf = (float)((mine.z + sz) - oz);
+                        // the home particles' own slots (self pairs) among this chunk
+                        const int self0 = i0 - cb; // neighbour k is home lane k - self0
+                        for (int k = 0; k < cnt; k++) {
+                            const float dx = pxf - bcastf(mxf, k);
+                            const float dy = pyf - bcastf(myf, k);
+                            const float dz = pzf - bcastf(mzf, k);
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            unsigned long long mask = __ballot(r2 <= rc2f) & act;
+                            const int sl = k - self0;
+                            if (sl >= 0 && sl < kWave)
+                                mask &= ~(1ull << sl);
+                            append(mask, (unsigned)(cb + k), os_cell);
                         }
                     }
                 }
             }
         }
-        __syncthreads();
-        if (count > 0)
-            drain_round(count);
-        if (active) {
-            const int p = perm[i];
-            atomicAdd(&fr[p], facc[0][lane]);
-            atomicAdd(&fr[(long)n + p], facc[1][lane]);
-            atomicAdd(&fr[2L * n + p], facc[2][lane]);
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (count > 0)
+        drain(count);
+    if (active) {
+        const int p = perm[i];
+        atomicAdd(&fr[p], sh.facc[0][lane]);
+        atomicAdd(&fr[stride + p], sh.facc[1][lane]);
+        atomicAdd(&fr[2 * stride + p], sh.facc[2][lane]);
+    }
+    __syncthreads();
+}
+
+// last index k with prefix[k] <= v
+__device__ __forceinline__ long prefix_find(const int *__restrict__ prefix, long count, int v) {
+    long lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const long mid = (lo + hi) / 2;
+        if (prefix[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Persistent grid: each single-wavefront workgroup claims work items from an
+// atomic counter until none are left.
+template <bool FP32, bool SMALL_BOX>
+__global__ void __launch_bounds__(kWave)
+k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
+            const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
+            int reach, double L, double rcut, double alpha, int *__restrict__ counter,
+            double *__restrict__ fr, double *__restrict__ energy) {
+    __shared__ RsShared sh;
+    const int total = istart[ncell];
+    double en_tot = 0.0;
+    load_erfcx_table(sh);
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        const int home = (int)prefix_find(istart, ncell, item);
+        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, perm, start, nc,
+                                        reach, L, rcut, alpha, n, fr, sh, en_tot);
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
             en_tot += __shfl_down(en_tot, off, kWave);
-        if (lane == 0 && en_tot != 0.0)
+        if (threadIdx.x == 0 && en_tot != 0.0)
             atomicAdd(energy, en_tot);
     }
 }
@@ -497,3 +531,121 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
 #undef LAUNCH
     return p3mk_launch_check("realspace");
 }
+
+// ---- batched real space over many systems -----------------------------------
+//
+// Cells of system s are global cells [cell0_s, cell0_s + nc_s^3); one
+// persistent launch claims work items of all systems from one counter.
+
+namespace {
+
+using p3m::BSys;
+
+__global__ void k_bcell_count(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
+                              const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, int *__restrict__ cell_id,
+                              int *__restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot)
+        return;
+    const BSys &S = sys[psys[i]];
+    const int nc = S.nc;
+    const double L = S.L, inv_side = (double)nc / L;
+    const int c = S.cell0 + (cell_of(fold(x[i], L), inv_side, nc) * nc +
+                             cell_of(fold(y[i], L), inv_side, nc)) * nc +
+                  cell_of(fold(z[i], L), inv_side, nc);
+    cell_id[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+__global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
+                                const double *__restrict__ x, const double *__restrict__ y,
+                                const double *__restrict__ z, const double *__restrict__ q,
+                                const int *__restrict__ cell_id, const int *__restrict__ start,
+                                int *__restrict__ cursor, int *__restrict__ perm,
+                                double4 *__restrict__ sorted) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot)
+        return;
+    const double L = sys[psys[i]].L;
+    const int c = cell_id[i];
+    const int pos = start[c] + atomicAdd(&cursor[c], 1);
+    perm[pos] = i;
+    sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+}
+
+template <bool FP32>
+__global__ void __launch_bounds__(kWave)
+k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
+             const double4 *__restrict__ sorted, const int *__restrict__ perm,
+             const int *__restrict__ start, const int *__restrict__ istart, long ncell,
+             int *__restrict__ counter, double *__restrict__ fr, double *__restrict__ energy) {
+    __shared__ RsShared sh;
+    const int total = istart[ncell];
+    load_erfcx_table(sh);
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        const int home = (int)prefix_find(istart, ncell, item);
+        const int s = p3m::fft::prefix_slot(cell0s, nsys, (long)home);
+        const BSys &S = sys[s];
+        if (!(S.rcut > 0.0))
+            continue; // no real-space part for this system
+        double en = 0.0;
+        if (S.small)
+            realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
+                                       start, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fr, sh, en);
+        else
+            realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
+                                        start, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fr, sh, en);
+        if (energy != nullptr) {
+            for (int off = 32; off > 0; off >>= 1)
+                en += __shfl_down(en, off, kWave);
+            if (threadIdx.x == 0 && en != 0.0)
+                atomicAdd(&energy[s], en);
+        }
+    }
+}
+
+}  // namespace
+
+namespace p3m {
+
+int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
+               long ncell, int ntot, const double *x, const double *y, const double *z,
+               const double *q, int *cell_id, int *count, int *start, int *cursor, int *perm,
+               int *items, int *istart, int *scan_scratch, int *counter, double4 *sorted,
+               int grid, double *fr, double *energy, p3mk_queue *qu) {
+    if (ntot <= 0)
+        return P3MK_OK;
+    hipStream_t s = p3mk_stream(qu);
+    const int nb = (ntot + 255) / 256;
+    P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * (ncell + 1), qu));
+    P3MK_TRY(p3mk_fill_zero(cursor, sizeof(int) * (ncell + 1), qu));
+    hipLaunchKernelGGL(k_bcell_count, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, cell_id,
+                       count);
+    P3MK_TRY(p3mk_launch_check("batched cell count"));
+    P3MK_TRY(p3mk_exclusive_scan(count, ncell, start, scan_scratch, qu));
+    hipLaunchKernelGGL(k_bcell_scatter, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, q,
+                       cell_id, start, cursor, perm, sorted);
+    P3MK_TRY(p3mk_launch_check("batched cell scatter"));
+    hipLaunchKernelGGL(k_cell_items, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, count,
+                       ncell, 1, items);
+    P3MK_TRY(p3mk_launch_check("batched cell items"));
+    P3MK_TRY(p3mk_exclusive_scan(items, ncell, istart, scan_scratch, qu));
+    P3MK_TRY(p3mk_fill_zero(counter, sizeof(int) * 4, qu));
+    if (f.fp32)
+        hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
+                           nsys, ntot, sorted, perm, start, istart, ncell, counter, fr, energy);
+    else
+        hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
+                           cell0s, nsys, ntot, sorted, perm, start, istart, ncell, counter, fr,
+                           energy);
+    return p3mk_launch_check("batched real space");
+}
+
+}  // namespace p3m

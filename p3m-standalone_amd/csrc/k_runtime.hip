@@ -308,3 +308,52 @@ extern "C" int p3mk_check_positions(int n, double L, const double *x, const doub
         return rc;
     return p3mk_queue_sync(q);
 }
+
+struct p3mk_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" int p3mk_capture_begin(p3mk_queue *q) {
+    hipError_t e = hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipStreamBeginCapture", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_capture_end(p3mk_queue *q, p3mk_graph **out) {
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(q->stream, &g);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipStreamEndCapture", (int)e);
+    p3mk_graph *pg = new (std::nothrow) p3mk_graph;
+    if (!pg) {
+        (void)hipGraphDestroy(g);
+        return p3mk_fail(P3MK_ENOMEM, "graph allocation", 0);
+    }
+    pg->graph = g;
+    e = hipGraphInstantiate(&pg->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete pg;
+        return p3mk_fail(P3MK_EDEVICE, "hipGraphInstantiate", (int)e);
+    }
+    *out = pg;
+    return P3MK_OK;
+}
+
+extern "C" int p3mk_graph_launch(p3mk_graph *g, p3mk_queue *q) {
+    hipError_t e = hipGraphLaunch(g->exec, q->stream);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipGraphLaunch", (int)e);
+    return P3MK_OK;
+}
+
+extern "C" void p3mk_graph_destroy(p3mk_graph *g) {
+    if (!g)
+        return;
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+}

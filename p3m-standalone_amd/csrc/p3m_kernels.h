@@ -155,6 +155,52 @@ int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, const double
                    const double *z, const double *q, double rcut, double alpha, int fp32,
                    double *fr, double *energy, p3mk_queue *qu);
 
+/* ---- batch of independent systems ------------------------------------ */
+typedef struct {
+    int n;         /* particles (slots [sum of earlier n, + n) of the arrays) */
+    double L;      /* box length */
+    int M, cao;    /* mesh (even, 13-smooth), assignment order 1..7 */
+    double rcut;   /* real-space cutoff (0: no real-space part), <= L/2 */
+    double alpha;  /* splitting parameter */
+    double prefac; /* back-interpolation prefactor */
+} p3mk_bsys_in;
+typedef struct p3mk_batch p3mk_batch;
+int p3mk_batch_create(p3mk_batch **b, int r2c, int interlaced, int ad, int fp32, int nsys,
+                      const p3mk_bsys_in *sys);
+void p3mk_batch_destroy(p3mk_batch *b);
+/* device arrays of system s: influence function (M^3), Dn (M), self-force
+ * coefficients (81) */
+int p3mk_batch_system_arrays(p3mk_batch *b, int s, double **G, double **Dn, double **selfb);
+/* precision, self-force correction, assignment tile side (4 or 8) and the
+ * real-space persistent grid (0: keep) */
+int p3mk_batch_configure(p3mk_batch *b, int fp32, int use_self_force, int tile_side, int rs_grid);
+int p3mk_batch_set_alpha(p3mk_batch *b, int s, double alpha);
+int p3mk_batch_info(const p3mk_batch *b, int *ntot, long *ntiles, long *ncell, int *tile_side,
+                    int *rs_grid);
+/* fk (3 blocks of ntot) += k-space force; ftot -= self force when enabled */
+int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y, const double *z,
+                      const double *q, double *fk, double *ftot, p3mk_queue *q_);
+/* fr (3 blocks of ntot) += real-space force; energy[s] += system energy */
+int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double *y, const double *z,
+                         const double *q, double *fr, double *energy, p3mk_queue *q_);
+
+/* ---- Ewald sum at sampled particles --------------------------------------- */
+/* f_total (3 ns) += Ewald force (k space: lattice vectors n^2 <= kmax^2 at
+ * alpha; real space: pairs within rcut <= L/2) on particles idx[0..ns);
+ * f_split (3 ns, may be NULL) += that force minus the real-space sum at
+ * alpha2 within rcut, i.e. the k-space force at alpha2.  Synchronises q. */
+int p3mk_ewald_sample(int n, double L, const double *x, const double *y, const double *z,
+                      const double *q, int ns, const int *idx, double alpha, double rcut, int kmax,
+                      double alpha2, double *f_total, double *f_split, p3mk_queue *q_);
+
+/* ---- graphs ------------------------------------------------------------- */
+typedef struct p3mk_graph p3mk_graph;
+/* start capturing the work enqueued on q; end it into an executable graph */
+int p3mk_capture_begin(p3mk_queue *q);
+int p3mk_capture_end(p3mk_queue *q, p3mk_graph **g);
+int p3mk_graph_launch(p3mk_graph *g, p3mk_queue *q);
+void p3mk_graph_destroy(p3mk_graph *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,272 @@
+// Per-item / per-particle bodies of charge assignment and back-interpolation,
+// shared by the single-system kernels and the batched kernels.  Particles
+// are in tile-sorted order (double4 x, y, z, q) with `perm` mapping a sorted
+// slot to the caller's index.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace p3m {
+namespace part {
+
+constexpr int kThreads = 256;
+constexpr int kChunk = 1024; // particles per assignment work item
+
+// work item -> tile: the last tile whose chunk prefix sum is <= item
+__device__ __forceinline__ long item_tile(const int *__restrict__ cstart, long ntiles, int item) {
+    long lo = 0, hi = ntiles;
+    while (hi - lo > 1) {
+        const long mid = (lo + hi) / 2;
+        if (cstart[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ---- halo-tile assignment ------------------------------------------------------
+
+// LDS accumulation is FP64 in both forms: fire-and-forget ds_add_f64 (no
+// returned value to wait for), and the FP32 form's products are summed
+// without rounding error growth; the mesh itself is stored in the form's
+// precision.
+// What a particle deposits (MODE): kCharge q W; kWeighted q g_i W with a
+// per-particle factor g (the adjoint's q g_d); kDipole
+// q sum_d g_{i,d} W^(d), W^(d) = W with the d-axis weight replaced by its
+// derivative (the adjoint of ad back-interpolation).
+enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
+
+template <int P, typename T, bool R2C, bool IL, int MODE>
+__device__ __forceinline__ void
+assign_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, int nt, int Tt, int M,
+            double Hi, T *__restrict__ mesh, const int *__restrict__ perm,
+            const double *__restrict__ gv, int n, double *acc) {
+    constexpr int NM = IL ? 2 : 1;
+    const int R = Tt + P - 1 + (IL ? 1 : 0); // halo region side
+    const int npts = R * R * R;
+    const int nall = NM * npts;
+    const int tz = (int)(tl % nt), ty = (int)((tl / nt) % nt), tx = (int)(tl / ((long)nt * nt));
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+
+    for (int k = threadIdx.x; k < nall; k += blockDim.x)
+        acc[k] = 0.0;
+    __syncthreads();
+    for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+        const double4 r = sorted[p];
+        T qv = (T)r.w;
+        T g0 = T(0), g1 = T(0), g2 = T(0);
+        if (MODE == kWeighted) {
+            qv = (T)(r.w * gv[perm[p]]);
+        } else if (MODE == kDipole) {
+            const int i = perm[p];
+            g0 = (T)gv[i];
+            g1 = (T)gv[(long)n + i];
+            g2 = (T)gv[2L * n + i];
+        }
+#pragma unroll
+        for (int ii = 0; ii < NM; ii++) {
+            int b0, b1, b2;
+            T w0[P], w1[P], w2[P];
+            const double half = 0.5 * ii;
+            if (MODE == kDipole) {
+                // W^(d) summed with g_d: fold g_d into the derivative weights
+                // of axis d, then deposit (d0 w1 w2 + w0 d1 w2 + w0 w1 d2)
+                T d0[P], d1[P], d2[P];
+                axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
+                axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
+                axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
+                int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+                o0 += (o0 < 0) ? M : 0;
+                o1 += (o1 < 0) ? M : 0;
+                o2 += (o2 < 0) ? M : 0;
+                double *a = acc + ii * npts;
+                for (int a0 = 0; a0 < P; a0++) {
+#pragma unroll
+                    for (int a1 = 0; a1 < P; a1++) {
+                        const T u = qv * (g0 * d0[a0] * w1[a1] + g1 * w0[a0] * d1[a1]);
+                        const T v = qv * (g2 * w0[a0] * w1[a1]);
+                        const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
+#pragma unroll
+                        for (int a2 = 0; a2 < P; a2++)
+                            atomicAdd(&a[row + a2], (double)(u * w2[a2] + v * d2[a2]));
+                    }
+                }
+                continue;
+            }
+            axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+            axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+            axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
+            // local offsets inside the halo region (base is within T + 1 of
+            // the tile start, modulo the periodic wrap)
+            int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+            o0 += (o0 < 0) ? M : 0;
+            o1 += (o1 < 0) ? M : 0;
+            o2 += (o2 < 0) ? M : 0;
+            double *a = acc + ii * npts;
+            for (int a0 = 0; a0 < P; a0++) {
+                const T t0 = qv * w0[a0];
+#pragma unroll
+                for (int a1 = 0; a1 < P; a1++) {
+                    const T t1 = t0 * w1[a1];
+                    const int row = ((o0 + a0) * R + (o1 + a1)) * R + o2;
+#pragma unroll
+                    for (int a2 = 0; a2 < P; a2++)
+                        atomicAdd(&a[row + a2], (double)(t1 * w2[a2]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // add the region to the mesh (periodic wrap of the halo)
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+        const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
+        const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
+        const long g = ((long)gx * M + gy) * M + gz;
+#pragma unroll
+        for (int ii = 0; ii < NM; ii++) {
+            const T v = (T)acc[ii * npts + k];
+            if (v != T(0)) {
+                if (R2C)
+                    atomicAdd(&mesh[g], v);
+                else
+                    atomicAdd(&mesh[2 * g + ii], v);
+            }
+        }
+    }
+}
+
+// ---- back-interpolation -----------------------------------------------------
+
+template <typename T, bool R2C>
+__device__ __forceinline__ T mesh_val(const T *__restrict__ m, long idx, int ii) {
+    return R2C ? m[idx] : m[2 * idx + ii];
+}
+
+template <typename T> struct Acc;
+template <> struct Acc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+template <> struct Acc<float> {
+    CompSum<float> s;
+    __device__ __forceinline__ void add(float v) { s.add(v); }
+    __device__ __forceinline__ double value() const { return s.value(); }
+};
+
+template <int P, typename T, bool R2C, bool IL>
+__device__ __forceinline__ void
+interp_ik_particle(int t, int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
+            int M, const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
+            double prefac, double *__restrict__ fk, long stride) {
+    constexpr int NM = IL ? 2 : 1;
+    Acc<T> acc[3];
+    const double4 r = sorted[t];
+    // interlaced: each mesh contributes with 0.5 (charge-assign.c:674-675)
+    const T scale = (T)(IL ? 0.5 * prefac : prefac) * (T)r.w;
+    for (int ii = 0; ii < NM; ii++) {
+        int b0, b1, b2;
+        T w0[P], w1[P], w2[P];
+        const double half = 0.5 * ii;
+        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
+        int kk[P];
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            kk[c] = wrap1(b2 + c, M);
+        for (int a = 0; a < P; a++) {
+            const int ia = wrap1(b0 + a, M);
+#pragma unroll
+            for (int b = 0; b < P; b++) {
+                const int jb = wrap1(b1 + b, M);
+                const T t1 = w0[a] * w1[b];
+                const long row = ((long)ia * M + jb) * M;
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const long idx = row + kk[c];
+                    const T B = scale * (t1 * w2[c]);
+                    acc[0].add(-mesh_val<T, R2C>(F0, idx, ii) * B);
+                    acc[1].add(-mesh_val<T, R2C>(F1, idx, ii) * B);
+                    acc[2].add(-mesh_val<T, R2C>(F2, idx, ii) * B);
+                }
+            }
+        }
+    }
+    const int i = perm[t];
+    fk[i] += acc[0].value();
+    fk[stride + i] += acc[1].value();
+    fk[2 * stride + i] += acc[2].value();
+}
+
+template <int P, typename T, bool R2C, bool IL>
+__device__ __forceinline__ void
+interp_ad_particle(int t, int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
+            double Leni, int M, const T *__restrict__ phi, double prefac,
+            const double *__restrict__ selfb, double h, double *__restrict__ fk,
+            double *__restrict__ f, long stride) {
+    constexpr int NM = IL ? 2 : 1;
+    Acc<T> acc[3];
+    const double4 r = sorted[t];
+    const double xi = r.x, yi = r.y, zi = r.z, qi = r.w;
+    // dQ_d = W'.W.W q/L; force_d -= prefac * phi * dQ_d, interlaced x 0.5
+    // (charge-assign.c:1123-1125, 1374-1377, 758-760)
+    const T scale = (T)((IL ? 0.5 * prefac : prefac) * (qi * Leni));
+    for (int ii = 0; ii < NM; ii++) {
+        int b0, b1, b2;
+        T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
+        const double half = 0.5 * ii;
+        axis_weights_d<P, T>(xi, Hi, half, M, b0, w0, d0);
+        axis_weights_d<P, T>(yi, Hi, half, M, b1, w1, d1);
+        axis_weights_d<P, T>(zi, Hi, half, M, b2, w2, d2);
+        int kk[P];
+#pragma unroll
+        for (int c = 0; c < P; c++)
+            kk[c] = wrap1(b2 + c, M);
+        for (int a = 0; a < P; a++) {
+            const int ia = wrap1(b0 + a, M);
+#pragma unroll
+            for (int b = 0; b < P; b++) {
+                const int jb = wrap1(b1 + b, M);
+                const long row = ((long)ia * M + jb) * M;
+                const T wx = d0[a] * w1[b], wy = w0[a] * d1[b], wz = w0[a] * w1[b];
+#pragma unroll
+                for (int c = 0; c < P; c++) {
+                    const T B = scale * mesh_val<T, R2C>(phi, row + kk[c], ii);
+                    acc[0].add(-B * (wx * w2[c]));
+                    acc[1].add(-B * (wy * w2[c]));
+                    acc[2].add(-B * (wz * d2[c]));
+                }
+            }
+        }
+    }
+    const int i = perm[t];
+    fk[i] += acc[0].value();
+    fk[stride + i] += acc[1].value();
+    fk[2 * stride + i] += acc[2].value();
+    if (selfb != nullptr) {
+        // f -= sum_m' b(m') q^2 sin(2 pi m'.r / h)  (p3m-ad-self-forces.c:122-135)
+        const double q2 = qi * qi;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        int ind = 0;
+        for (int m0 = -1; m0 <= 1; m0++)
+            for (int m1 = -1; m1 <= 1; m1++)
+                for (int m2 = -1; m2 <= 1; m2++) {
+                    const double st =
+                        q2 * sin(2 * 3.14159265358979323846 * (m0 * xi / h + m1 * yi / h + m2 * zi / h));
+                    s0 += selfb[ind] * st;
+                    s1 += selfb[ind + 1] * st;
+                    s2 += selfb[ind + 2] * st;
+                    ind += 3;
+                }
+        f[i] -= s0;
+        f[stride + i] -= s1;
+        f[2 * stride + i] -= s2;
+    }
+}
+
+}  // namespace part
+}  // namespace p3m

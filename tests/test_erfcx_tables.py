@@ -50,3 +50,31 @@ def test_float_tables(x):
     K, D, c = load("Float")
     e = math.exp(-x * x) * float(erfcx_eval(x, K, D, c, np.float32))
     assert abs(e - math.erfc(x)) <= 1e-6 * math.erfc(x)
+
+
+def load_single():
+    text = open(HDR).read()
+    a = float(re.search(r"kErfcxSingleA = ([-+0-9.e]+)f", text).group(1))
+    b = float(re.search(r"kErfcxSingleB = ([-+0-9.e]+)f", text).group(1))
+    x1 = float(re.search(r"kErfcxSingleX1 = ([-+0-9.e]+)f", text).group(1))
+    body = text.split("kErfcxSingleCoef")[1].split("};")[0].split("{")[1]
+    c = [np.float32(float(v.strip().rstrip("f"))) for v in body.split(",")]
+    return a, b, x1, c
+
+
+def test_single_float_polynomial_matches_libm_erfc():
+    """The one-piece FP32 erfcx of the mixed-precision pair kernel, evaluated
+    in float32 as the kernel does (t = y a + b by FMA, Horner), against libm's
+    erfc up to the clamp point."""
+    a, b, x1, c = load_single()
+    worst = 0.0
+    for x in np.linspace(0.0, x1, 6001):
+        y = np.float32(1.0) / (np.float32(1.0) + np.float32(0.5) * np.float32(x))
+        t = np.float32(y * np.float32(a) + np.float32(b))
+        acc = np.float32(0.0)
+        for cc in reversed(c):
+            acc = np.float32(acc * t + cc)
+        xf = float(np.float32(x))
+        ref = math.erfc(xf) * math.exp(xf * xf)
+        worst = max(worst, abs(float(acc) - ref) / ref)
+    assert worst < 1e-6
