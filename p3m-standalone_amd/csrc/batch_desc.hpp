@@ -67,11 +67,22 @@ int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, 
 int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int ntot,
                  const double4 *sorted, const int *perm, double *fk, double *ftot, p3mk_queue *qu);
 
-// real space: cells per system, one persistent launch over all items
+// real space: cell-list arrays (octant-sorted cells, work items of 64
+// particles with bounding boxes) and one persistent launch over all items
+struct CellArrays {
+    int *key = nullptr;                               // fine key per particle
+    int *fcount = nullptr, *fstart = nullptr, *fcursor = nullptr; // 8 ncell + 1
+    int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
+    int *perm = nullptr, *scan = nullptr, *counter = nullptr;
+    double4 *sorted = nullptr;
+    float4 *bbox = nullptr; // 2 per work item
+    long max_items = 0;
+};
+int cells_alloc(CellArrays &a, long n, long ncell);
+void cells_release(CellArrays &a);
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
-               const double *q, int *cell_id, int *count, int *start, int *cursor, int *perm,
-               int *items, int *istart, int *scan_scratch, int *counter, double4 *sorted,
-               int grid, double *fr, double *energy, p3mk_queue *qu);
+               const double *q, CellArrays &a, int grid, double *fr, double *energy,
+               p3mk_queue *qu);
 
 }  // namespace p3m

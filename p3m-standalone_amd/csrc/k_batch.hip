@@ -59,9 +59,7 @@ struct p3mk_batch {
         *chunks = nullptr, *cstart = nullptr, *scan = nullptr;
     double4 *sorted = nullptr;
     // cells
-    int *cell_id = nullptr, *ccount = nullptr, *cstart_rs = nullptr, *ccursor = nullptr,
-        *cperm = nullptr, *items = nullptr, *istart = nullptr, *cscan = nullptr, *counter = nullptr;
-    double4 *csorted = nullptr;
+    CellArrays cells;
     // twiddles of each system (host copies kept for precision changes)
     std::vector<size_t> tw_off;
 };
@@ -69,13 +67,13 @@ struct p3mk_batch {
 namespace {
 
 void release_all(p3mk_batch *b) {
-    void *ptrs[] = {b->dsys,   b->psys,    b->p0s,     b->cell0s,  b->tile0s,   b->blk_z,
-                    b->blk_y,  b->blk_x,   b->mesh_arena, b->aux_arena, b->key,  b->count,
-                    b->start,  b->cursor,  b->perm,    b->chunks,  b->cstart,   b->scan,
-                    b->sorted, b->cell_id, b->ccount,  b->cstart_rs, b->ccursor, b->cperm,
-                    b->items,  b->istart,  b->cscan,   b->counter, b->csorted};
+    void *ptrs[] = {b->dsys,  b->psys,   b->p0s,        b->cell0s,    b->tile0s, b->blk_z,
+                    b->blk_y, b->blk_x,  b->mesh_arena, b->aux_arena, b->key,    b->count,
+                    b->start, b->cursor, b->perm,       b->chunks,    b->cstart, b->scan,
+                    b->sorted};
     for (void *p : ptrs)
         p3mk_release(p);
+    cells_release(b->cells);
 }
 
 // cells of side >= rcut / reach, as the single-system cell list
@@ -242,7 +240,7 @@ extern "C" int p3mk_batch_create(p3mk_batch **out, int r2c, int interlaced, int 
     }
     p0s[nsys] = (int)ntot;
     cell0s[nsys] = (int)ncell;
-    if (ntot > 0x7fffffffL / 2 || ncell > 0x7fffffffL / 2) {
+    if (ntot > 0x7fffffffL / 2 || ncell > 0x7fffffffL / 16) {
         delete b;
         return p3mk_fail(P3MK_EINVAL, "batch too large", 0);
     }
@@ -328,17 +326,7 @@ extern "C" int p3mk_batch_create(p3mk_batch **out, int r2c, int interlaced, int 
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->key, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->perm, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->sorted, sizeof(double4) * nn);
-    const size_t nc1 = (size_t)ncell + 1;
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->cell_id, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->ccount, sizeof(int) * nc1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->cstart_rs, sizeof(int) * nc1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->ccursor, sizeof(int) * nc1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->cperm, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->items, sizeof(int) * nc1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->istart, sizeof(int) * nc1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->cscan, sizeof(int) * p3mk_scan_scratch_ints(ncell));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->counter, sizeof(int) * 4);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&b->csorted, sizeof(double4) * nn);
+    if (rc == P3MK_OK) rc = cells_alloc(b->cells, ntot, ncell);
     if (rc == P3MK_OK) rc = build_descriptors(b);
     if (rc != P3MK_OK) {
         p3mk_batch_destroy(b);
@@ -449,6 +437,5 @@ extern "C" int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double
                                     const double *z, const double *q, double *fr, double *energy,
                                     p3mk_queue *qu) {
     return brs_forces(b->form, b->dsys, b->psys, b->cell0s, b->nsys, b->ncell, b->ntot, x, y, z, q,
-                      b->cell_id, b->ccount, b->cstart_rs, b->ccursor, b->cperm, b->items,
-                      b->istart, b->cscan, b->counter, b->csorted, b->rs_grid, fr, energy, qu);
+                      b->cells, b->rs_grid, fr, energy, qu);
 }

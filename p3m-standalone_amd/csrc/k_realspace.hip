@@ -1,30 +1,23 @@
 // Real-space (short-range) Ewald force on a linked-cell list.
 //
-// Pair set and arithmetic of Realteil (realpart.c:109-152): every ordered
+// Pair set and arithmetic of Realteil (realpart.c:124-167): every ordered
 // pair i != j whose minimum-image distance r satisfies r <= rcut contributes
 //   f_i += q_i q_j (erfc(alpha r)/r + 2 alpha/sqrt(pi) exp(-alpha^2 r^2)) / r^2 * d
 // and 0.5 q_i q_j erfc(alpha r)/r to the energy.
 //
-// Layout: particles are counting-sorted into cells of side >= rcut/R (R = 2
-// when the box holds >= 5 cells per side), folded into [0, L).  A work item
-// is (home cell, chunk of 64 home particles) with its (2R+1)^3 neighbour
-// cells; a persistent grid of single-wavefront workgroups claims
-// items from an atomic counter, so the dense cells of a clustered system are
-// spread over the whole GPU.  Inside an item the 64 lanes own 64 home
-// particles and walk the slab's neighbour cells in the same order:
-// neighbours arrive in chunks of 64 with one coalesced load and are
-// broadcast lane by lane (readlane), so the candidate loop touches no
-// memory.  Periodic images are handled by a per-neighbour-cell shift of 0 or
-// +-L, which equals the reference's round(dx/L)*L for every pair inside the
-// cutoff (|dx| <= rcut <= L/2), so dx = (x_i - x_j) - shift is the
-// reference's dx bit for bit.
-//
-// A cheap distance test accepts ~30% of the candidates; accepted neighbour
-// indices go to a per-lane queue in LDS, and the expensive erfc/exp part
-// runs when the queue is drained, so lanes stay busy instead of idling
-// through other lanes' accepted pairs (all lanes of an item see the whole
-// neighbourhood, so their acceptance rates and queue lengths are alike).
-// Forces of an item are added to fr with FP64 atomics.
+// Layout: particles are folded into [0, L) and counting-sorted into cells of
+// side >= rcut/R (R = 2 when the box holds >= 5 cells per side), and inside a
+// cell by octant, so that 64 consecutive particles of a cell are spatially
+// compact.  A work item is (home cell, chunk of 64 home particles); a
+// persistent grid of single-wavefront workgroups claims items from an
+// atomic counter, so the dense cells of a clustered system are spread over
+// the whole GPU.  Every item has a bounding box (computed after the sort),
+// and neighbour cells and neighbour chunks whose box is beyond rcut from the
+// item's box are skipped.  Periodic images are a per-neighbour-cell shift
+// of 0 or +-L, which equals the reference's round(dx/L)*L for every pair
+// inside the cutoff (rcut <= L/2), so dx = (x_i - x_j) - shift is the
+// reference's dx bit for bit.  (Newton's third law was tried and measured
+// slower here: the partner forces need global atomics per neighbour chunk.)
 
 #include <hip/hip_runtime.h>
 
@@ -38,8 +31,10 @@
 
 namespace {
 
+using p3m::CellArrays;
+
 constexpr int kWave = 64;
-constexpr int kQueue = 2 * kWave; // flat queue: at most one round pending plus one append
+constexpr int kQueue = 2 * kWave; // ring: at most one round pending plus one append
 constexpr int kMaxCellsPerSide = 512;
 
 __device__ __forceinline__ double fold(double r, double L) {
@@ -50,43 +45,94 @@ __device__ __forceinline__ double fold(double r, double L) {
     return (f >= L || f < 0.0) ? 0.0 : f;
 }
 
-__device__ __forceinline__ int cell_of(double rf, double inv_side, int nc) {
-    int c = (int)(rf * inv_side);
-    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+// cell * 8 + octant of the folded position inside its cell
+__device__ __forceinline__ int fine_key(double xf, double yf, double zf, double inv_side, int nc) {
+    const double ux = xf * inv_side, uy = yf * inv_side, uz = zf * inv_side;
+    int cx = (int)ux, cy = (int)uy, cz = (int)uz;
+    cx = cx < 0 ? 0 : (cx >= nc ? nc - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= nc ? nc - 1 : cy);
+    cz = cz < 0 ? 0 : (cz >= nc ? nc - 1 : cz);
+    const int oct = ((ux - cx >= 0.5) ? 4 : 0) | ((uy - cy >= 0.5) ? 2 : 0) | ((uz - cz >= 0.5) ? 1 : 0);
+    return ((cx * nc + cy) * nc + cz) * 8 + oct;
 }
 
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, double L, double inv_side, int nc,
-                             int *__restrict__ cell_id, int *__restrict__ count) {
+                             int *__restrict__ key, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int c = (cell_of(fold(x[i], L), inv_side, nc) * nc + cell_of(fold(y[i], L), inv_side, nc)) * nc +
-                  cell_of(fold(z[i], L), inv_side, nc);
-    cell_id[i] = c;
-    atomicAdd(&count[c], 1);
+    const int k = fine_key(fold(x[i], L), fold(y[i], L), fold(z[i], L), inv_side, nc);
+    key[i] = k;
+    atomicAdd(&count[k], 1);
 }
 
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
-                               const int *__restrict__ cell_id, const int *__restrict__ start,
+                               const int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int c = cell_id[i];
-    const int pos = start[c] + atomicAdd(&cursor[c], 1);
+    const int k = key[i];
+    const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
 }
 
-// work items of a cell: ceil(count / 64) chunks times nslab slabs
-__global__ void k_cell_items(const int *__restrict__ count, long ncell, int nslab,
-                             int *__restrict__ items) {
+// cell starts from the octant starts; work items per cell: ceil(count / 64)
+__global__ void k_cell_start_items(const int *__restrict__ fstart, long ncell, int *__restrict__ start,
+                                   int *__restrict__ items) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell)
+        return;
+    start[c] = fstart[8 * c];
     if (c < ncell)
-        items[c] = ((count[c] + 63) / 64) * nslab;
+        items[c] = (fstart[8 * (c + 1)] - fstart[8 * c] + 63) / 64;
+}
+
+// last index k with prefix[k] <= v
+__device__ __forceinline__ long prefix_find(const int *__restrict__ prefix, long count, int v) {
+    long lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const long mid = (lo + hi) / 2;
+        if (prefix[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float down(double v) {
+    float f = (float)v;
+    return ((double)f > v) ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float up(double v) {
+    float f = (float)v;
+    return ((double)f < v) ? nextafterf(f, INFINITY) : f;
+}
+
+// bounding box of every work item's particles (outward-rounded FP32):
+// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0)
+__global__ void k_item_bbox(const int *__restrict__ start, const int *__restrict__ istart, long ncell,
+                            long max_items, const double4 *__restrict__ sorted,
+                            float4 *__restrict__ bbox) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= max_items || t >= istart[ncell])
+        return;
+    const long c = prefix_find(istart, ncell, (int)t);
+    const int p0 = start[c] + (int)(t - istart[c]) * kWave;
+    const int p1 = min(start[c + 1], p0 + kWave);
+    double xl = 1e300, yl = 1e300, zl = 1e300, xh = -1e300, yh = -1e300, zh = -1e300;
+    for (int p = p0; p < p1; p++) {
+        const double4 r = sorted[p];
+        xl = fmin(xl, r.x); yl = fmin(yl, r.y); zl = fmin(zl, r.z);
+        xh = fmax(xh, r.x); yh = fmax(yh, r.y); zh = fmax(zh, r.z);
+    }
+    bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), 0.0f);
+    bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), 0.0f);
 }
 
 // value of lane k (uniform) in every lane
@@ -198,45 +244,66 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     }
 }
 
-// LDS of one wavefront's work item: a ring of queued pairs (neighbour's
-// sorted index; owner lane and periodic shift packed beside it), the home
-// particles, the owners' force accumulators and, for FP64, the erfcx table.
-struct RsShared {
-    uint2 q[kQueue];
+// LDS of one wavefront's work item: the ring of queued pairs, the home
+// particles, their force accumulators and, for FP64, the erfcx table.
+template <bool FP32> struct RsShared {
+    uint2 q[kQueue]; // neighbour index, owner | shift
     double4 home_pos[kWave];
     double facc[3][kWave];
-    double tab[kErfcxTab];
+    double tab[FP32 ? 1 : kErfcxTab];
 };
 
-__device__ __forceinline__ void load_erfcx_table(RsShared &sh) {
-    for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
-        sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
+template <bool FP32> __device__ __forceinline__ void load_erfcx_table(RsShared<FP32> &sh) {
+    if (!FP32) {
+        for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
+            sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
+    }
     __syncthreads();
 }
 
-// queue entry: y = owner (bits 0..5) | shift index + 1 per axis (2 bits each
-// at 6, 8, 10: 0, 1, 2 for -L, 0, +L)
+// queue entry: x = neighbour's sorted index; y = owner lane (bits 0..5) and
+// the periodic shift per axis as a signed 2-bit field (bits 6, 8, 10: -1, 0,
+// +1 in units of L)
 __device__ __forceinline__ unsigned int pack_owner_shift(int lane, int wx, int wy, int wz) {
-    return (unsigned)lane | ((unsigned)(wx + 1) << 6) | ((unsigned)(wy + 1) << 8) |
-           ((unsigned)(wz + 1) << 10);
+    return (unsigned)lane | ((unsigned)(wx & 3) << 6) | ((unsigned)(wy & 3) << 8) |
+           ((unsigned)(wz & 3) << 10);
+}
+__device__ __forceinline__ double shift_of(unsigned int e, int bit, double L) {
+    // sign-extend the 2-bit field: move it to bits 30..31, shift back arithmetically
+    const int v = (int)(e << (30 - bit)) >> 30;
+    return (double)v * L;
 }
 
-// One work item (home cell, chunk of 64 home particles).  Lanes own the 64
-// home particles and walk the (2 reach + 1)^3 neighbour cells together;
-// neighbours arrive in chunks of 64 with one coalesced load and are
-// broadcast lane by lane (readlane).  A cheap FP32 distance test in the home
-// cell's frame accepts ~30% of the candidates; the accepted pairs of all
-// lanes go to one ring queue per wavefront (rank by mbcnt over the ballot),
-// and every 64 queued pairs are evaluated by the 64 lanes at once, whoever
-// owns them, with the forces added to the owners' LDS accumulators.  Cells
-// of the item's system are cell0 + (cx nc + cy) nc + cz; forces go to
-// fr[d stride + perm[i]].
+// squared distance between two boxes (0 if they overlap)
+__device__ __forceinline__ float box_dist2(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    const float dx = fmaxf(0.0f, fmaxf(blo.x - ahi.x, alo.x - bhi.x));
+    const float dy = fmaxf(0.0f, fmaxf(blo.y - ahi.y, alo.y - bhi.y));
+    const float dz = fmaxf(0.0f, fmaxf(blo.z - ahi.z, alo.z - bhi.z));
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One work item (home cell, chunk of 64 home particles); every ordered pair
+// is evaluated for its home particle.  Lanes own the 64 home particles and
+// walk the neighbour cells together: (2 reach + 1)^3 cells around the home
+// cell, or, in a box of fewer cells per side (SMALL_BOX), every cell once
+// with the nearest image taken per pair.  Neighbour cells, and neighbour
+// chunks of 64 particles, whose bounding box lies beyond rcut from the
+// item's box are skipped.  The others arrive in chunks of 64 with one
+// coalesced load and are broadcast lane by lane (readlane); a cheap FP32
+// distance test in the home cell's frame accepts the candidates, and the
+// accepted pairs of all lanes go to one ring queue per wavefront (rank by
+// mbcnt over the ballot).  Every 64 queued pairs are evaluated by the 64
+// lanes at once, whoever owns them, with the exact FP64 difference and
+// inclusion test, and the forces are added to the owners' LDS accumulators.
+// Cells of the item's system are cell0 + (cx nc + cy) nc + cz with items
+// from istart[cell]; forces go to fr[d stride + perm[i]].
 template <bool FP32, bool SMALL_BOX>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-               const int *__restrict__ perm, const int *__restrict__ start, int nc, int reach,
+               const int *__restrict__ perm, const int *__restrict__ start,
+               const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
-               RsShared &sh, double &en_tot) {
+               RsShared<FP32> &sh, double &en_tot) {
     constexpr int kMask = kQueue - 1;
     const int lane = threadIdx.x;
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
@@ -244,14 +311,19 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     // prefilter thresholds; the exact test runs in pair()
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
     const float rc2f = (float)(rcut * rcut) * (1.0f + 1e-5f);
+    // boxes are compared in FP32: margin for their rounding
+    const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
+    const float rskip2 = rskip * rskip;
     const int lo = SMALL_BOX ? 0 : -reach, hi = SMALL_BOX ? nc - 1 : reach;
-    const int hb = start[cell0 + home], he = start[cell0 + home + 1];
+    const int hcell = cell0 + home;
+    const int hb = start[hcell], he = start[hcell + 1];
     const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
     const int i0 = hb + chunk * kWave; // sorted index of lane 0's home particle
     const int i = i0 + lane;
     const bool active = i < he;
-    const unsigned long long act = __ballot(active);
     const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int hitem = istart[hcell] + chunk;
+    const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
     // home-cell-relative single-precision coordinates for the prefilter
     const double ox = hx * side, oy = hy * side, oz = hz * side;
     const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
@@ -267,11 +339,9 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         if (lane < m) {
             const uint2 e = sh.q[(head + lane) & kMask];
             const int o = (int)(e.y & 63u);
-            const double sx = (double)((int)((e.y >> 6) & 3u) - 1) * L;
-            const double sy = (double)((int)((e.y >> 8) & 3u) - 1) * L;
-            const double sz = (double)((int)((e.y >> 10) & 3u) - 1) * L;
             PairAcc pa = {0.0, 0.0, 0.0, 0.0};
-            pair<FP32>(sh.home_pos[o], sorted[e.x], sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+            pair<FP32>(sh.home_pos[o], sorted[e.x], shift_of(e.y, 6, L), shift_of(e.y, 8, L),
+                       shift_of(e.y, 10, L), rcut, alpha, two_a_spi, sh.tab, pa);
             atomicAdd(&sh.facc[0][o], pa.f0);
             atomicAdd(&sh.facc[1][o], pa.f1);
             atomicAdd(&sh.facc[2][o], pa.f2);
@@ -282,10 +352,11 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         __syncthreads();
     };
 
-    // lanes in `mask` queue the pair (own home particle, neighbour j)
-    auto append = [&](unsigned long long mask, unsigned int j, unsigned int os) {
+    // the accepting lanes (ok) queue the pair (own home particle, j)
+    auto append = [&](bool ok, unsigned int j, unsigned int os) {
+        const unsigned long long mask = __ballot(ok);
         if (mask) {
-            if ((mask >> lane) & 1ull) {
+            if (ok) {
                 const unsigned int rank = __builtin_amdgcn_mbcnt_hi(
                     (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
                 sh.q[(head + count + (int)rank) & kMask] = make_uint2(j, os);
@@ -307,17 +378,37 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
             for (int c = lo; c <= hi; c++) {
                 int cz = SMALL_BOX ? c : hz + c, wz = 0;
                 if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
+                const double sx = wx * L, sy = wy * L, sz = wz * L;
+                if (!SMALL_BOX) {
+                    // the whole neighbour cell beyond the cutoff from the item's box?
+                    const float4 clo = make_float4((float)(cx * side + sx), (float)(cy * side + sy),
+                                                   (float)(cz * side + sz), 0.0f);
+                    const float4 chi = make_float4((float)((cx + 1) * side + sx),
+                                                   (float)((cy + 1) * side + sy),
+                                                   (float)((cz + 1) * side + sz), 0.0f);
+                    if (box_dist2(hlo, hhi, clo, chi) > rskip2)
+                        continue;
+                }
                 const int cell = cell0 + (cx * nc + cy) * nc + cz;
                 const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
                 const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-                const double sx = wx * L, sy = wy * L, sz = wz * L;
+                const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
                 const unsigned int os_cell = pack_owner_shift(lane, wx, wy, wz);
+                const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
                 // neighbours in chunks of 64: one coalesced load, then
                 // lane k's particle is broadcast to all lanes
                 for (int cb = jb; cb < je; cb += kWave) {
-                    const int jj = cb + lane;
-                    const double4 mine = (jj < je) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
                     const int cnt = min(kWave, je - cb);
+                    if (!SMALL_BOX) {
+                        const int nitem = item0 + (cb - jb) / kWave;
+                        float4 nlo = bbox[2 * nitem], nhi = bbox[2 * nitem + 1];
+                        nlo.x += fsx; nlo.y += fsy; nlo.z += fsz;
+                        nhi.x += fsx; nhi.y += fsy; nhi.z += fsz;
+                        if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
+                            continue;
+                    }
+                    const int jj = cb + lane;
+                    const double4 mine = (lane < cnt) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
                     if (SMALL_BOX) {
                         for (int k = 0; k < cnt; k++) {
                             const int j = cb + k;
@@ -329,26 +420,19 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                             dy -= ry * L;
                             dz -= rz * L;
                             const double r2 = dx * dx + dy * dy + dz * dz;
-                            const bool ok = active && j != i && r2 <= rc2;
-                            append(__ballot(ok), (unsigned)j,
+                            append(active && j != i && r2 <= rc2, (unsigned)j,
                                    pack_owner_shift(lane, (int)rx, (int)ry, (int)rz));
                         }
                     } else {
                         // neighbour in the home cell's frame, image shift included
                         const float mxf = (float)((mine.x + sx) - ox), myf = (float)((mine.y + sy) - oy),
                                     mzf = (float)((mine.z + sz) - oz);
-                        // the home particles' own slots (self pairs) among this chunk
-                        const int self0 = i0 - cb; // neighbour k is home lane k - self0
                         for (int k = 0; k < cnt; k++) {
                             const float dx = pxf - bcastf(mxf, k);
                             const float dy = pyf - bcastf(myf, k);
                             const float dz = pzf - bcastf(mzf, k);
                             const float r2 = dx * dx + dy * dy + dz * dz;
-                            unsigned long long mask = __ballot(r2 <= rc2f) & act;
-                            const int sl = k - self0;
-                            if (sl >= 0 && sl < kWave)
-                                mask &= ~(1ull << sl);
-                            append(mask, (unsigned)(cb + k), os_cell);
+                            append(active && r2 <= rc2f && cb + k != i, (unsigned)(cb + k), os_cell);
                         }
                     }
                 }
@@ -367,28 +451,16 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     __syncthreads();
 }
 
-// last index k with prefix[k] <= v
-__device__ __forceinline__ long prefix_find(const int *__restrict__ prefix, long count, int v) {
-    long lo = 0, hi = count;
-    while (hi - lo > 1) {
-        const long mid = (lo + hi) / 2;
-        if (prefix[mid] <= v)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Persistent grid: each single-wavefront workgroup claims work items from an
 // atomic counter until none are left.
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
 k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
-            const int *__restrict__ start, const int *__restrict__ istart, long ncell, int nc,
-            int reach, double L, double rcut, double alpha, int *__restrict__ counter,
-            double *__restrict__ fr, double *__restrict__ energy) {
-    __shared__ RsShared sh;
+            const int *__restrict__ start, const int *__restrict__ istart,
+            const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L, double rcut,
+            double alpha, int *__restrict__ counter, double *__restrict__ fr,
+            double *__restrict__ energy) {
+    __shared__ RsShared<FP32> sh;
     const int total = istart[ncell];
     double en_tot = 0.0;
     load_erfcx_table(sh);
@@ -400,8 +472,8 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, perm, start, nc,
-                                        reach, L, rcut, alpha, n, fr, sh, en_tot);
+        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, perm, start, istart,
+                                        bbox, nc, reach, L, rcut, alpha, n, fr, sh, en_tot);
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
@@ -413,29 +485,79 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
 
 }  // namespace
 
+// ---- cell-list arrays and launchers ---------------------------------------
+
+namespace p3m {
+
+void cells_release(CellArrays &a) {
+    void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
+                    a.scan, a.counter, a.sorted, a.bbox};
+    for (void *p : ptrs)
+        p3mk_release(p);
+    a = CellArrays{};
+}
+
+int cells_alloc(CellArrays &a, long n, long ncell) {
+    a = CellArrays{};
+    const size_t nn = (size_t)(n > 0 ? n : 1), nf = (size_t)(8 * ncell + 1), nc1 = (size_t)ncell + 1;
+    a.max_items = ncell + n / kWave + 1;
+    int rc = P3MK_OK;
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.key, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcount, sizeof(int) * nf);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fstart, sizeof(int) * nf);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcursor, sizeof(int) * nf);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.start, sizeof(int) * nc1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.items, sizeof(int) * nc1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.istart, sizeof(int) * nc1);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.perm, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.scan, sizeof(int) * p3mk_scan_scratch_ints(8 * ncell));
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.counter, sizeof(int) * 4);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
+    if (rc != P3MK_OK)
+        cells_release(a);
+    return rc;
+}
+
+// zero the octant counters before the keys are counted
+static int cells_begin(CellArrays &a, long ncell, p3mk_queue *qu) {
+    P3MK_TRY(p3mk_fill_zero(a.fcount, sizeof(int) * (8 * ncell + 1), qu));
+    return p3mk_fill_zero(a.fcursor, sizeof(int) * (8 * ncell + 1), qu);
+}
+
+// octant starts after the count (before the scatter)
+static int cells_scan(CellArrays &a, long ncell, p3mk_queue *qu) {
+    return p3mk_exclusive_scan(a.fcount, 8 * ncell, a.fstart, a.scan, qu);
+}
+
+// after the scatter: cell starts, work items, item boxes, counter reset
+static int cells_end(CellArrays &a, long ncell, p3mk_queue *qu) {
+    hipStream_t s = p3mk_stream(qu);
+    hipLaunchKernelGGL(k_cell_start_items, dim3((unsigned)((ncell + 1 + 255) / 256)), dim3(256), 0, s,
+                       a.fstart, ncell, a.start, a.items);
+    P3MK_TRY(p3mk_launch_check("cell starts"));
+    P3MK_TRY(p3mk_exclusive_scan(a.items, ncell, a.istart, a.scan, qu));
+    hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items + 255) / 256)), dim3(256), 0, s,
+                       a.start, a.istart, ncell, a.max_items, a.sorted, a.bbox);
+    P3MK_TRY(p3mk_launch_check("item boxes"));
+    return p3mk_fill_zero(a.counter, sizeof(int) * 4, qu);
+}
+
+}  // namespace p3m
+
 struct p3mk_cells {
     int nmax;
     int nc, reach;
     long ncell;
     double L, rcut, inv_side;
-    int *cell_id, *count, *start, *cursor, *perm, *scan_scratch, *items, *istart, *counter;
-    double4 *sorted;
+    CellArrays a;
     int grid;
 };
 
 extern "C" void p3mk_cells_destroy(p3mk_cells *c) {
     if (!c)
         return;
-    p3mk_release(c->cell_id);
-    p3mk_release(c->count);
-    p3mk_release(c->start);
-    p3mk_release(c->cursor);
-    p3mk_release(c->perm);
-    p3mk_release(c->scan_scratch);
-    p3mk_release(c->items);
-    p3mk_release(c->istart);
-    p3mk_release(c->counter);
-    p3mk_release(c->sorted);
+    p3m::cells_release(c->a);
     delete c;
 }
 
@@ -465,18 +587,7 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     c->L = L;
     c->rcut = rcut;
     c->inv_side = (double)nc / L;
-    int rc = P3MK_OK;
-    const size_t nn = (size_t)(n > 0 ? n : 1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cell_id, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->count, sizeof(int) * (c->ncell + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->start, sizeof(int) * (c->ncell + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->cursor, sizeof(int) * (c->ncell + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->perm, sizeof(int) * nn);
-    if (rc == P3MK_OK)
-        rc = p3mk_alloc((void **)&c->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(c->ncell));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->items, sizeof(int) * (c->ncell + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->istart, sizeof(int) * (c->ncell + 1));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->counter, sizeof(int) * 4);
+    int rc = p3m::cells_alloc(c->a, n, c->ncell);
     if (rc == P3MK_OK) {
         // persistent grid: enough single-wave workgroups to fill every CU
         int dev = 0, cus = 256;
@@ -484,7 +595,6 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         c->grid = (cus > 0 ? cus : 256) * 32;
     }
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&c->sorted, sizeof(double4) * nn);
     if (rc != P3MK_OK) {
         p3mk_cells_destroy(c);
         return rc;
@@ -500,29 +610,24 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
         return P3MK_OK;
     if (n > c->nmax || L != c->L || rcut != c->rcut)
         return p3mk_fail(P3MK_EINVAL, "cell list was built for another system size, box or cutoff", 0);
+    CellArrays &a = c->a;
     hipStream_t s = p3mk_stream(qu);
     const int nb = (n + 255) / 256;
-    P3MK_TRY(p3mk_fill_zero(c->count, sizeof(int) * (c->ncell + 1), qu));
-    P3MK_TRY(p3mk_fill_zero(c->cursor, sizeof(int) * (c->ncell + 1), qu));
+    P3MK_TRY(p3m::cells_begin(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, s, n, x, y, z, L, c->inv_side, c->nc,
-                       c->cell_id, c->count);
+                       a.key, a.fcount);
     P3MK_TRY(p3mk_launch_check("cell count"));
-    P3MK_TRY(p3mk_exclusive_scan(c->count, c->ncell, c->start, c->scan_scratch, qu));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, c->cell_id,
-                       c->start, c->cursor, c->perm, c->sorted);
+    P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, a.key, a.fstart,
+                       a.fcursor, a.perm, a.sorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
+    P3MK_TRY(p3m::cells_end(a, c->ncell, qu));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
-    const int nslab = 1; // one item walks the whole neighbourhood
-    hipLaunchKernelGGL(k_cell_items, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, s,
-                       c->count, c->ncell, nslab, c->items);
-    P3MK_TRY(p3mk_launch_check("cell items"));
-    P3MK_TRY(p3mk_exclusive_scan(c->items, c->ncell, c->istart, c->scan_scratch, qu));
-    P3MK_TRY(p3mk_fill_zero(c->counter, sizeof(int) * 4, qu));
     const dim3 grid((unsigned)c->grid);
 #define LAUNCH(F32, SB)                                                                            \
-    hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, c->sorted, c->perm,     \
-                       c->start, c->istart, c->ncell, c->nc, c->reach, L, rcut, alpha, c->counter, \
-                       fr, energy)
+    hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.perm,       \
+                       a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
+                       a.counter, fr, energy)
     if (fp32) {
         if (all_cells) LAUNCH(true, true); else LAUNCH(true, false);
     } else {
@@ -543,33 +648,30 @@ using p3m::BSys;
 
 __global__ void k_bcell_count(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
                               const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ z, int *__restrict__ cell_id,
+                              const double *__restrict__ z, int *__restrict__ key,
                               int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot)
         return;
     const BSys &S = sys[psys[i]];
-    const int nc = S.nc;
-    const double L = S.L, inv_side = (double)nc / L;
-    const int c = S.cell0 + (cell_of(fold(x[i], L), inv_side, nc) * nc +
-                             cell_of(fold(y[i], L), inv_side, nc)) * nc +
-                  cell_of(fold(z[i], L), inv_side, nc);
-    cell_id[i] = c;
-    atomicAdd(&count[c], 1);
+    const double L = S.L;
+    const int k = 8 * S.cell0 + fine_key(fold(x[i], L), fold(y[i], L), fold(z[i], L), (double)S.nc / L, S.nc);
+    key[i] = k;
+    atomicAdd(&count[k], 1);
 }
 
 __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
                                 const double *__restrict__ x, const double *__restrict__ y,
                                 const double *__restrict__ z, const double *__restrict__ q,
-                                const int *__restrict__ cell_id, const int *__restrict__ start,
+                                const int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
                                 double4 *__restrict__ sorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot)
         return;
     const double L = sys[psys[i]].L;
-    const int c = cell_id[i];
-    const int pos = start[c] + atomicAdd(&cursor[c], 1);
+    const int k = key[i];
+    const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
 }
@@ -578,9 +680,10 @@ template <bool FP32>
 __global__ void __launch_bounds__(kWave)
 k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
              const double4 *__restrict__ sorted, const int *__restrict__ perm,
-             const int *__restrict__ start, const int *__restrict__ istart, long ncell,
-             int *__restrict__ counter, double *__restrict__ fr, double *__restrict__ energy) {
-    __shared__ RsShared sh;
+             const int *__restrict__ start, const int *__restrict__ istart,
+             const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
+             double *__restrict__ fr, double *__restrict__ energy) {
+    __shared__ RsShared<FP32> sh;
     const int total = istart[ncell];
     load_erfcx_table(sh);
     for (;;) {
@@ -598,10 +701,12 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
         double en = 0.0;
         if (S.small)
             realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
-                                       start, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fr, sh, en);
+                                       start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot,
+                                       fr, sh, en);
         else
             realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
-                                        start, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fr, sh, en);
+                                        start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
+                                        ntot, fr, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
                 en += __shfl_down(en, off, kWave);
@@ -617,34 +722,29 @@ namespace p3m {
 
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
-               const double *q, int *cell_id, int *count, int *start, int *cursor, int *perm,
-               int *items, int *istart, int *scan_scratch, int *counter, double4 *sorted,
-               int grid, double *fr, double *energy, p3mk_queue *qu) {
+               const double *q, CellArrays &a, int grid, double *fr, double *energy,
+               p3mk_queue *qu) {
     if (ntot <= 0)
         return P3MK_OK;
     hipStream_t s = p3mk_stream(qu);
     const int nb = (ntot + 255) / 256;
-    P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * (ncell + 1), qu));
-    P3MK_TRY(p3mk_fill_zero(cursor, sizeof(int) * (ncell + 1), qu));
-    hipLaunchKernelGGL(k_bcell_count, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, cell_id,
-                       count);
+    P3MK_TRY(cells_begin(a, ncell, qu));
+    hipLaunchKernelGGL(k_bcell_count, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, a.key,
+                       a.fcount);
     P3MK_TRY(p3mk_launch_check("batched cell count"));
-    P3MK_TRY(p3mk_exclusive_scan(count, ncell, start, scan_scratch, qu));
-    hipLaunchKernelGGL(k_bcell_scatter, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, q,
-                       cell_id, start, cursor, perm, sorted);
+    P3MK_TRY(cells_scan(a, ncell, qu));
+    hipLaunchKernelGGL(k_bcell_scatter, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, q, a.key,
+                       a.fstart, a.fcursor, a.perm, a.sorted);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
-    hipLaunchKernelGGL(k_cell_items, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, count,
-                       ncell, 1, items);
-    P3MK_TRY(p3mk_launch_check("batched cell items"));
-    P3MK_TRY(p3mk_exclusive_scan(items, ncell, istart, scan_scratch, qu));
-    P3MK_TRY(p3mk_fill_zero(counter, sizeof(int) * 4, qu));
+    P3MK_TRY(cells_end(a, ncell, qu));
     if (f.fp32)
         hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, sorted, perm, start, istart, ncell, counter, fr, energy);
+                           nsys, ntot, a.sorted, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
+                           fr, energy);
     else
-        hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
-                           cell0s, nsys, ntot, sorted, perm, start, istart, ncell, counter, fr,
-                           energy);
+        hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
+                           nsys, ntot, a.sorted, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
+                           fr, energy);
     return p3mk_launch_check("batched real space");
 }
 
