@@ -75,7 +75,8 @@ struct CellArrays {
     int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
     int *perm = nullptr, *scan = nullptr, *counter = nullptr;
     double4 *sorted = nullptr;
-    float4 *bbox = nullptr; // 2 per work item
+    float4 *sorted32 = nullptr; // FP32 copy of the sorted positions (x, y, z, 0)
+    float4 *bbox = nullptr;     // 2 per work item
     long max_items = 0;
 };
 int cells_alloc(CellArrays &a, long n, long ncell);

@@ -71,14 +71,16 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
                                const double *__restrict__ z, const double *__restrict__ q, double L,
                                const int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
-                               double4 *__restrict__ sorted) {
+                               double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
-    sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+    const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+    sorted[pos] = r;
+    sorted32[pos] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.0f);
 }
 
 // cell starts from the octant starts; work items per cell: ceil(count / 64)
@@ -247,7 +249,8 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
 // LDS of one wavefront's work item: the ring of queued pairs, the home
 // particles, their force accumulators and, for FP64, the erfcx table.
 template <bool FP32> struct RsShared {
-    uint2 q[kQueue]; // neighbour index, owner | shift
+    uint2 q[kQueue]; // neighbour index, owner | shift index << 6
+    double4 shift[27]; // image shift (sx, sy, sz) of shift index (wx+1)*9 + (wy+1)*3 + wz+1
     double4 home_pos[kWave];
     double facc[3][kWave];
     double tab[FP32 ? 1 : kErfcxTab];
@@ -262,16 +265,17 @@ template <bool FP32> __device__ __forceinline__ void load_erfcx_table(RsShared<F
 }
 
 // queue entry: x = neighbour's sorted index; y = owner lane (bits 0..5) and
-// the periodic shift per axis as a signed 2-bit field (bits 6, 8, 10: -1, 0,
-// +1 in units of L)
+// the index of the periodic image shift (wx+1)*9 + (wy+1)*3 + (wz+1) above
 __device__ __forceinline__ unsigned int pack_owner_shift(int lane, int wx, int wy, int wz) {
-    return (unsigned)lane | ((unsigned)(wx & 3) << 6) | ((unsigned)(wy & 3) << 8) |
-           ((unsigned)(wz & 3) << 10);
+    return (unsigned)lane | ((unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1)) << 6);
 }
-__device__ __forceinline__ double shift_of(unsigned int e, int bit, double L) {
-    // sign-extend the 2-bit field: move it to bits 30..31, shift back arithmetically
-    const int v = (int)(e << (30 - bit)) >> 30;
-    return (double)v * L;
+
+// the 27 image shifts of a box of side L
+template <bool FP32> __device__ __forceinline__ void fill_shifts(RsShared<FP32> &sh, double L) {
+    if (threadIdx.x < 27) {
+        const int t = threadIdx.x;
+        sh.shift[t] = make_double4((t / 9 - 1) * L, ((t / 3) % 3 - 1) * L, (t % 3 - 1) * L, 0.0);
+    }
 }
 
 // squared distance between two boxes (0 if they overlap)
@@ -300,17 +304,19 @@ __device__ __forceinline__ float box_dist2(float4 alo, float4 ahi, float4 blo, f
 template <bool FP32, bool SMALL_BOX>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-               const int *__restrict__ perm, const int *__restrict__ start,
-               const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
+               const float4 *__restrict__ sorted32, const int *__restrict__ perm,
+               const int *__restrict__ start, const int *__restrict__ istart,
+               const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
                RsShared<FP32> &sh, double &en_tot) {
     constexpr int kMask = kQueue - 1;
     const int lane = threadIdx.x;
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
     const double Li = 1.0 / L, side = L / nc;
-    // prefilter thresholds; the exact test runs in pair()
+    // prefilter thresholds; the exact test runs in pair().  The FP32 test
+    // works on absolute FP32 coordinates (error ~ 6e-8 L per coordinate)
     const double rc2 = rcut * rcut * (1.0 + 1e-12);
-    const float rc2f = (float)(rcut * rcut) * (1.0f + 1e-5f);
+    const float rc2f = (float)(rcut * rcut * (1.0 + 1e-5 + 1e-6 * L / rcut));
     // boxes are compared in FP32: margin for their rounding
     const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
     const float rskip2 = rskip * rskip;
@@ -324,13 +330,15 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     const int hitem = istart[hcell] + chunk;
     const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
-    // home-cell-relative single-precision coordinates for the prefilter
-    const double ox = hx * side, oy = hy * side, oz = hz * side;
-    const float pxf = (float)(pi.x - ox), pyf = (float)(pi.y - oy), pzf = (float)(pi.z - oz);
+    // absolute single-precision coordinates for the prefilter; inactive
+    // lanes are placed far away so that they never pass it
+    const float hxf = active ? (float)pi.x : 1e30f, hyf = active ? (float)pi.y : 1e30f,
+                hzf = active ? (float)pi.z : 1e30f;
     sh.home_pos[lane] = pi;
     sh.facc[0][lane] = 0.0;
     sh.facc[1][lane] = 0.0;
     sh.facc[2][lane] = 0.0;
+    fill_shifts(sh, L);
     int head = 0, count = 0; // wave-uniform ring state
     __syncthreads();
 
@@ -339,9 +347,10 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         if (lane < m) {
             const uint2 e = sh.q[(head + lane) & kMask];
             const int o = (int)(e.y & 63u);
+            const double4 s3 = sh.shift[e.y >> 6];
             PairAcc pa = {0.0, 0.0, 0.0, 0.0};
-            pair<FP32>(sh.home_pos[o], sorted[e.x], shift_of(e.y, 6, L), shift_of(e.y, 8, L),
-                       shift_of(e.y, 10, L), rcut, alpha, two_a_spi, sh.tab, pa);
+            pair<FP32>(sh.home_pos[o], sorted[e.x], s3.x, s3.y, s3.z, rcut, alpha, two_a_spi,
+                       sh.tab, pa);
             atomicAdd(&sh.facc[0][o], pa.f0);
             atomicAdd(&sh.facc[1][o], pa.f1);
             atomicAdd(&sh.facc[2][o], pa.f2);
@@ -395,8 +404,9 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                 const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
                 const unsigned int os_cell = pack_owner_shift(lane, wx, wy, wz);
                 const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
-                // neighbours in chunks of 64: one coalesced load, then
-                // lane k's particle is broadcast to all lanes
+                // home particles relative to the neighbour cell's image
+                const float pxs = hxf - fsx, pys = hyf - fsy, pzs = hzf - fsz;
+                // neighbours in chunks of 64
                 for (int cb = jb; cb < je; cb += kWave) {
                     const int cnt = min(kWave, je - cb);
                     if (!SMALL_BOX) {
@@ -407,9 +417,10 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                         if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
                             continue;
                     }
-                    const int jj = cb + lane;
-                    const double4 mine = (lane < cnt) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
                     if (SMALL_BOX) {
+                        // one coalesced load, then lane k's particle is broadcast
+                        const int jj = cb + lane;
+                        const double4 mine = (lane < cnt) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
                         for (int k = 0; k < cnt; k++) {
                             const int j = cb + k;
                             double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
@@ -423,17 +434,33 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                             append(active && j != i && r2 <= rc2, (unsigned)j,
                                    pack_owner_shift(lane, (int)rx, (int)ry, (int)rz));
                         }
-                    } else {
-                        // neighbour in the home cell's frame, image shift included
-                        const float mxf = (float)((mine.x + sx) - ox), myf = (float)((mine.y + sy) - oy),
-                                    mzf = (float)((mine.z + sz) - oz);
+                    } else if (cb == i0) {
+                        // the item's own chunk: exclude the self pair
                         for (int k = 0; k < cnt; k++) {
-                            const float dx = pxf - bcastf(mxf, k);
-                            const float dy = pyf - bcastf(myf, k);
-                            const float dz = pzf - bcastf(mzf, k);
+                            const float4 v = sorted32[cb + k];
+                            const float dx = pxs - v.x, dy = pys - v.y, dz = pzs - v.z;
                             const float r2 = dx * dx + dy * dy + dz * dz;
-                            append(active && r2 <= rc2f && cb + k != i, (unsigned)(cb + k), os_cell);
+                            append(r2 <= rc2f && k != lane, (unsigned)(cb + k), os_cell);
                         }
+                    } else {
+                        // neighbours read with scalar (wave-uniform) loads, four
+                        // at a time so that their latencies overlap
+                        auto test = [&](const float4 &v, int k) {
+                            const float dx = pxs - v.x, dy = pys - v.y, dz = pzs - v.z;
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            append(r2 <= rc2f, (unsigned)(cb + k), os_cell);
+                        };
+                        int k = 0;
+                        for (; k + 4 <= cnt; k += 4) {
+                            const float4 v0 = sorted32[cb + k], v1 = sorted32[cb + k + 1],
+                                         v2 = sorted32[cb + k + 2], v3 = sorted32[cb + k + 3];
+                            test(v0, k);
+                            test(v1, k + 1);
+                            test(v2, k + 2);
+                            test(v3, k + 3);
+                        }
+                        for (; k < cnt; k++)
+                            test(sorted32[cb + k], k);
                     }
                 }
             }
@@ -455,7 +482,8 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
 // atomic counter until none are left.
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
-k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm,
+k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
+            const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart,
             const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L, double rcut,
             double alpha, int *__restrict__ counter, double *__restrict__ fr,
@@ -472,8 +500,8 @@ k_realspace(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, perm, start, istart,
-                                        bbox, nc, reach, L, rcut, alpha, n, fr, sh, en_tot);
+        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, perm, start,
+                                        istart, bbox, nc, reach, L, rcut, alpha, n, fr, sh, en_tot);
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
@@ -491,7 +519,7 @@ namespace p3m {
 
 void cells_release(CellArrays &a) {
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.bbox};
+                    a.scan, a.counter, a.sorted, a.sorted32, a.bbox};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -513,6 +541,7 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.scan, sizeof(int) * p3mk_scan_scratch_ints(8 * ncell));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.counter, sizeof(int) * 4);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
     if (rc != P3MK_OK)
         cells_release(a);
@@ -619,13 +648,14 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, a.key, a.fstart,
-                       a.fcursor, a.perm, a.sorted);
+                       a.fcursor, a.perm, a.sorted, a.sorted32);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     P3MK_TRY(p3m::cells_end(a, c->ncell, qu));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
     const dim3 grid((unsigned)c->grid);
 #define LAUNCH(F32, SB)                                                                            \
-    hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.perm,       \
+    hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,   \
+                       a.perm,                                                                     \
                        a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
                        a.counter, fr, energy)
     if (fp32) {
@@ -665,7 +695,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
                                 const double *__restrict__ z, const double *__restrict__ q,
                                 const int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
-                                double4 *__restrict__ sorted) {
+                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot)
         return;
@@ -673,13 +703,16 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
-    sorted[pos] = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+    const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
+    sorted[pos] = r;
+    sorted32[pos] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.0f);
 }
 
 template <bool FP32>
 __global__ void __launch_bounds__(kWave)
 k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
-             const double4 *__restrict__ sorted, const int *__restrict__ perm,
+             const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
+             const int *__restrict__ perm,
              const int *__restrict__ start, const int *__restrict__ istart,
              const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
              double *__restrict__ fr, double *__restrict__ energy) {
@@ -700,12 +733,12 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
             continue; // no real-space part for this system
         double en = 0.0;
         if (S.small)
-            realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
-                                       start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot,
-                                       fr, sh, en);
+            realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
+                                       perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
+                                       ntot, fr, sh, en);
         else
-            realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, perm,
-                                        start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
+            realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
+                                        perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
                                         ntot, fr, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
@@ -734,16 +767,16 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
     P3MK_TRY(p3mk_launch_check("batched cell count"));
     P3MK_TRY(cells_scan(a, ncell, qu));
     hipLaunchKernelGGL(k_bcell_scatter, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, q, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted);
+                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
     if (f.fp32)
         hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
+                           nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
                            fr, energy);
     else
         hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
+                           nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
                            fr, energy);
     return p3mk_launch_check("batched real space");
 }
