@@ -216,20 +216,45 @@ def roofline(w, res, s, pairs):
     return dom, per
 
 
-def roofline_object(dom, per):
+# Kernel symbol of each stage's dominant launch in the mixed form, as the
+# profiler names it.
+STAGE_KERNEL = {"realspace": "k_realspace<true, false>", "interp": "k_interp_ad<7, float, false, true>",
+                "assign": "k_assign_halo<7, float, false, true, 0>", "fft_multiply": "k_fft_pass<float>"}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    same counters over a plain device copy; the table's last column).  The
+    counters cannot be read from inside the process, so the figure is that of
+    the profiled build; None when no summary covers the kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == "`%s`" % kernel:
+            return float(cells[3])
+    return None
+
+
+def roofline_object(dom, per, workload):
     """The dominant stage of the mixed form against its bound.  Real space
     runs its pair arithmetic on the vector ALUs in FP32 (no MFMA: nothing on
     the path is a dense contraction), so its bound is compute, priced at the
     FP32 dense peak; the mesh stages are priced against HBM."""
     st = per[dom]
+    kernel = STAGE_KERNEL.get(dom, dom)
+    traffic = measured_traffic(workload, kernel)
     if dom == "realspace":
-        return {"bound": "mfma", "kernel": "k_realspace (cell lists + pair kernel, FP32 pairs)",
+        return {"bound": "mfma", "kernel": kernel + " (cell lists + pair kernel, FP32 pairs)",
                 "achieved": st["TFLOP/s"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": st["frac_valu"], "traffic": None,
+                "frac": st["frac_valu"], "traffic": traffic,
                 "note": "vector-ALU compute bound; achieved = %d ordered pairs within rcut x %d "
-                        "flops / median stage time" % (st["pairs"], PAIR_FLOPS)}
-    return {"bound": "hbm", "kernel": dom, "achieved": st["GB/s"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": st["frac_hbm"], "traffic": None}
+                        "flops / median stage time; traffic = HBM bytes per launch from "
+                        "profiles/traffic_%s.md" % (st["pairs"], PAIR_FLOPS, workload)}
+    return {"bound": "hbm", "kernel": kernel, "achieved": st["GB/s"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": st["frac_hbm"], "traffic": traffic}
 
 
 def cpu_baseline(w, sysd):
@@ -435,7 +460,7 @@ def main():
         "config": {"workload": args.workload, "method": w["method"], "N": N, "box": w["L"],
                    "mesh": w["M"], "cao": w["cao"], "rcut": w["rcut"], "alpha": w["alpha"],
                    "parallelism": "single GPU"},
-        "roofline": roofline_object(dom, per),
+        "roofline": roofline_object(dom, per, args.workload),
         "stages_mixed": per,
         "fp64": {"value": round(value64, 1), "ms_per_step": round(r64["ms_per_step"], 4),
                  "stages": per64},
