@@ -69,13 +69,20 @@ int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int 
 
 // real space: cell-list arrays (octant-sorted cells, work items of 64
 // particles with bounding boxes) and one persistent launch over all items
+// Most particles a cell list holds: the real-space kernels address the
+// sorted FP64 positions (and 64 padding entries) with 32-bit byte offsets.
+constexpr long kCellMaxParticles = (1L << 27) - 64;
+
 struct CellArrays {
     int *key = nullptr;                               // fine key per particle
     int *fcount = nullptr, *fstart = nullptr, *fcursor = nullptr; // 8 ncell + 1
     int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
     int *perm = nullptr, *scan = nullptr, *counter = nullptr;
     double4 *sorted = nullptr;
-    float4 *sorted32 = nullptr; // FP32 copy of the sorted positions (x, y, z, 0)
+    // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, 1) and |x|^2;
+    // the three sorted arrays carry 64 padding entries past the last particle
+    float4 *sorted32 = nullptr;
+    float *norm2 = nullptr;
     float4 *bbox = nullptr;     // 2 per work item
     long max_items = 0;
 };

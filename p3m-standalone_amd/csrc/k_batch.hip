@@ -240,7 +240,7 @@ extern "C" int p3mk_batch_create(p3mk_batch **out, int r2c, int interlaced, int 
     }
     p0s[nsys] = (int)ntot;
     cell0s[nsys] = (int)ncell;
-    if (ntot > 0x7fffffffL / 2 || ncell > 0x7fffffffL / 16) {
+    if (ntot > kCellMaxParticles || ncell > 0x7fffffffL / 16) {
         delete b;
         return p3mk_fail(P3MK_EINVAL, "batch too large", 0);
     }

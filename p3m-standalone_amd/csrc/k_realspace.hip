@@ -13,16 +13,31 @@
 // atomic counter, so the dense cells of a clustered system are spread over
 // the whole GPU.  Every item has a bounding box (computed after the sort),
 // and neighbour cells and neighbour chunks whose box is beyond rcut from the
-// item's box are skipped.  Periodic images are a per-neighbour-cell shift
-// of 0 or +-L, which equals the reference's round(dx/L)*L for every pair
-// inside the cutoff (rcut <= L/2), so dx = (x_i - x_j) - shift is the
-// reference's dx bit for bit.  (Newton's third law was tried and measured
-// slower here: the partner forces need global atomics per neighbour chunk.)
+// item's box are skipped.
+//
+// Candidate search on the matrix cores: for a home chunk and a neighbour
+// chunk, r^2 - rcut^2 of all 64 x 64 pairs is one K = 4 product per 16 x 16
+// tile,
+//   [-2 n, 1] . [h', |h'|^2 - rcut^2] + |n|^2 = |h' - n|^2 - rcut^2,
+// with n the neighbour relative to its cell centre and h' the home particle
+// relative to the same centre (FP32, |error| < 5e-6 rcut^2 for pairs near
+// the cutoff; the test keeps a 2e-5 margin).  Accepted pairs of a tile go to
+// one queue per wavefront (rank by mbcnt over the ballot); every 64 queued
+// pairs are evaluated by the 64 lanes at once, whoever owns them, with the
+// exact FP64 difference and the reference's inclusion test, and the forces
+// are added to the owners' LDS accumulators.
+//
+// Periodic images are a per-neighbour-cell shift of 0 or +-L, which equals
+// the reference's round(dx/L)*L for every pair inside the cutoff
+// (rcut <= L/2), so dx = (x_i - x_j) - shift is the reference's dx bit for
+// bit.  (Newton's third law was tried and measured slower here: the
+// partner forces need global atomics per neighbour chunk.)
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <new>
+#include <type_traits>
 
 #include "batch_desc.hpp"
 #include "device_common.hpp"
@@ -34,8 +49,14 @@ namespace {
 using p3m::CellArrays;
 
 constexpr int kWave = 64;
-constexpr int kQueue = 2 * kWave; // ring: at most one round pending plus one append
+// at most 63 pending pairs plus the appends of one 16 x 16 tile (4 x 64)
+constexpr int kQueue = 5 * kWave;
 constexpr int kMaxCellsPerSide = 512;
+// sorted arrays carry kPad entries past the last particle: a 16-row tile of
+// a chunk's last rows may read up to 15 rows beyond it
+constexpr int kPad = 64;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double fold(double r, double L) {
     // into [0, L); identity for positions already inside the box
@@ -56,6 +77,29 @@ __device__ __forceinline__ int fine_key(double xf, double yf, double zf, double 
     return ((cx * nc + cy) * nc + cz) * 8 + oct;
 }
 
+// Sorted copies of one particle: FP64 (x, y, z, q), and relative to the
+// centre of its cell (cx, cy, cz) in FP32 the matrix row (-2 x, -2 y, -2 z, 1)
+// and |x|^2 of the candidate search.
+__device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int nc, double side,
+                                             double4 *__restrict__ sorted,
+                                             float4 *__restrict__ sorted32,
+                                             float *__restrict__ norm2) {
+    const int cz = cell % nc, cy = (cell / nc) % nc, cx = cell / (nc * nc);
+    const float lx = (float)(r.x - (cx + 0.5) * side), ly = (float)(r.y - (cy + 0.5) * side),
+                lz = (float)(r.z - (cz + 0.5) * side);
+    sorted[pos] = r;
+    sorted32[pos] = make_float4(-2.0f * lx, -2.0f * ly, -2.0f * lz, 1.0f);
+    norm2[pos] = lx * lx + ly * ly + lz * lz;
+}
+
+// the kPad entries past the last particle: finite rows that never pass
+__device__ __forceinline__ void store_pad(int pos, double4 *__restrict__ sorted,
+                                          float4 *__restrict__ sorted32, float *__restrict__ norm2) {
+    sorted[pos] = make_double4(0.0, 0.0, 0.0, 0.0);
+    sorted32[pos] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    norm2[pos] = 1e30f;
+}
+
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, double L, double inv_side, int nc,
                              int *__restrict__ key, int *__restrict__ count) {
@@ -67,20 +111,25 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
     atomicAdd(&count[k], 1);
 }
 
+// n + kPad threads
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
-                               const int *__restrict__ key, const int *__restrict__ start,
+                               int nc, const int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
-                               double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
+                               double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
+                               float *__restrict__ norm2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    if (i >= n + kPad)
         return;
+    if (i >= n) {
+        store_pad(i, sorted, sorted32, norm2);
+        return;
+    }
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    sorted[pos] = r;
-    sorted32[pos] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.0f);
+    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32, norm2);
 }
 
 // cell starts from the octant starts; work items per cell: ceil(count / 64)
@@ -145,21 +194,13 @@ __device__ __forceinline__ double bcast(double v, int k) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ float bcastf(float v, int k) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-    for (int off = 32; off > 0; off >>= 1)
-        v = max(v, __shfl_xor(v, off, kWave));
-    return v;
-}
-
 // 2 alpha / sqrt(pi) with the reference's sqrt(pi) literal (realpart.c:120)
 __device__ __forceinline__ double sqrt_pi() { return 1.77245385090551602729816748334; }
 
 struct PairAcc {
-    double f0, f1, f2, en;
+    double f0, f1, f2;
+    double en;  // FP64 form
+    float enf;  // FP32 form
 };
 
 constexpr int kErfcxTab = p3m::kErfcxDoublePieces * (p3m::kErfcxDoubleDegree + 1);
@@ -193,44 +234,66 @@ __device__ __forceinline__ float erfcx_f(float x) {
     return acc;
 }
 
-// Pair evaluation with the reference's inclusion test r <= rcut
-// (realpart.c:134-148); dx is the reference's minimum-image difference
-// bit for bit.  erfc(x) = exp(-x^2) erfcx(x) shares the one exponential with
-// the exp(-x^2) term of the force; in FP64 x^2 is split exactly (hi + lo by
-// FMA) so exp(-x^2) keeps its accuracy for large x.
+// The reference's inclusion test r <= rcut (realpart.c:134-148) on its dx,
+// decided exactly: r2 against rcut^2, with the correctly rounded sqrt only
+// within 1e-12 of the edge.
+__device__ __forceinline__ bool within_cutoff(double dx, double dy, double dz, double rcut) {
+    double r2;
+    {
+#pragma clang fp contract(off)
+        r2 = dx * dx + dy * dy + dz * dz;
+    }
+    const double rc2 = rcut * rcut;
+    bool inside = r2 < rc2 * (1.0 - 1e-12);
+    if (__builtin_expect(!inside && r2 <= rc2 * (1.0 + 1e-12), 0))
+        inside = sqrt(r2) <= rcut;
+    return inside;
+}
+
+// Pair evaluation; dx is the reference's minimum-image difference bit for
+// bit and the pair set is the reference's.  erfc(x) = exp(-x^2) erfcx(x)
+// shares the one exponential with the exp(-x^2) term of the force.
+//  - FP64: x^2 is split exactly (hi + lo by FMA) so exp(-x^2) keeps its
+//    accuracy for large x.
+//  - FP32: d = dx rounded to FP32 gives r2f within 5 2^-24 of r2, so the
+//    inclusion test is decided on r2f except within 1e-6 of rcut^2, where
+//    the FP64 test runs.
 template <bool FP32>
 __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, double sx, double sy,
                                      double sz, double rcut, double alpha, double two_a_spi,
                                      const double *tab, PairAcc &acc) {
-    double dx, dy, dz, r2;
+    double dx, dy, dz;
     {
 #pragma clang fp contract(off)
         dx = (pi.x - pj.x) - sx;
         dy = (pi.y - pj.y) - sy;
         dz = (pi.z - pj.z) - sz;
-        r2 = dx * dx + dy * dy + dz * dz;
     }
-    // r <= rcut decided exactly (correctly rounded sqrt) only near the edge
-    const double rc2 = rcut * rcut;
-    bool inside = r2 < rc2 * (1.0 - 1e-12);
-    const bool edge = !inside && r2 <= rc2 * (1.0 + 1e-12);
-    if (__builtin_expect(edge, 0))
-        inside = sqrt(r2) <= rcut;
-    if (!inside)
-        return;
     const double qq = pi.w * pj.w;
     if (FP32) {
-        const float r2f = (float)r2;
+        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+        const float r2f = fx * fx + fy * fy + fz * fz;
+        const double rc2 = rcut * rcut;
+        bool inside = r2f <= (float)(rc2 * (1.0 - 1e-6));
+        if (__builtin_expect(!inside && r2f <= (float)(rc2 * (1.0 + 1e-6)), 0))
+            inside = within_cutoff(dx, dy, dz, rcut);
         const float rinv = __builtin_amdgcn_rsqf(r2f);
         const float x = (float)alpha * (r2f * rinv);
         const float e = __expf(-x * x);
         const float ec = e * erfcx_f(x);
-        const float fak = (float)qq * (ec * rinv + (float)two_a_spi * e) * (rinv * rinv);
-        acc.f0 = (double)(fak * (float)dx);
-        acc.f1 = (double)(fak * (float)dy);
-        acc.f2 = (double)(fak * (float)dz);
-        acc.en = 0.5 * qq * (double)(ec * rinv);
+        const float fak = inside ? (float)qq * (ec * rinv + (float)two_a_spi * e) * (rinv * rinv) : 0.0f;
+        acc.f0 = (double)(fak * fx);
+        acc.f1 = (double)(fak * fy);
+        acc.f2 = (double)(fak * fz);
+        acc.enf = inside ? (float)qq * (ec * rinv) : 0.0f; // twice the energy
     } else {
+        if (!within_cutoff(dx, dy, dz, rcut))
+            return;
+        double r2;
+        {
+#pragma clang fp contract(off)
+            r2 = dx * dx + dy * dy + dz * dz;
+        }
         const double rinv = rsqrt(r2);
         const double x = alpha * (r2 * rinv);
         const double x2 = x * x;
@@ -246,10 +309,11 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     }
 }
 
-// LDS of one wavefront's work item: the ring of queued pairs, the home
+// LDS of one wavefront's work item: the queue of accepted pairs, the home
 // particles, their force accumulators and, for FP64, the erfcx table.
 template <bool FP32> struct RsShared {
-    uint2 q[kQueue]; // neighbour index, owner | shift index << 6
+    unsigned int qj[kQueue]; // neighbour's sorted index
+    unsigned int qo[kQueue]; // pack_pair(owner lane, shift index)
     double4 shift[27]; // image shift (sx, sy, sz) of shift index (wx+1)*9 + (wy+1)*3 + wz+1
     double4 home_pos[kWave];
     double facc[3][kWave];
@@ -264,10 +328,15 @@ template <bool FP32> __device__ __forceinline__ void load_erfcx_table(RsShared<F
     __syncthreads();
 }
 
-// queue entry: x = neighbour's sorted index; y = owner lane (bits 0..5) and
-// the index of the periodic image shift (wx+1)*9 + (wy+1)*3 + (wz+1) above
-__device__ __forceinline__ unsigned int pack_owner_shift(int lane, int wx, int wy, int wz) {
-    return (unsigned)lane | ((unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1)) << 6);
+// index of the periodic image shift (wx, wy, wz) in {-1, 0, 1}^3
+__device__ __forceinline__ unsigned int shift_index(int wx, int wy, int wz) {
+    return (unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1));
+}
+
+// queue entry of a pair: the LDS byte offsets of the owner's home_pos entry
+// (bits 0..15) and of the image shift (bits 16..31), both double4
+__device__ __forceinline__ unsigned int pack_pair(int owner, unsigned int shift) {
+    return (unsigned)owner * 32u | (shift * 32u) << 16;
 }
 
 // the 27 image shifts of a box of side L
@@ -286,181 +355,309 @@ __device__ __forceinline__ float box_dist2(float4 alo, float4 ahi, float4 blo, f
     return dx * dx + dy * dy + dz * dz;
 }
 
-// One work item (home cell, chunk of 64 home particles); every ordered pair
-// is evaluated for its home particle.  Lanes own the 64 home particles and
-// walk the neighbour cells together: (2 reach + 1)^3 cells around the home
-// cell, or, in a box of fewer cells per side (SMALL_BOX), every cell once
-// with the nearest image taken per pair.  Neighbour cells, and neighbour
-// chunks of 64 particles, whose bounding box lies beyond rcut from the
-// item's box are skipped.  The others arrive in chunks of 64 with one
-// coalesced load and are broadcast lane by lane (readlane); a cheap FP32
-// distance test in the home cell's frame accepts the candidates, and the
-// accepted pairs of all lanes go to one ring queue per wavefront (rank by
-// mbcnt over the ballot).  Every 64 queued pairs are evaluated by the 64
-// lanes at once, whoever owns them, with the exact FP64 difference and
-// inclusion test, and the forces are added to the owners' LDS accumulators.
-// Cells of the item's system are cell0 + (cx nc + cy) nc + cz with items
-// from istart[cell]; forces go to fr[d stride + perm[i]].
+// distance along one axis between [alo, ahi] and [blo, bhi]
+__device__ __forceinline__ float gap(float alo, float ahi, float blo, float bhi) {
+    return fmaxf(0.0f, fmaxf(blo - ahi, alo - bhi));
+}
+
+// One neighbour chunk [cb, cb + cnt) against the item's home particles on
+// the matrix cores (v_mfma_f32_16x16x4f32: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; result register r of lane l is
+// D[4 (l / 16) + r][l % 16]).  Rows are neighbours, columns home particles:
+// A = (-2 n, 1) from sorted32, B = bop (h', |h'|^2 - rc2f) per row tile of
+// home particles, C = |n|^2.  D <= 0 queues the pair (neighbour, owner);
+// the queue is drained after every tile.  SELF: the chunk is the item's
+// own, whose diagonal is excluded.
+template <bool SELF, typename Append, typename Flush>
+__device__ __forceinline__ void scan_chunk(Append &append, Flush &flush,
+                                           const float4 *__restrict__ sorted32,
+                                           const float *__restrict__ norm2, int cb, int cnt,
+                                           int ntr, int i0, int col, int kc, floatx4 bop,
+                                           unsigned int shp) {
+    const int je = cb + cnt;
+    const int ntc = (cnt + 15) >> 4;
+    for (int tc = 0; tc < ntc; tc++) {
+        const int row0 = cb + 16 * tc;
+        const float av = reinterpret_cast<const float *>(sorted32)[4 * (row0 + col) + kc];
+        const int rb = row0 + 4 * kc; // neighbour rows of this lane's results
+        floatx4 cv;
+        cv[0] = norm2[rb];
+        cv[1] = norm2[rb + 1];
+        cv[2] = norm2[rb + 2];
+        cv[3] = norm2[rb + 3];
+        if (row0 + 16 > je) {
+            // rows past the chunk's end never pass
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (rb + r >= je)
+                    cv[r] = 1e30f;
+        }
+        for (int tr = 0; tr < ntr; tr++) {
+            const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bop[tr], cv, 0, 0, 0);
+            const unsigned int os = pack_pair(16 * tr + col, 0) | shp;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                bool ok = d[r] <= 0.0f;
+                if (SELF)
+                    ok = ok && (rb + r != i0 + 16 * tr + col);
+                append(ok, (unsigned)(rb + r), os);
+            }
+            flush();
+        }
+    }
+}
+
+// One neighbour chunk [cb, cb + cnt) against the item's home particles on
+// the vector ALUs (the FP64 form): neighbours are read with scalar
+// (wave-uniform) loads, four at a time so that their latencies overlap, and
+// each lane tests its own home particle hl (sorted32 row of the home
+// particle) against them in the frame of the home cell centre; (ox, oy, oz)
+// is the neighbour cell centre relative to it.  os: owner lane | shift.
+template <typename Append, typename Flush>
+__device__ __forceinline__ void scan_chunk_scalar(Append &append, Flush &flush,
+                                                  const float4 *__restrict__ sorted32, int cb,
+                                                  int cnt, int i0, int lane, float4 hl, double ox,
+                                                  double oy, double oz, float rc2f,
+                                                  unsigned int os) {
+    // h - o; the neighbour is n = -v / 2 of its row v, so h - o - n = p + v / 2
+    const float px = -0.5f * hl.x - (float)ox, py = -0.5f * hl.y - (float)oy,
+                pz = -0.5f * hl.z - (float)oz;
+    auto test = [&](const float4 &v, int k) {
+        const float dx = fmaf(0.5f, v.x, px), dy = fmaf(0.5f, v.y, py), dz = fmaf(0.5f, v.z, pz);
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        append(r2 <= rc2f, (unsigned)(cb + k), os);
+        flush();
+    };
+    if (cb == i0) {
+        // the item's own chunk: exclude the self pair
+        for (int k = 0; k < cnt; k++) {
+            const float4 v = sorted32[cb + k];
+            const float dx = fmaf(0.5f, v.x, px), dy = fmaf(0.5f, v.y, py), dz = fmaf(0.5f, v.z, pz);
+            append(dx * dx + dy * dy + dz * dz <= rc2f && k != lane, (unsigned)(cb + k), os);
+            flush();
+        }
+        return;
+    }
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const float4 v0 = sorted32[cb + k], v1 = sorted32[cb + k + 1], v2 = sorted32[cb + k + 2],
+                     v3 = sorted32[cb + k + 3];
+        test(v0, k);
+        test(v1, k + 1);
+        test(v2, k + 2);
+        test(v3, k + 3);
+    }
+    for (; k < cnt; k++)
+        test(sorted32[cb + k], k);
+}
+
+// One work item (home cell, chunk of up to 64 home particles); every
+// ordered pair is evaluated for its home particle, by lane = owner.
+// Neighbour cells are the (2 reach + 1)^3 cells around the home cell, or,
+// in a box of fewer cells per side (SMALL_BOX), every cell once with the
+// nearest image taken per pair.  Neighbour cells and neighbour chunks whose
+// bounding box lies beyond rcut from the item's box are skipped.  Cells of
+// the item's system are cell0 + (cx nc + cy) nc + cz with items from
+// istart[cell]; forces go to fr[d stride + perm[i]].
 template <bool FP32, bool SMALL_BOX>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-               const float4 *__restrict__ sorted32, const int *__restrict__ perm,
-               const int *__restrict__ start, const int *__restrict__ istart,
-               const float4 *__restrict__ bbox, int nc, int reach,
+               const float4 *__restrict__ sorted32, const float *__restrict__ norm2,
+               const int *__restrict__ perm, const int *__restrict__ start,
+               const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
                RsShared<FP32> &sh, double &en_tot) {
-    constexpr int kMask = kQueue - 1;
     const int lane = threadIdx.x;
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
-    const double Li = 1.0 / L, side = L / nc;
-    // prefilter thresholds; the exact test runs in pair().  The FP32 test
-    // works on absolute FP32 coordinates (error ~ 6e-8 L per coordinate)
-    const double rc2 = rcut * rcut * (1.0 + 1e-12);
-    const float rc2f = (float)(rcut * rcut * (1.0 + 1e-5 + 1e-6 * L / rcut));
+    const double side = L / nc;
     // boxes are compared in FP32: margin for their rounding
     const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
     const float rskip2 = rskip * rskip;
-    const int lo = SMALL_BOX ? 0 : -reach, hi = SMALL_BOX ? nc - 1 : reach;
     const int hcell = cell0 + home;
     const int hb = start[hcell], he = start[hcell + 1];
     const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
     const int i0 = hb + chunk * kWave; // sorted index of lane 0's home particle
+    const int nh = min(kWave, he - i0);
     const int i = i0 + lane;
-    const bool active = i < he;
+    const bool active = lane < nh;
     const double4 pi = active ? sorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const int hitem = istart[hcell] + chunk;
-    const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
-    // absolute single-precision coordinates for the prefilter; inactive
-    // lanes are placed far away so that they never pass it
-    const float hxf = active ? (float)pi.x : 1e30f, hyf = active ? (float)pi.y : 1e30f,
-                hzf = active ? (float)pi.z : 1e30f;
     sh.home_pos[lane] = pi;
     sh.facc[0][lane] = 0.0;
     sh.facc[1][lane] = 0.0;
     sh.facc[2][lane] = 0.0;
     fill_shifts(sh, L);
-    int head = 0, count = 0; // wave-uniform ring state
+    int count = 0; // wave-uniform queue length
     __syncthreads();
 
-    // lane l evaluates ring entry head + l (l < m) for its owner
+    float en32 = 0.0f; // FP32 form: twice this item's energy, per lane
+
+    // lane l < m evaluates queue entry count - m + l for its owner
     auto drain = [&](int m) {
+        const int base = count - m;
         if (lane < m) {
-            const uint2 e = sh.q[(head + lane) & kMask];
-            const int o = (int)(e.y & 63u);
-            const double4 s3 = sh.shift[e.y >> 6];
-            PairAcc pa = {0.0, 0.0, 0.0, 0.0};
-            pair<FP32>(sh.home_pos[o], sorted[e.x], s3.x, s3.y, s3.z, rcut, alpha, two_a_spi,
-                       sh.tab, pa);
+            const unsigned int j = sh.qj[base + lane], os = sh.qo[base + lane];
+            const unsigned int ho = os & 0xffffu; // owner * 32
+            const int o = (int)(ho >> 5);
+            const char *lds_home = reinterpret_cast<const char *>(sh.home_pos);
+            const char *lds_shift = reinterpret_cast<const char *>(sh.shift);
+            const double4 s3 = *reinterpret_cast<const double4 *>(lds_shift + (os >> 16));
+            // 32-bit byte offset (the particle count is bounded on the host)
+            const double4 pj = *reinterpret_cast<const double4 *>(
+                reinterpret_cast<const char *>(sorted) + (j << 5));
+            PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
+            pair<FP32>(*reinterpret_cast<const double4 *>(lds_home + ho), pj, s3.x, s3.y, s3.z, rcut,
+                       alpha, two_a_spi, sh.tab, pa);
             atomicAdd(&sh.facc[0][o], pa.f0);
             atomicAdd(&sh.facc[1][o], pa.f1);
             atomicAdd(&sh.facc[2][o], pa.f2);
-            en_tot += pa.en;
+            if (FP32)
+                en32 += pa.enf;
+            else
+                en_tot += pa.en;
         }
-        head = (head + m) & kMask;
-        count -= m;
+        count = base;
         __syncthreads();
     };
 
-    // the accepting lanes (ok) queue the pair (own home particle, j)
+    // the accepting lanes (ok) queue the pair (neighbour j, owner | shift)
     auto append = [&](bool ok, unsigned int j, unsigned int os) {
         const unsigned long long mask = __ballot(ok);
         if (mask) {
             if (ok) {
-                const unsigned int rank = __builtin_amdgcn_mbcnt_hi(
-                    (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
-                sh.q[(head + count + (int)rank) & kMask] = make_uint2(j, os);
+                const unsigned int slot = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned int)(mask >> 32),
+                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, (unsigned int)count));
+                sh.qj[slot] = j;
+                sh.qo[slot] = os;
             }
             count += __popcll(mask);
-            if (count >= kWave) {
-                __syncthreads();
+        }
+    };
+    // evaluate full rounds of 64 queued pairs
+    auto flush = [&]() {
+        if (count >= kWave) {
+            __syncthreads();
+            do
                 drain(kWave);
-            }
+            while (count >= kWave);
         }
     };
 
-    for (int a = lo; a <= hi; a++) {
-        int cx = SMALL_BOX ? a : hx + a, wx = 0;
-        if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
-        for (int b = lo; b <= hi; b++) {
-            int cy = SMALL_BOX ? b : hy + b, wy = 0;
-            if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
-            for (int c = lo; c <= hi; c++) {
-                int cz = SMALL_BOX ? c : hz + c, wz = 0;
-                if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
-                const double sx = wx * L, sy = wy * L, sz = wz * L;
-                if (!SMALL_BOX) {
-                    // the whole neighbour cell beyond the cutoff from the item's box?
-                    const float4 clo = make_float4((float)(cx * side + sx), (float)(cy * side + sy),
-                                                   (float)(cz * side + sz), 0.0f);
-                    const float4 chi = make_float4((float)((cx + 1) * side + sx),
-                                                   (float)((cy + 1) * side + sy),
-                                                   (float)((cz + 1) * side + sz), 0.0f);
-                    if (box_dist2(hlo, hhi, clo, chi) > rskip2)
-                        continue;
+    if (SMALL_BOX) {
+        // every cell once, the nearest image per pair (the reference's
+        // round(dx/L)); FP64 test with a margin, the exact one in pair()
+        const double Li = 1.0 / L;
+        const double rc2 = rcut * rcut * (1.0 + 1e-12);
+        const int ncell = nc * nc * nc;
+        for (int cl = 0; cl < ncell; cl++) {
+            const int cell = cell0 + cl;
+            const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
+            const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+            for (int cb = jb; cb < je; cb += kWave) {
+                const int cnt = min(kWave, je - cb);
+                // one coalesced load, then lane k's particle is broadcast
+                const double4 mine = (lane < cnt) ? sorted[cb + lane] : make_double4(0.0, 0.0, 0.0, 0.0);
+                for (int k = 0; k < cnt; k++) {
+                    const int j = cb + k;
+                    double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
+                           dz = pi.z - bcast(mine.z, k);
+                    const double rx = round(dx * Li), ry = round(dy * Li), rz = round(dz * Li);
+                    dx -= rx * L;
+                    dy -= ry * L;
+                    dz -= rz * L;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    append(active && j != i && r2 <= rc2, (unsigned)j,
+                           pack_pair(lane, shift_index((int)rx, (int)ry, (int)rz)));
+                    flush();
                 }
-                const int cell = cell0 + (cx * nc + cy) * nc + cz;
-                const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
-                const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-                const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
-                const unsigned int os_cell = pack_owner_shift(lane, wx, wy, wz);
-                const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
-                // home particles relative to the neighbour cell's image
-                const float pxs = hxf - fsx, pys = hyf - fsy, pzs = hzf - fsz;
-                // neighbours in chunks of 64
-                for (int cb = jb; cb < je; cb += kWave) {
-                    const int cnt = min(kWave, je - cb);
-                    if (!SMALL_BOX) {
+            }
+        }
+    } else {
+        const int hitem = istart[hcell] + chunk;
+        const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
+        const int kc = lane >> 4, col = lane & 15;
+        const int ntr = (nh + 15) >> 4; // row tiles of home particles
+        const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
+        // home particle 16 tr + col relative to the home cell centre
+        // this lane's home particle relative to the home cell centre (scalar
+        // search; inactive lanes far away)
+        const float4 hl = active ? sorted32[i] : make_float4(-2e30f, -2e30f, -2e30f, 1.0f);
+        // component kc < 3 of home particle 16 tr + col relative to the home
+        // cell centre (0 in the lanes kc = 3, which sum the squares)
+        floatx4 hk;
+        bool hv[4];
+#pragma unroll
+        for (int tr = 0; tr < 4; tr++) {
+            hv[tr] = 16 * tr + col < nh;
+            const float *s = reinterpret_cast<const float *>(sorted32 + i0 + 16 * tr + col);
+            hk[tr] = (hv[tr] && kc < 3) ? -0.5f * s[kc] : 0.0f;
+        }
+        for (int a = -reach; a <= reach; a++) {
+            int cx = hx + a, wx = 0;
+            if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
+            // the neighbour cell's image spans [(hx + a) side, (hx + a + 1) side]
+            const float gx = gap(hlo.x, hhi.x, (float)((hx + a) * side), (float)((hx + a + 1) * side));
+            for (int b = -reach; b <= reach; b++) {
+                int cy = hy + b, wy = 0;
+                if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
+                const float gy = gap(hlo.y, hhi.y, (float)((hy + b) * side), (float)((hy + b + 1) * side));
+                if (gx * gx + gy * gy > rskip2)
+                    continue;
+                for (int c = -reach; c <= reach; c++) {
+                    int cz = hz + c, wz = 0;
+                    if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
+                    const float gz = gap(hlo.z, hhi.z, (float)((hz + c) * side), (float)((hz + c + 1) * side));
+                    // the whole neighbour cell beyond the cutoff from the item's box?
+                    if (gx * gx + gy * gy + gz * gz > rskip2)
+                        continue;
+                    const int cell = cell0 + (cx * nc + cy) * nc + cz;
+                    const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
+                    const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+                    if (jb == je)
+                        continue;
+                    const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
+                    // B operand: h' = h - o (o the neighbour cell's centre
+                    // relative to the home cell's) in the lanes kc < 3, and
+                    // |h'|^2 - rc2f in the lanes kc = 3, summed over the four
+                    // lanes of a column (lanes col + 16 k) by two lane swaps
+                    floatx4 bop;
+                    if (FP32) {
+                        const int ok = kc == 0 ? a : (kc == 1 ? b : (kc == 2 ? c : 0));
+                        const float oc = (float)(ok * side);
+#pragma unroll
+                        for (int tr = 0; tr < 4; tr++) {
+                            const float p = hk[tr] - oc;
+                            const float q = p * p;
+                            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(q),
+                                                                              __float_as_uint(q), false, false);
+                            const float q2 = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+                            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q2),
+                                                                              __float_as_uint(q2), false, false);
+                            const float n2 = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+                            bop[tr] = kc < 3 ? p : (hv[tr] ? n2 - rc2f : 1e30f);
+                        }
+                    }
+                    const unsigned int shp = pack_pair(0, shift_index(wx, wy, wz));
+                    const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
+                    for (int cb = jb; cb < je; cb += kWave) {
+                        const int cnt = min(kWave, je - cb);
                         const int nitem = item0 + (cb - jb) / kWave;
                         float4 nlo = bbox[2 * nitem], nhi = bbox[2 * nitem + 1];
                         nlo.x += fsx; nlo.y += fsy; nlo.z += fsz;
                         nhi.x += fsx; nhi.y += fsy; nhi.z += fsz;
                         if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
                             continue;
-                    }
-                    if (SMALL_BOX) {
-                        // one coalesced load, then lane k's particle is broadcast
-                        const int jj = cb + lane;
-                        const double4 mine = (lane < cnt) ? sorted[jj] : make_double4(0.0, 0.0, 0.0, 0.0);
-                        for (int k = 0; k < cnt; k++) {
-                            const int j = cb + k;
-                            double dx = pi.x - bcast(mine.x, k), dy = pi.y - bcast(mine.y, k),
-                                   dz = pi.z - bcast(mine.z, k);
-                            // nearest image per axis (the reference's round(dx/L))
-                            const double rx = round(dx * Li), ry = round(dy * Li), rz = round(dz * Li);
-                            dx -= rx * L;
-                            dy -= ry * L;
-                            dz -= rz * L;
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            append(active && j != i && r2 <= rc2, (unsigned)j,
-                                   pack_owner_shift(lane, (int)rx, (int)ry, (int)rz));
+                        if (FP32) {
+                            if (cb == i0)
+                                scan_chunk<true>(append, flush, sorted32, norm2, cb, cnt, ntr, i0, col,
+                                                 kc, bop, shp);
+                            else
+                                scan_chunk<false>(append, flush, sorted32, norm2, cb, cnt, ntr, i0, col,
+                                                  kc, bop, shp);
+                        } else {
+                            scan_chunk_scalar(append, flush, sorted32, cb, cnt, i0, lane, hl, a * side,
+                                              b * side, c * side, rc2f, pack_pair(lane, 0) | shp);
                         }
-                    } else if (cb == i0) {
-                        // the item's own chunk: exclude the self pair
-                        for (int k = 0; k < cnt; k++) {
-                            const float4 v = sorted32[cb + k];
-                            const float dx = pxs - v.x, dy = pys - v.y, dz = pzs - v.z;
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            append(r2 <= rc2f && k != lane, (unsigned)(cb + k), os_cell);
-                        }
-                    } else {
-                        // neighbours read with scalar (wave-uniform) loads, four
-                        // at a time so that their latencies overlap
-                        auto test = [&](const float4 &v, int k) {
-                            const float dx = pxs - v.x, dy = pys - v.y, dz = pzs - v.z;
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            append(r2 <= rc2f, (unsigned)(cb + k), os_cell);
-                        };
-                        int k = 0;
-                        for (; k + 4 <= cnt; k += 4) {
-                            const float4 v0 = sorted32[cb + k], v1 = sorted32[cb + k + 1],
-                                         v2 = sorted32[cb + k + 2], v3 = sorted32[cb + k + 3];
-                            test(v0, k);
-                            test(v1, k + 1);
-                            test(v2, k + 2);
-                            test(v3, k + 3);
-                        }
-                        for (; k < cnt; k++)
-                            test(sorted32[cb + k], k);
                     }
                 }
             }
@@ -469,6 +666,8 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     __syncthreads();
     if (count > 0)
         drain(count);
+    if (FP32)
+        en_tot += 0.5 * (double)en32;
     if (active) {
         const int p = perm[i];
         atomicAdd(&fr[p], sh.facc[0][lane]);
@@ -483,7 +682,7 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
 k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-            const int *__restrict__ perm,
+            const float *__restrict__ norm2, const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart,
             const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L, double rcut,
             double alpha, int *__restrict__ counter, double *__restrict__ fr,
@@ -500,8 +699,9 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, perm, start,
-                                        istart, bbox, nc, reach, L, rcut, alpha, n, fr, sh, en_tot);
+        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, norm2, perm,
+                                        start, istart, bbox, nc, reach, L, rcut, alpha, n, fr, sh,
+                                        en_tot);
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
@@ -513,13 +713,12 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
 
 }  // namespace
 
-// ---- cell-list arrays and launchers ---------------------------------------
 
 namespace p3m {
 
 void cells_release(CellArrays &a) {
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.sorted32, a.bbox};
+                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -540,8 +739,10 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.perm, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.scan, sizeof(int) * p3mk_scan_scratch_ints(8 * ncell));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.counter, sizeof(int) * 4);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * nn);
+    const size_t np = nn + kPad;
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * np);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * np);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.norm2, sizeof(float) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
     if (rc != P3MK_OK)
         cells_release(a);
@@ -594,6 +795,8 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     *out = nullptr;
     if (!(rcut > 0.0) || !(L > 0.0) || rcut > 0.5 * L)
         return p3mk_fail(P3MK_EINVAL, "real space needs 0 < rcut <= L/2", 0);
+    if ((long)n > p3m::kCellMaxParticles)
+        return p3mk_fail(P3MK_EINVAL, "real space supports at most 2^27 - 64 particles", 0);
     p3mk_cells *c = new (std::nothrow) p3mk_cells();
     if (!c)
         return p3mk_fail(P3MK_ENOMEM, "cell list", 0);
@@ -641,21 +844,21 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
         return p3mk_fail(P3MK_EINVAL, "cell list was built for another system size, box or cutoff", 0);
     CellArrays &a = c->a;
     hipStream_t s = p3mk_stream(qu);
-    const int nb = (n + 255) / 256;
+    const int nb = (n + 255) / 256, nbp = (n + kPad + 255) / 256;
     P3MK_TRY(p3m::cells_begin(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, s, n, x, y, z, L, c->inv_side, c->nc,
                        a.key, a.fcount);
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, n, x, y, z, q, L, a.key, a.fstart,
-                       a.fcursor, a.perm, a.sorted, a.sorted32);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
+                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.norm2);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     P3MK_TRY(p3m::cells_end(a, c->ncell, qu));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
     const dim3 grid((unsigned)c->grid);
 #define LAUNCH(F32, SB)                                                                            \
     hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,   \
-                       a.perm,                                                                     \
+                       a.norm2, a.perm,                                                            \
                        a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
                        a.counter, fr, energy)
     if (fp32) {
@@ -695,24 +898,29 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
                                 const double *__restrict__ z, const double *__restrict__ q,
                                 const int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
-                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
+                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
+                                float *__restrict__ norm2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot)
+    if (i >= ntot + kPad)
         return;
-    const double L = sys[psys[i]].L;
+    if (i >= ntot) {
+        store_pad(i, sorted, sorted32, norm2);
+        return;
+    }
+    const BSys &S = sys[psys[i]];
+    const double L = S.L;
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    sorted[pos] = r;
-    sorted32[pos] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.0f);
+    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32, norm2);
 }
 
 template <bool FP32>
 __global__ void __launch_bounds__(kWave)
 k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
              const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-             const int *__restrict__ perm,
+             const float *__restrict__ norm2, const int *__restrict__ perm,
              const int *__restrict__ start, const int *__restrict__ istart,
              const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
              double *__restrict__ fr, double *__restrict__ energy) {
@@ -734,11 +942,11 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
         double en = 0.0;
         if (S.small)
             realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
-                                       perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
+                                       norm2, perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
                                        ntot, fr, sh, en);
         else
             realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
-                                        perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
+                                        norm2, perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
                                         ntot, fr, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
@@ -766,18 +974,19 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
                        a.fcount);
     P3MK_TRY(p3mk_launch_check("batched cell count"));
     P3MK_TRY(cells_scan(a, ncell, qu));
-    hipLaunchKernelGGL(k_bcell_scatter, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, q, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32);
+    hipLaunchKernelGGL(k_bcell_scatter, dim3((ntot + kPad + 255) / 256), dim3(256), 0, s, ntot, sys,
+                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32,
+                       a.norm2);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
     if (f.fp32)
         hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
-                           fr, energy);
+                           nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart, a.bbox,
+                           ncell, a.counter, fr, energy);
     else
         hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart, a.bbox, ncell, a.counter,
-                           fr, energy);
+                           nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart, a.bbox,
+                           ncell, a.counter, fr, energy);
     return p3mk_launch_check("batched real space");
 }
 
