@@ -95,6 +95,23 @@ typedef struct {
     double (*U_hat)(int, double);
 } interpolation_t;
 
+/* types.h:229-235: statistics of one stage's time over the tuning samples */
+typedef struct {
+    double avg;
+    double sgm;
+    double min;
+    double max;
+    int n;
+} timing_t;
+
+/* types.h:237-242: t = t_c + t_g + t_f (avg, min and max summed stage-wise) */
+typedef struct {
+    timing_t t;
+    timing_t t_c;
+    timing_t t_g;
+    timing_t t_f;
+} runtime_stat_t;
+
 /* types.h:243-248 */
 typedef struct {
     double t;
@@ -359,6 +376,37 @@ int p3m_hip_batch_autotune(p3m_hip_batch *b, const double *x, const double *y, c
 /* alpha for a real-space rms force error `precision` at cutoff rcut, the
  * formula of the reference's tuner (tuning.c:293). */
 double p3m_hip_tune_alpha(int n, double length, double q2, double rcut, double precision);
+
+/* ---------------------------------------------------------------------- */
+/* Tuner                                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* Tune (tuning.c:250-370) with the GPU as the timer: for a target rms force
+ * error `precision` at the cutoff p->rcut, alpha follows from the real-space
+ * estimate (tuning.c:293); meshes of the reference's candidate list
+ * (tuning.c:38, 8 .. 300) are scanned upwards and, per mesh, orders from the
+ * last accepted one downwards (from 7, or only p->cao when it is nonzero)
+ * while Error(mesh, cao) <= precision; every accepted candidate is timed by
+ * 50 Kspace_force calls (time_full, tuning.c:92-147: stage times of a data_t
+ * made with p->tuning set) and the fastest by summed minimum stage times
+ * wins.  The scan stops at the first mesh whose cheapest order (cao 2) is
+ * already slower than the best.  On success *p receives mesh, cao, ip, cao3,
+ * alpha and precision (= its error estimate) and the winner's statistics are
+ * returned; if no candidate meets the precision, p is unchanged and the
+ * returned t.avg is -1.  On a failure it prints the reason and exits (as the
+ * method table entry points do).  Timed in FP64. */
+runtime_stat_t p3m_hip_tune(const method_t *m, system_t *s, parameters_t *p, double precision);
+
+/* The timer of one candidate: fill *out (seconds) for the parameters p
+ * (mesh, cao, alpha, rcut set); return P3M_HIP_OK or an error code. */
+typedef int (*p3m_hip_tune_timer)(void *ctx, int method_id, system_t *s, parameters_t *p,
+                                  runtime_stat_t *out);
+/* The scan of p3m_hip_tune with a status and a choice of timer: timer NULL
+ * times on the GPU in `form` (P3M_HIP_FP64 or P3M_HIP_MIXED); otherwise
+ * timer(ctx, ...) is called for every timed candidate.  *out receives the
+ * winner's statistics (t.avg = -1 if none met the precision). */
+int p3m_hip_tune_scan(int method_id, system_t *s, parameters_t *p, double precision, int form,
+                      p3m_hip_tune_timer timer, void *ctx, runtime_stat_t *out);
 
 /* Stage times (ms, HIP events on the library's stream) of the last calls:
  * [0] charge assignment, [1] FFTs + influence-function multiply,

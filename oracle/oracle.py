@@ -232,3 +232,61 @@ def rms(f, fref):
     f = np.asarray(f).reshape(3, -1)
     fref = np.asarray(fref).reshape(3, -1)
     return float(np.sqrt(np.sum((f - fref) ** 2) / f.shape[1]))
+
+
+# Candidate meshes of the reference's tuner (tuning.c:38), ascending.
+TUNE_MESHES = (8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40, 42, 44, 48, 50, 52, 54,
+               56, 60, 64, 66, 70, 72, 78, 80, 84, 88, 90, 96, 98, 100, 104, 108, 110, 112, 120,
+               126, 128, 130, 132, 140, 144, 150, 154, 156, 160, 162, 168, 176, 180, 182, 192,
+               196, 198, 200, 208, 210, 216, 220, 224, 234, 240, 242, 250, 252, 256, 260, 264,
+               270, 280, 288, 294, 300)
+
+
+def tune_alpha(N, L, q2, rcut, precision):
+    """alpha of the tuner (tuning.c:293): the real-space estimate
+    (realpart.c:343-346) equal to precision / sqrt(2)."""
+    V = L ** 3
+    return np.sqrt(-np.log(precision * np.sqrt(N * rcut * V) / (2.0 * np.sqrt(2.0) * q2))) / rcut
+
+
+def total_error(method, N, L, q2, M, cao, alpha, rcut):
+    """Error of the method table: sqrt(real^2 + k^2) (e.g. p3m-ik.c:249-261)."""
+    r = realspace_error(N, L, q2, rcut, alpha)
+    k = kspace_error(method, N, L, q2, M, cao, alpha)
+    return float(np.sqrt(r * r + k * k))
+
+
+def tune_scan(method, N, L, q2, rcut, cao, precision, timer, error=None):
+    """Tune (tuning.c:250-370) with an injected timer(mesh, cao) -> seconds
+    (the summed minimum stage time the reference compares) and, optionally,
+    an injected error(mesh, cao, alpha) (default: the method's estimate).
+    Meshes ascend through TUNE_MESHES; orders descend from 7 (or only `cao`
+    when nonzero) while the estimate meets the precision, later meshes from
+    one below the last order that met it; a mesh whose order-2 time already
+    exceeds the best ends the scan.  Returns (mesh, cao, alpha, error, time)
+    of the winner, or None."""
+    error = error or (lambda M, c, a: total_error(method, N, L, q2, M, c, a, rcut))
+    alpha = tune_alpha(N, L, q2, rcut, precision)
+    lo, hi = (cao, cao) if cao else (2, 7)
+    last, found = hi, False
+    best, best_t, best_mc = None, float("inf"), (float("inf"), 8)
+    for M in TUNE_MESHES:
+        if found:
+            if last <= lo:
+                break
+            first = last - 1
+        else:
+            first = last = hi
+        if timer(M, lo) > best_t:
+            break
+        for c in range(first, lo - 1, -1):
+            err = error(M, c, alpha)
+            if not err <= precision:
+                break
+            last, found = c, True
+            if M >= best_mc[0] and c >= best_mc[1]:
+                break
+            t = timer(M, c)
+            if t < best_t:
+                best, best_t, best_mc = (M, c, alpha, err, t), t, (M, c)
+    return best

@@ -67,6 +67,23 @@ class Data(ctypes.Structure):
                 ("runtime", Runtime)]
 
 
+class Timing(ctypes.Structure):
+    """timing_t (types.h:229-235)."""
+    _fields_ = [("avg", ctypes.c_double), ("sgm", ctypes.c_double), ("min", ctypes.c_double),
+                ("max", ctypes.c_double), ("n", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in ("avg", "sgm", "min", "max", "n")}
+
+
+class RuntimeStat(ctypes.Structure):
+    """runtime_stat_t (types.h:237-242)."""
+    _fields_ = [("t", Timing), ("t_c", Timing), ("t_g", Timing), ("t_f", Timing)]
+
+    def as_dict(self):
+        return {k: getattr(self, k).as_dict() for k in ("t", "t_c", "t_g", "t_f")}
+
+
 class BatchSystem(ctypes.Structure):
     _fields_ = [("nparticles", ctypes.c_int), ("length", ctypes.c_double), ("mesh", ctypes.c_int),
                 ("cao", ctypes.c_int), ("alpha", ctypes.c_double), ("rcut", ctypes.c_double)]
@@ -78,6 +95,8 @@ InitFn = ctypes.CFUNCTYPE(_DPT, _SP, _PP)
 InfluenceFn = ctypes.CFUNCTYPE(None, _SP, _PP, _DPT)
 KspaceFn = ctypes.CFUNCTYPE(None, _SP, _PP, _DPT, _FP)
 ErrorFn = ctypes.CFUNCTYPE(ctypes.c_double, _SP, _PP)
+TuneTimerFn = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _SP, _PP,
+                               ctypes.POINTER(RuntimeStat))
 
 
 class Method(ctypes.Structure):
@@ -137,6 +156,8 @@ def lib():
             "p3m_hip_batch_synchronize": (i, [vp]),
             "p3m_hip_batch_autotune": (i, [vp] + [vp] * 7 + [ip, ip]),
             "p3m_hip_tune_alpha": (d, [i, d, d, d, d]),
+            "p3m_hip_tune_scan": (i, [i, _SP, _PP, d, i, TuneTimerFn, vp, ctypes.POINTER(RuntimeStat)]),
+            "p3m_hip_tune": (RuntimeStat, [ctypes.POINTER(Method), _SP, _PP, d]),
             "p3m_hip_ewald_sample_device": (i, [i, d, vp, vp, vp, vp, i, vp, d, d, i, d, vp, vp]),
             "p3m_hip_stage_ring_start": (i, [_DPT, i]),
             "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
@@ -418,6 +439,35 @@ class Batch:
 def tune_alpha(n, length, q2, rcut, precision):
     """alpha of the reference's tuner for a real-space error target (tuning.c:293)."""
     return lib().p3m_hip_tune_alpha(int(n), float(length), float(q2), float(rcut), float(precision))
+
+
+def tune(method_id, system, params, precision, form=FP64, timer=None):
+    """Tune (tuning.c:250-370): the fastest (mesh, cao) meeting `precision`
+    at params.rcut (params.cao fixes the order when nonzero), alpha from the
+    real-space target.  Returns (parameters, statistics) -- statistics as
+    runtime_stat_t in seconds, t.avg = -1 and params unchanged if nothing met
+    the precision.  `timer` (optional) replaces the GPU timer: a callable
+    (method_id, mesh, cao, alpha) -> seconds, used for every stage statistic
+    of t and t_c (t_g = t_f = 0)."""
+    p = Parameters()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(params), ctypes.sizeof(Parameters))
+    out = RuntimeStat()
+    cb = TuneTimerFn()
+    if timer is not None:
+        def _cb(ctx, mid, sp, pp, res):
+            try:
+                v = float(timer(mid, pp.contents.mesh, pp.contents.cao, pp.contents.alpha))
+            except Exception:
+                return EINVAL
+            r = res.contents
+            for name in ("t", "t_c", "t_g", "t_f"):
+                tv = v if name in ("t", "t_c") else 0.0
+                setattr(r, name, Timing(tv, 0.0, tv, tv, 50))
+            return OK
+        cb = TuneTimerFn(_cb)
+    check(lib().p3m_hip_tune_scan(method_id, system.ptr, ctypes.pointer(p), float(precision), form,
+                                  cb, None, ctypes.byref(out)))
+    return p, out.as_dict()
 
 
 def ewald_error_estimates(n, length, q2, alpha, rcut, kmax):

@@ -69,6 +69,8 @@ int main(void) {
   P(data_t, forward_plans); P(data_t, forward_plan); P(data_t, neighbor_list);
   P(data_t, self_force_corrections); P(data_t, method_data); P(data_t, runtime);
   S(method_t); P(method_t, flags); P(method_t, Init); P(method_t, Error_k);
+  S(timing_t); P(timing_t, min); P(timing_t, n);
+  S(runtime_stat_t); P(runtime_stat_t, t_c); P(runtime_stat_t, t_f);
   return 0;
 }
 """
@@ -81,7 +83,8 @@ def test_struct_layout_matches_python_mirror(pkg, tmp_path):
     subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     out = subprocess.check_output([str(exe)]).decode().split("\n")
     table = {"vector_array_t": pkg.VectorArray, "forces_t": pkg.Forces, "system_t": pkg.System,
-             "parameters_t": pkg.Parameters, "data_t": pkg.Data, "method_t": pkg.Method}
+             "parameters_t": pkg.Parameters, "data_t": pkg.Data, "method_t": pkg.Method,
+             "timing_t": pkg.Timing, "runtime_stat_t": pkg.RuntimeStat}
     for line in out:
         if not line:
             continue
