@@ -408,6 +408,34 @@ typedef int (*p3m_hip_tune_timer)(void *ctx, int method_id, system_t *s, paramet
 int p3m_hip_tune_scan(int method_id, system_t *s, parameters_t *p, double precision, int form,
                       p3m_hip_tune_timer timer, void *ctx, runtime_stat_t *out);
 
+/* ---------------------------------------------------------------------- */
+/* On-disk formats of the reference driver                                 */
+/* ---------------------------------------------------------------------- */
+
+/* A host system_t laid out as Init_system (common.c:192-220) makes it:
+ * positions and reference forces (f, f_k, f_r) as vector_array_t with
+ * fields[3] aliasing x, y, z, zeroed; release with p3m_hip_system_free. */
+int p3m_hip_system_new(int n, double length, system_t **out);
+int p3m_hip_system_free(system_t *s);
+/* Read_system (io.c:100-157): "# Teilchenzahl: N", "# Len: L", then N lines
+ * "x y z q" (any white space); q2 = sum q^2.  *out is a p3m_hip_system_new
+ * system.  Errors are returned (the reference exits). */
+int p3m_hip_read_system(const char *path, system_t **out);
+/* Write_system (io.c:199-225): the same format, six decimals. */
+int p3m_hip_write_system(const system_t *s, const char *path);
+/* Read_exact_forces (io.c:70-98) into s->reference->f: per line "i fx fy fz";
+ * lines with the seven columns Write_exact_forces writes also fill
+ * s->reference->f_k from columns 5-7 (the reference's four-field read loses
+ * step on them). */
+int p3m_hip_read_exact_forces(system_t *s, const char *path);
+/* Write_exact_forces (io.c:159-177): "i fx fy fz fkx fky fkz" of
+ * s->reference (total and k-space), six decimals. */
+int p3m_hip_write_exact_forces(const system_t *s, const char *path);
+/* One row of the driver's out.dat (main.c:452-455): v = alpha, DeltaF,
+ * estimate, real-space estimate, k-space estimate, k-space DeltaF, Ewald
+ * k-space estimate, inhomogeneous estimate, as "% lf\t% e ... % e\n". */
+int p3m_hip_format_out_row(char *buf, size_t size, const double v[8]);
+
 /* Stage times (ms, HIP events on the library's stream) of the last calls:
  * [0] charge assignment, [1] FFTs + influence-function multiply,
  * [2] back-interpolation, [3] whole Kspace_force, [4] last real-space call

@@ -158,6 +158,13 @@ def lib():
             "p3m_hip_tune_alpha": (d, [i, d, d, d, d]),
             "p3m_hip_tune_scan": (i, [i, _SP, _PP, d, i, TuneTimerFn, vp, ctypes.POINTER(RuntimeStat)]),
             "p3m_hip_tune": (RuntimeStat, [ctypes.POINTER(Method), _SP, _PP, d]),
+            "p3m_hip_system_new": (i, [i, d, ctypes.POINTER(_SP)]),
+            "p3m_hip_system_free": (i, [_SP]),
+            "p3m_hip_read_system": (i, [ctypes.c_char_p, ctypes.POINTER(_SP)]),
+            "p3m_hip_write_system": (i, [_SP, ctypes.c_char_p]),
+            "p3m_hip_read_exact_forces": (i, [_SP, ctypes.c_char_p]),
+            "p3m_hip_write_exact_forces": (i, [_SP, ctypes.c_char_p]),
+            "p3m_hip_format_out_row": (i, [ctypes.c_char_p, ctypes.c_size_t, _dp]),
             "p3m_hip_ewald_sample_device": (i, [i, d, vp, vp, vp, vp, i, vp, d, d, i, d, vp, vp]),
             "p3m_hip_stage_ring_start": (i, [_DPT, i]),
             "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
@@ -468,6 +475,85 @@ def tune(method_id, system, params, precision, form=FP64, timer=None):
     check(lib().p3m_hip_tune_scan(method_id, system.ptr, ctypes.pointer(p), float(precision), form,
                                   cb, None, ctypes.byref(out)))
     return p, out.as_dict()
+
+
+def _columns(v, n):
+    return np.stack([np.ctypeslib.as_array(v.fields[d], shape=(n,)).copy() for d in range(3)])
+
+
+def read_system(path):
+    """Read_system (io.c:100-157): {"x", "y", "z", "q", "L"} from a
+    reference system file."""
+    sp = _SP()
+    check(lib().p3m_hip_read_system(os.fsencode(path), ctypes.byref(sp)))
+    try:
+        s = sp.contents
+        n = s.nparticles
+        pos = _columns(s.p.contents, n)
+        return {"x": pos[0], "y": pos[1], "z": pos[2],
+                "q": np.ctypeslib.as_array(s.q, shape=(n,)).copy(), "L": s.length}
+    finally:
+        lib().p3m_hip_system_free(sp)
+
+
+def _c_system(sysd, f=None, fk=None):
+    n = len(sysd["q"])
+    sp = _SP()
+    check(lib().p3m_hip_system_new(n, float(sysd["L"]), ctypes.byref(sp)))
+    s = sp.contents
+    for d, k in enumerate("xyz"):
+        np.ctypeslib.as_array(s.p.contents.fields[d], shape=(n,))[:] = sysd[k]
+        if f is not None:
+            np.ctypeslib.as_array(s.reference.contents.f.contents.fields[d], shape=(n,))[:] = f[d]
+        if fk is not None:
+            np.ctypeslib.as_array(s.reference.contents.f_k.contents.fields[d], shape=(n,))[:] = fk[d]
+    np.ctypeslib.as_array(s.q, shape=(n,))[:] = sysd["q"]
+    s.q2 = float(np.sum(np.asarray(sysd["q"]) ** 2))
+    return sp
+
+
+def write_system(path, sysd):
+    """Write_system (io.c:199-225)."""
+    sp = _c_system(sysd)
+    try:
+        check(lib().p3m_hip_write_system(sp, os.fsencode(path)))
+    finally:
+        lib().p3m_hip_system_free(sp)
+
+
+def read_exact_forces(path, n):
+    """Read_exact_forces (io.c:70-98): (f, f_k) as (3, n) arrays; f_k is
+    zero unless the file has the seven columns Write_exact_forces writes."""
+    sp = _SP()
+    check(lib().p3m_hip_system_new(int(n), 1.0, ctypes.byref(sp)))
+    try:
+        check(lib().p3m_hip_read_exact_forces(sp, os.fsencode(path)))
+        ref = sp.contents.reference.contents
+        return _columns(ref.f.contents, n), _columns(ref.f_k.contents, n)
+    finally:
+        lib().p3m_hip_system_free(sp)
+
+
+def write_exact_forces(path, f, fk):
+    """Write_exact_forces (io.c:159-177): total and k-space force, (3, n)."""
+    f, fk = np.asarray(f, dtype=np.float64), np.asarray(fk, dtype=np.float64)
+    n = f.shape[1]
+    sysd = {"x": np.zeros(n), "y": np.zeros(n), "z": np.zeros(n), "q": np.zeros(n), "L": 1.0}
+    sp = _c_system(sysd, f, fk)
+    try:
+        check(lib().p3m_hip_write_exact_forces(sp, os.fsencode(path)))
+    finally:
+        lib().p3m_hip_system_free(sp)
+
+
+def format_out_row(alpha, delta_f, estimate, rs_estimate, k_estimate, delta_f_k, ewald_k_estimate,
+                   inhomo_estimate=0.0):
+    """One row of the driver's out.dat (main.c:452-455)."""
+    v = np.array([alpha, delta_f, estimate, rs_estimate, k_estimate, delta_f_k, ewald_k_estimate,
+                  inhomo_estimate], dtype=np.float64)
+    buf = ctypes.create_string_buffer(512)
+    check(lib().p3m_hip_format_out_row(buf, 512, v.ctypes.data_as(_dp)))
+    return buf.value.decode()
 
 
 def ewald_error_estimates(n, length, q2, alpha, rcut, kmax):
