@@ -678,11 +678,12 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
 }
 
 // Persistent grid: each single-wavefront workgroup claims work items from an
-// atomic counter until none are left.  Five waves per SIMD (at most 102
-// VGPRs): measured faster than the unconstrained four and than six, which
-// spills.
+// atomic counter until none are left.  FP32 form: five waves per SIMD (at
+// most 102 VGPRs), measured faster than the unconstrained four and than
+// six, which spills; the FP64 form is left to the compiler (a four-wave
+// hint gave fewer registers but a slower kernel).
 template <bool FP32, bool SMALL_BOX>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 4)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
 k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
             const float *__restrict__ norm2, const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart,
@@ -919,7 +920,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
 }
 
 template <bool FP32>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 4)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
 k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
              const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
              const float *__restrict__ norm2, const int *__restrict__ perm,
