@@ -409,6 +409,30 @@ int p3m_hip_tune_scan(int method_id, system_t *s, parameters_t *p, double precis
                       p3m_hip_tune_timer timer, void *ctx, runtime_stat_t *out);
 
 /* ---------------------------------------------------------------------- */
+/* Inhomogeneous error estimate and error map                               */
+/* ---------------------------------------------------------------------- */
+
+/* Generic_error_estimate_inhomo (p3m-common.c:437-719), the driver's
+ * optional diagnostic (main.c:436-440): the k-space error of a clustered
+ * system from its squared-charge density rho^2 (order-cao assignment of q_i^2
+ * on a mesh of `mesh` points, or q2 / mesh^3 everywhere if `uniform`) and the
+ * squared error kernel of the aliasing copies m in [-mc, mc]^3 (the copy-free
+ * homogeneous kernel vanishes to rounding, as in the reference); *estimate =
+ * sqrt(sum_i rms_i / N) with rms_i the gathered error density times q_i^2.
+ * Uses p->alpha.  out_file (may be NULL) receives per mesh point
+ * "nx ny nz rho^2 dens.re dens.im K^2.re K^2.im dens.re*rho^2" as
+ * p3m-common.c:686-696 writes it.  The gather uses the estimate's mesh and
+ * order (the reference passes the method's, p3m-common.c:681). */
+int p3m_hip_error_inhomo(system_t *s, parameters_t *p, int uniform, int mesh, int cao, int mc,
+                         const char *out_file, double *estimate);
+/* Error_map (p3m-common.c:402-435): |f_ref->f_k - f->f_k|^2 of every particle
+ * assigned with order cao to a complex mesh of `mesh` points;
+ * error_mesh (host, 2 mesh^3 doubles, interleaved re/im, x-major) receives
+ * it (real parts; imaginary parts 0). */
+int p3m_hip_error_map(system_t *s, forces_t *f, forces_t *f_ref, int mesh, int cao,
+                      double *error_mesh);
+
+/* ---------------------------------------------------------------------- */
 /* On-disk formats of the reference driver                                 */
 /* ---------------------------------------------------------------------- */
 

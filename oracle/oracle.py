@@ -290,3 +290,118 @@ def tune_scan(method, N, L, q2, rcut, cao, precision, timer, error=None):
             if t < best_t:
                 best, best_t, best_mc = (M, c, alpha, err, t), t, (M, c)
     return best
+
+
+# ---- inhomogeneous error estimate (p3m-common.c:356-719) --------------------
+
+def _sinc(d):
+    """sinc of p3m-common.c:71-75."""
+    d = np.asarray(d, dtype=np.float64)
+    pd = np.pi * d
+    out = np.ones_like(d)
+    nz = d != 0.0
+    out[nz] = np.sin(pd[nz]) / pd[nz]
+    return out
+
+
+def _window(nx, ny, nz, M, cao, power):
+    """U (power 1) and U^2 (power 2) of p3m-common.c:370-377."""
+    return (_sinc(nx / M) * _sinc(ny / M) * _sinc(nz / M)) ** int(power * cao)
+
+
+def _wave_numbers(M):
+    i = np.arange(M)
+    s = np.where(i >= M // 2, i - M, i).astype(np.float64)
+    return np.meshgrid(s, s, s, indexing="ij")
+
+
+def inhomo_kernel_terms(M, L, alpha, cao):
+    """Per mesh point n (signed wave numbers, x-major): (n, G, U) with G the
+    aliasing-free b / a of p3m-common.c:356-396 and U the window; n = 0
+    carries G = 0."""
+    n = _wave_numbers(M)
+    k2 = (2.0 * np.pi / L) ** 2 * (n[0] ** 2 + n[1] ** 2 + n[2] ** 2)
+    zero = k2 == 0.0
+    k2s = np.where(zero, 1.0, k2)
+    gc = (4.0 * np.pi / k2s) * np.exp(-k2s / (4.0 * alpha * alpha))
+    u2 = _window(n[0], n[1], n[2], M, cao, 2)
+    G = np.where(zero, 0.0, (u2 * gc * k2s) / (u2 * (k2s * u2)))
+    return n, G, gc, u2, zero
+
+
+def _backward(a):
+    return np.fft.ifftn(a) * a.size  # unnormalised, sign +1
+
+
+def inhomo_k2(M, L, alpha, cao, mc, factorised=True):
+    """K^2(r) = sum over the homogeneous and aliasing kernels of
+    sum_d Re(K_d(r))^2 (p3m-common.c:490-640).  factorised=False evaluates
+    the reference's double sum over (n, n2) literally (tiny M only)."""
+    n, G, gc, u2, zero = inhomo_kernel_terms(M, L, alpha, cao)
+    K3 = np.zeros((M, M, M))
+    k2 = np.where(zero, 0.0, u2 * G - gc)
+    for d in range(3):
+        K3 += _backward(1j * (-2.0 * np.pi * n[d] / L) * k2).real ** 2
+    u = _window(n[0], n[1], n[2], M, cao, 1)
+    for mx in range(-mc, mc + 1):
+        for my in range(-mc, mc + 1):
+            for mz in range(-mc, mc + 1):
+                if mx == 0 and my == 0 and mz == 0:
+                    continue
+                um = _window(n[0] + mx * M, n[1] + my * M, n[2] + mz * M, M, cao, 1)
+                um2 = _window(-n[0] - mx * M, -n[1] - my * M, -n[2] - mz * M, M, cao, 1)
+                k1 = np.where(zero, 0.0, 2.0 * np.pi * u * um * G)
+                kk2 = np.where(zero, 0.0, 2.0 * np.pi * u * um2 * G)
+                for d in range(3):
+                    if factorised:
+                        S = np.sum((n[d] / L) * k1)
+                        Kd = -(n[d] / L) * kk2 * S
+                    else:
+                        a = ((n[d] / L) * k1).ravel()
+                        b = ((n[d] / L) * kk2).ravel()
+                        Kd = (-np.outer(a, b)).sum(axis=0).reshape(M, M, M)
+                    K3 += _backward(1j * Kd).real ** 2
+    return K3
+
+
+def _gather(sysd, mesh_real, M, cao):
+    """sum_stencil q_i^2 W mesh (collect_rms_nocf, charge-assign.c:1655-1702)
+    with the mesh's own spacing and order."""
+    L = sysd["L"]
+    Hi = M / L
+    out = np.zeros(len(sysd["q"]))
+    for i in range(len(out)):
+        idx, w = [], []
+        for k in "xyz":
+            _, base, arg = locate(float(sysd[k][i]), Hi, cao, 0, M)
+            xq = lib().orc_quantized_offset(arg)
+            idx.append((base + np.arange(cao)) % M)
+            w.append(np.array([weight(j, xq, cao) for j in range(cao)]))
+        W = np.einsum("a,b,c->abc", *w)
+        out[i] = sysd["q"][i] ** 2 * np.sum(mesh_real[np.ix_(*idx)] * W)
+    return out
+
+
+def error_inhomo(sysd, alpha, M, cao, mc, uniform=False):
+    """Generic_error_estimate_inhomo (p3m-common.c:437-719) with the gather
+    on the estimate's own mesh and order: sqrt(sum_i rms_i / N)."""
+    q = np.asarray(sysd["q"], dtype=np.float64)
+    if uniform:
+        Q = np.full((M, M, M), np.sum(q ** 2) / M ** 3, dtype=np.complex128)
+    else:
+        sq = dict(sysd)
+        sq["q"] = q ** 2
+        Q = assign_mesh(0, sq, M, cao).view(np.complex128).reshape(M, M, M)
+    K3h = np.fft.fftn(inhomo_k2(M, sysd["L"], alpha, cao, mc))
+    Qh = np.fft.fftn(Q)
+    dens = _backward(Qh.real * K3h.real + 1j * (Qh.imag * K3h.imag))
+    rms = _gather(sysd, dens.real, M, cao)
+    return float(np.sqrt(np.sum(rms) / len(q)))
+
+
+def error_map(sysd, fk, fk_ref, M, cao):
+    """Error_map (p3m-common.c:402-435): |fk_ref - fk|^2 per particle
+    assigned with order cao; real (M, M, M)."""
+    d = dict(sysd)
+    d["q"] = np.sum((np.asarray(fk_ref) - np.asarray(fk)) ** 2, axis=0)
+    return assign_mesh(0, d, M, cao)[0::2].reshape(M, M, M)

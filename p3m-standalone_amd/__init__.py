@@ -158,6 +158,8 @@ def lib():
             "p3m_hip_tune_alpha": (d, [i, d, d, d, d]),
             "p3m_hip_tune_scan": (i, [i, _SP, _PP, d, i, TuneTimerFn, vp, ctypes.POINTER(RuntimeStat)]),
             "p3m_hip_tune": (RuntimeStat, [ctypes.POINTER(Method), _SP, _PP, d]),
+            "p3m_hip_error_inhomo": (i, [_SP, _PP, i, i, i, i, ctypes.c_char_p, _dp]),
+            "p3m_hip_error_map": (i, [_SP, _FP, _FP, i, i, _dp]),
             "p3m_hip_system_new": (i, [i, d, ctypes.POINTER(_SP)]),
             "p3m_hip_system_free": (i, [_SP]),
             "p3m_hip_read_system": (i, [ctypes.c_char_p, ctypes.POINTER(_SP)]),
@@ -219,6 +221,10 @@ class _Vec:
 
     def numpy(self):
         return np.stack(self.arrays)
+
+    def set(self, a):
+        for d in range(3):
+            self.arrays[d][:] = a[d]
 
 
 class HostSystem:
@@ -380,6 +386,30 @@ def error_estimate(method_id, system, params):
     check(lib().p3m_hip_error(method_id, system.ptr, ctypes.pointer(params), ctypes.byref(t),
                               ctypes.byref(k)))
     return t.value, k.value
+
+
+def error_inhomo(system, alpha, mesh=64, cao=5, mc=0, uniform=False, out_file=None):
+    """Generic_error_estimate_inhomo (p3m-common.c:437-719) at alpha with the
+    estimate's own mesh and order (the driver's defaults: 64, 5, mc 0)."""
+    p = parameters(mesh, cao, alpha)
+    v = ctypes.c_double()
+    check(lib().p3m_hip_error_inhomo(system.ptr, ctypes.pointer(p), int(bool(uniform)), int(mesh),
+                                     int(cao), int(mc),
+                                     os.fsencode(out_file) if out_file else None, ctypes.byref(v)))
+    return v.value
+
+
+def error_map(system, fk, fk_ref, mesh=64, cao=1):
+    """Error_map (p3m-common.c:402-435): (mesh, mesh, mesh) real array of
+    |fk_ref - fk|^2 per particle assigned with order cao."""
+    n = system.n
+    f, fr = HostForces(n), HostForces(n)
+    f.f_k.set(np.asarray(fk).reshape(3, n))
+    fr.f_k.set(np.asarray(fk_ref).reshape(3, n))
+    out = np.zeros(2 * mesh ** 3)
+    check(lib().p3m_hip_error_map(system.ptr, f.ptr, fr.ptr, int(mesh), int(cao),
+                                  out.ctypes.data_as(_dp)))
+    return out[0::2].reshape(mesh, mesh, mesh)
 
 
 def set_error_table(kind, mesh, cao, table):

@@ -129,6 +129,31 @@ int p3mk_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F
 int p3mk_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void *phi,
                    double prefac, const double *selfb, double *fk, double *f, p3mk_queue *qu);
 
+/* ---- inhomogeneous error estimate and error map (p3m-common.c:402-719) ---- */
+/* complex FP64 M^3 meshes, x-major; scratch of p3mk_inhomo_scratch_doubles() */
+int p3mk_inhomo_scratch_doubles(void);
+/* homogeneous error kernel K_d(n) = i (-2 pi n_d / L) (U^2 G - G_m), mc = 0 */
+int p3mk_inhomo_homo(int M, double L, double alpha, int cao, void *const K[3], p3mk_queue *q);
+/* S_d = sum_{n != 0} (n_d / L) 2 pi U(n) U(n + m M) G(n) (host result) */
+int p3mk_inhomo_sums(int M, double L, double alpha, int cao, const int m[3], double S[3],
+                     double *scratch, p3mk_queue *q);
+/* aliasing-copy kernel K_d(n) = i (-(n_d / L) 2 pi U(n) U(-n - m M) G(n) S_d) */
+int p3mk_inhomo_fill(int M, double L, double alpha, int cao, const int m[3], const double S[3],
+                     void *const K[3], p3mk_queue *q);
+/* K3 = (first) or += sum_d Re(K_d)^2 in the real part */
+int p3mk_inhomo_square_add(int M, void *const K[3], void *K3, int first, p3mk_queue *q);
+/* Km.re *= K3.re, Km.im *= K3.im */
+int p3mk_inhomo_product(int M, void *Km, const void *K3, p3mk_queue *q);
+/* mesh = v + 0 i at every point */
+int p3mk_fill_real(int M, double v, void *mesh, p3mk_queue *q);
+/* out_i = q_i^2; out_i = |b_i - a_i|^2 of two 3n force arrays */
+int p3mk_square_charges(int n, const double *q, double *out, p3mk_queue *qu);
+int p3mk_diff_charges(int n, const double *a, const double *b, double *out, p3mk_queue *qu);
+/* rms_i = sum over the order-cao stencil of q_i^2 W Re(mesh), spacing L/M */
+int p3mk_collect_rms(int M, int cao, double L, int n, const double *x, const double *y,
+                     const double *z, const double *q, const void *mesh, double *rms,
+                     p3mk_queue *qu);
+
 /* ---- influence function, self force, estimates --------------------------- */
 /* G (M^3 FP64, r_ind order) of method id 0..3, 6, 7 at alpha; mc aliasing
  * copies per side.  Dn and nshift are computed on the device too. */
