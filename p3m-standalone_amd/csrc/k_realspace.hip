@@ -55,6 +55,9 @@ constexpr int kMaxCellsPerSide = 512;
 // sorted arrays carry kPad entries past the last particle: a 16-row tile of
 // a chunk's last rows may read up to 15 rows beyond it
 constexpr int kPad = 64;
+// home cells of more particles than this use the matrix-core candidate search
+// (FP32 form); sparser ones the vector search, whose per-cell cost is lower
+constexpr int kMfmaMinCell = 4;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -578,8 +581,8 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         const int kc = lane >> 4, col = lane & 15;
         const int ntr = (nh + 15) >> 4; // row tiles of home particles
         const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
-        // home particle 16 tr + col relative to the home cell centre
-        // this lane's home particle relative to the home cell centre (scalar
+        const bool tiles = FP32 && he - hb > kMfmaMinCell;
+        // this lane's home particle relative to the home cell centre (vector
         // search; inactive lanes far away)
         const float4 hl = active ? sorted32[i] : make_float4(-2e30f, -2e30f, -2e30f, 1.0f);
         // component kc < 3 of home particle 16 tr + col relative to the home
@@ -590,7 +593,7 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         for (int tr = 0; tr < 4; tr++) {
             hv[tr] = 16 * tr + col < nh;
             const float *s = reinterpret_cast<const float *>(sorted32 + i0 + 16 * tr + col);
-            hk[tr] = (hv[tr] && kc < 3) ? -0.5f * s[kc] : 0.0f;
+            hk[tr] = (tiles && hv[tr] && kc < 3) ? -0.5f * s[kc] : 0.0f;
         }
         for (int a = -reach; a <= reach; a++) {
             int cx = hx + a, wx = 0;
@@ -621,11 +624,13 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                     // |h'|^2 - rc2f in the lanes kc = 3, summed over the four
                     // lanes of a column (lanes col + 16 k) by two lane swaps
                     floatx4 bop;
-                    if (FP32) {
+                    if (tiles) {
                         const int ok = kc == 0 ? a : (kc == 1 ? b : (kc == 2 ? c : 0));
                         const float oc = (float)(ok * side);
 #pragma unroll
                         for (int tr = 0; tr < 4; tr++) {
+                            if (tr >= ntr)
+                                break;
                             const float p = hk[tr] - oc;
                             const float q = p * p;
                             const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(q),
@@ -647,7 +652,7 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                         nhi.x += fsx; nhi.y += fsy; nhi.z += fsz;
                         if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
                             continue;
-                        if (FP32) {
+                        if (tiles) {
                             if (cb == i0)
                                 scan_chunk<true>(append, flush, sorted32, norm2, cb, cnt, ntr, i0, col,
                                                  kc, bop, shp);
