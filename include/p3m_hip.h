@@ -297,6 +297,21 @@ int p3m_hip_kspace_force_vjp_device(data_t *d, int n, double length, const doubl
                                     const double *y, const double *z, const double *q,
                                     const double *gF, double *dpos, double *dq);
 
+/* Adjoint of the real-space force (Realteil, realpart.c:124-167); no
+ * counterpart in the reference (the differentiable form asked of this
+ * library, for losses over total forces).  Over the pair set of Realteil,
+ * with f_ij = q_i q_j s(r) d and s = (erfc(alpha r)/r + 2 alpha/sqrt(pi)
+ * exp(-alpha^2 r^2)) / r^2: for a cotangent dF of f_r, adds
+ * sum_j q_i q_j (s dg + s'(r)/r (d . dg) d) to dpos_i and
+ * sum_j q_j s (d . dg) to dq_i, dg = dF_i - dF_j.  FP64.  Host arrays. */
+int p3m_hip_realteil_vjp(system_t *s, parameters_t *p, const vector_array_t *dF,
+                         vector_array_t *dpos, double *dq);
+/* Device-resident: gF (3n) in, dpos (3n) and dq (n) accumulated; enqueued
+ * on the library's stream of d. */
+int p3m_hip_realteil_vjp_device(data_t *d, int n, double length, const double *x, const double *y,
+                                const double *z, const double *q, double rcut, double alpha,
+                                const double *gF, double *dpos, double *dq);
+
 /* Error estimates with a status (Error / Error_k of the method table):
  * realpart.c:343-346 with p3m-ik.c:234-286, p3m-ik-i.c:280-305,394-399,
  * p3m-ad.c:475-505, p3m-ad-i.c:237-279.  The k-space sums run on the GPU. */

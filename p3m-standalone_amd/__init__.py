@@ -145,6 +145,9 @@ def lib():
             "p3m_hip_kspace_force_vjp": (i, [_SP, _PP, _DPT, ctypes.POINTER(VectorArray),
                                              ctypes.POINTER(VectorArray), _dp]),
             "p3m_hip_kspace_force_vjp_device": (i, [_DPT, i, d, vp, vp, vp, vp, vp, vp, vp]),
+            "p3m_hip_realteil_vjp": (i, [_SP, _PP, ctypes.POINTER(VectorArray),
+                                         ctypes.POINTER(VectorArray), _dp]),
+            "p3m_hip_realteil_vjp_device": (i, [_DPT, i, d, vp, vp, vp, vp, d, d, vp, vp, vp]),
             "p3m_hip_batch_create": (i, [i, i, i, ctypes.POINTER(BatchSystem),
                                          ctypes.POINTER(vp)]),
             "p3m_hip_batch_free": (i, [vp]),
@@ -345,6 +348,10 @@ class Solver:
     def kspace_force_vjp_device(self, n, length, x, y, z, q, gF, dpos, dq):
         check(lib().p3m_hip_kspace_force_vjp_device(self.d, n, length, x, y, z, q, gF, dpos, dq))
 
+    def realteil_vjp_device(self, n, length, x, y, z, q, rcut, alpha, gF, dpos, dq):
+        check(lib().p3m_hip_realteil_vjp_device(self.d, n, length, x, y, z, q, rcut, alpha, gF, dpos,
+                                                dq))
+
     def stage_ring_start(self, nsteps):
         """Record the stage events of the next `nsteps` device force calls."""
         check(lib().p3m_hip_stage_ring_start(self.d, int(nsteps)))
@@ -378,6 +385,20 @@ def realteil(system, rcut, alpha, forces=None):
     p = parameters(2, 1, alpha, rcut)
     check(lib().p3m_hip_realteil(system.ptr, ctypes.pointer(p), forces.ptr))
     return forces
+
+
+def realteil_vjp(system, rcut, alpha, dF):
+    """Adjoint of the real-space force (host arrays): (dpos (3, n), dq (n))
+    of sum dF . f_r."""
+    n = system.n
+    g = _Vec(*np.asarray(dF, dtype=np.float64).reshape(3, n))
+    z = np.zeros(n)
+    dpos = _Vec(z, z, z)
+    dq = np.zeros(n)
+    p = parameters(2, 1, alpha, rcut)
+    check(lib().p3m_hip_realteil_vjp(system.ptr, ctypes.pointer(p), ctypes.pointer(g.c),
+                                     ctypes.pointer(dpos.c), dq.ctypes.data_as(_dp)))
+    return dpos.numpy(), dq
 
 
 def error_estimate(method_id, system, params):

@@ -1,4 +1,4 @@
-"""torch.autograd binding of the k-space force and its adjoint.
+"""torch.autograd bindings of the k-space and total forces and their adjoints.
 
 `KspaceForce.apply(pos, q, solver)` returns the k-space force (3, n) of an
 initialised Solver for device positions pos (3, n) and charges q (n), FP64
@@ -63,3 +63,53 @@ class KspaceForce(torch.autograd.Function):
 def kspace_force(pos, q, solver):
     """Differentiable k-space force (3, n) of `solver` at pos (3, n), q (n)."""
     return KspaceForce.apply(pos, q, solver)
+
+
+class TotalForce(torch.autograd.Function):
+    """Calculate_forces (common.c:253-312): f = f_k + f_r (ad / ad-r with the
+    self-force term), differentiable in positions and charges; the backward
+    pass adds the k-space adjoint and the real-space adjoint
+    (p3m_hip_realteil_vjp_device)."""
+
+    @staticmethod
+    def forward(ctx, pos, q, solver, rcut, alpha):
+        _check(pos, q, solver)
+        n = q.numel()
+        pos = pos.contiguous()
+        q = q.contiguous()
+        L = float(solver.system.c.length)
+        f = torch.empty(3 * n, dtype=torch.float64, device=pos.device)
+        fk = torch.empty_like(f)
+        fr = torch.empty_like(f)
+        torch.cuda.current_stream().synchronize()
+        solver.calculate_forces_device(n, L, pos[0].data_ptr(), pos[1].data_ptr(), pos[2].data_ptr(),
+                                       q.data_ptr(), float(rcut), float(alpha), f.data_ptr(),
+                                       fk.data_ptr(), fr.data_ptr(), None)
+        solver.synchronize()
+        ctx.save_for_backward(pos, q)
+        ctx.solver, ctx.rcut, ctx.alpha = solver, float(rcut), float(alpha)
+        return f.view(3, n)
+
+    @staticmethod
+    def backward(ctx, gF):
+        pos, q = ctx.saved_tensors
+        solver = ctx.solver
+        n = q.numel()
+        L = float(solver.system.c.length)
+        g = gF.to(torch.float64).contiguous().view(3 * n)
+        dpos = torch.zeros(3 * n, dtype=torch.float64, device=pos.device)
+        dq = torch.zeros(n, dtype=torch.float64, device=pos.device)
+        ptrs = (pos[0].data_ptr(), pos[1].data_ptr(), pos[2].data_ptr(), q.data_ptr())
+        torch.cuda.current_stream().synchronize()
+        solver.kspace_force_vjp_device(n, L, *ptrs, g.data_ptr(), dpos.data_ptr(), dq.data_ptr())
+        if ctx.rcut != 0.0:
+            solver.realteil_vjp_device(n, L, *ptrs, ctx.rcut, ctx.alpha, g.data_ptr(),
+                                       dpos.data_ptr(), dq.data_ptr())
+        solver.synchronize()
+        return dpos.view(3, n), dq, None, None, None
+
+
+def total_force(pos, q, solver, rcut, alpha):
+    """Differentiable total force (3, n): k space of `solver` plus real space
+    at (rcut, alpha); rcut = 0 leaves the real-space part out."""
+    return TotalForce.apply(pos, q, solver, rcut, alpha)

@@ -83,6 +83,7 @@ struct CellArrays {
     // the three sorted arrays carry 64 padding entries past the last particle
     float4 *sorted32 = nullptr;
     float *norm2 = nullptr;
+    double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
     float4 *bbox = nullptr;     // 2 per work item
     long max_items = 0;
 };

@@ -675,6 +675,80 @@ int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f) {
     return rc;
 }
 
+int p3m_hip_realteil_vjp_device(data_t *d, int n, double length, const double *x, const double *y,
+                                const double *z, const double *q, double rcut, double alpha,
+                                const double *gF, double *dpos, double *dq) {
+    p3m_ctx *c = ctx_of(d);
+    if (c == NULL)
+        return p3mh_fail(P3M_HIP_EINVAL, "data_t was not made by p3m_hip_init");
+    if (gF == NULL || dpos == NULL || dq == NULL)
+        return p3mh_fail(P3M_HIP_EINVAL, "cotangent and result arrays must not be NULL");
+    if (n > c->n)
+        return p3mh_fail(P3M_HIP_EINVAL, "more particles than Init's %d", c->n);
+    if (!(rcut > 0.0) || rcut > 0.5 * length)
+        return p3mh_fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
+    TRY(ensure_cells(c, c->n, length, rcut));
+    TRYK(p3mk_realspace_vjp(c->cells, n, length, x, y, z, q, rcut, alpha, gF, dpos, dq, c->q),
+         "real-space adjoint");
+    return P3M_HIP_OK;
+}
+
+int p3m_hip_realteil_vjp(system_t *s, parameters_t *p, const vector_array_t *dF,
+                         vector_array_t *dpos, double *dq) {
+    p3mk_queue *q = NULL;
+    p3mk_cells *cells = NULL;
+    double *pos = NULL, *g = NULL, *out = NULL, *h = NULL;
+    size_t n;
+    int rc = P3M_HIP_OK, dim;
+    if (s == NULL || p == NULL || dF == NULL || dpos == NULL || dq == NULL)
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
+    TRY(check_host_system(s));
+    n = (size_t)s->nparticles;
+    if (n == 0)
+        return P3M_HIP_OK;
+    if (!(p->rcut > 0.0) || p->rcut > 0.5 * s->length)
+        return p3mh_fail(P3M_HIP_EINVAL, "real space needs 0 < rcut <= L/2");
+    if (p3mk_queue_create(&q) != P3MK_OK)
+        return p3mh_kfail(P3M_HIP_ENODEVICE, "stream");
+#define CHK(expr, what)                     \
+    if (rc == P3M_HIP_OK) {                 \
+        int r_ = (expr);                    \
+        if (r_ != P3MK_OK)                  \
+            rc = p3mh_kfail(r_, what);      \
+    }
+    CHK(p3mk_cells_create(&cells, (int)n, s->length, p->rcut), "cell list");
+    CHK(p3mk_alloc((void **)&pos, sizeof(double) * 4 * n), "positions");
+    CHK(p3mk_alloc((void **)&g, sizeof(double) * 3 * n), "cotangent");
+    CHK(p3mk_alloc((void **)&out, sizeof(double) * 4 * n), "adjoint");
+    CHK(p3mk_alloc_host((void **)&h, sizeof(double) * 4 * n), "staging");
+    CHK(p3mk_copy_h2d(pos, s->p->x, sizeof(double) * n, q), "upload");
+    CHK(p3mk_copy_h2d(pos + n, s->p->y, sizeof(double) * n, q), "upload");
+    CHK(p3mk_copy_h2d(pos + 2 * n, s->p->z, sizeof(double) * n, q), "upload");
+    CHK(p3mk_copy_h2d(pos + 3 * n, s->q, sizeof(double) * n, q), "upload");
+    for (dim = 0; dim < 3; dim++)
+        CHK(p3mk_copy_h2d(g + dim * n, field((vector_array_t *)dF, dim), sizeof(double) * n, q),
+            "upload");
+    CHK(p3mk_fill_zero(out, sizeof(double) * 4 * n, q), "zero");
+    CHK(p3mk_realspace_vjp(cells, (int)n, s->length, pos, pos + n, pos + 2 * n, pos + 3 * n,
+                           p->rcut, p->alpha, g, out, out + 3 * n, q),
+        "real-space adjoint");
+    CHK(p3mk_copy_d2h(h, out, sizeof(double) * 4 * n, q), "download");
+    CHK(p3mk_queue_sync(q), "sync");
+#undef CHK
+    if (rc == P3M_HIP_OK) {
+        for (dim = 0; dim < 3; dim++)
+            add_host(field(dpos, dim), h + dim * n, n);
+        add_host(dq, h + 3 * n, n);
+    }
+    p3mk_release(pos);
+    p3mk_release(g);
+    p3mk_release(out);
+    p3mk_release_host(h);
+    p3mk_cells_destroy(cells);
+    p3mk_queue_destroy(q);
+    return rc;
+}
+
 int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, data_t *d,
                              forces_t *f) {
     p3m_ctx *c = ctx_of(d);

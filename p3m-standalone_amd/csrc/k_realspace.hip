@@ -114,18 +114,22 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
     atomicAdd(&count[k], 1);
 }
 
-// n + kPad threads
+// n + kPad threads; with g (3n, x/y/z blocks) also gsorted[pos] = (g_x, g_y,
+// g_z, 0) of the particle at pos (the adjoint's cotangent in sorted order)
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
                                int nc, const int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
-                               float *__restrict__ norm2) {
+                               float *__restrict__ norm2, const double *__restrict__ g,
+                               double4 *__restrict__ gsorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n + kPad)
         return;
     if (i >= n) {
         store_pad(i, sorted, sorted32, norm2);
+        if (g != nullptr)
+            gsorted[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
     }
     const int k = key[i];
@@ -133,6 +137,8 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     perm[pos] = i;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
     store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32, norm2);
+    if (g != nullptr)
+        gsorted[pos] = make_double4(g[i], g[n + i], g[2 * (long)n + i], 0.0);
 }
 
 // cell starts from the octant starts; work items per cell: ceil(count / 64)
@@ -312,18 +318,63 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
     }
 }
 
+// Adjoint of the pair force for the ordered pair (i, j) of the reference's
+// pair set.  With f_ij = q_i q_j s(r) d, s = (erfc(alpha r)/r + c e^{-alpha^2
+// r^2}) / r^2 (c = 2 alpha / sqrt(pi)), the Jacobian d f_ij / d d is
+// q_i q_j (s I + (s'/r) d d^T), symmetric and even in d, so for a cotangent g
+// of the forces the owner i collects, over its partners j,
+//   d pos_i += q_i q_j (s dg + (s'/r) (d . dg) d),  d q_i += q_j s (d . dg),
+// dg = g_i - g_j (each ordered pair is evaluated once for its owner).
+// acc = (d pos_i, d q_i).  FP64.
+__device__ __forceinline__ void pair_vjp(const double4 &pi, const double4 &pj, double sx, double sy,
+                                         double sz, double rcut, double alpha, double two_a_spi,
+                                         const double *tab, const double4 &gi, const double4 &gj,
+                                         double acc[4]) {
+    double dx, dy, dz;
+    {
+#pragma clang fp contract(off)
+        dx = (pi.x - pj.x) - sx;
+        dy = (pi.y - pj.y) - sy;
+        dz = (pi.z - pj.z) - sz;
+    }
+    if (!within_cutoff(dx, dy, dz, rcut))
+        return;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double rinv = rsqrt(r2);
+    const double r = r2 * rinv;
+    const double x = alpha * r;
+    const double x2 = x * x;
+    const double e0 = exp(-x2);
+    const double e = fma(-e0, fma(x, x, -x2), e0); // exp(-alpha^2 r^2)
+    const double ec = e * erfcx_d(x, tab);           // erfc(alpha r)
+    const double phi = ec * rinv + two_a_spi * e;
+    const double dphi = -ec * (rinv * rinv) - two_a_spi * e * (rinv + 2.0 * alpha * alpha * r);
+    const double s = phi * (rinv * rinv);
+    const double ds_r = (dphi * (rinv * rinv) - 2.0 * phi * (rinv * rinv * rinv)) * rinv;
+    const double gx = gi.x - gj.x, gy = gi.y - gj.y, gz = gi.z - gj.z;
+    const double ddg = dx * gx + dy * gy + dz * gz;
+    const double qq = pi.w * pj.w;
+    acc[0] += qq * (s * gx + ds_r * ddg * dx);
+    acc[1] += qq * (s * gy + ds_r * ddg * dy);
+    acc[2] += qq * (s * gz + ds_r * ddg * dz);
+    acc[3] += pj.w * s * ddg;
+}
+
 // LDS of one wavefront's work item: the queue of accepted pairs, the home
-// particles, their force accumulators and, for FP64, the erfcx table.
-template <bool FP32> struct RsShared {
+// particles, their accumulators (force, or for the adjoint d pos and d q and
+// the home cotangents) and, for FP64, the erfcx table.
+template <bool FP32, bool VJP = false> struct RsShared {
     unsigned int qj[kQueue]; // neighbour's sorted index
     unsigned int qo[kQueue]; // pack_pair(owner lane, shift index)
     double4 shift[27]; // image shift (sx, sy, sz) of shift index (wx+1)*9 + (wy+1)*3 + wz+1
     double4 home_pos[kWave];
-    double facc[3][kWave];
+    double facc[VJP ? 4 : 3][kWave];
+    double4 home_g[VJP ? kWave : 1];
     double tab[FP32 ? 1 : kErfcxTab];
 };
 
-template <bool FP32> __device__ __forceinline__ void load_erfcx_table(RsShared<FP32> &sh) {
+template <bool FP32, bool VJP>
+__device__ __forceinline__ void load_erfcx_table(RsShared<FP32, VJP> &sh) {
     if (!FP32) {
         for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
             sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
@@ -343,7 +394,8 @@ __device__ __forceinline__ unsigned int pack_pair(int owner, unsigned int shift)
 }
 
 // the 27 image shifts of a box of side L
-template <bool FP32> __device__ __forceinline__ void fill_shifts(RsShared<FP32> &sh, double L) {
+template <bool FP32, bool VJP>
+__device__ __forceinline__ void fill_shifts(RsShared<FP32, VJP> &sh, double L) {
     if (threadIdx.x < 27) {
         const int t = threadIdx.x;
         sh.shift[t] = make_double4((t / 9 - 1) * L, ((t / 3) % 3 - 1) * L, (t % 3 - 1) * L, 0.0);
@@ -462,14 +514,18 @@ __device__ __forceinline__ void scan_chunk_scalar(Append &append, Flush &flush,
 // bounding box lies beyond rcut from the item's box are skipped.  Cells of
 // the item's system are cell0 + (cx nc + cy) nc + cz with items from
 // istart[cell]; forces go to fr[d stride + perm[i]].
-template <bool FP32, bool SMALL_BOX>
+// VJP: the adjoint instead of the force (FP64 only): gsorted holds the
+// cotangent in sorted order, fr receives d pos and dq d q.
+template <bool FP32, bool SMALL_BOX, bool VJP = false>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
                const float4 *__restrict__ sorted32, const float *__restrict__ norm2,
                const int *__restrict__ perm, const int *__restrict__ start,
                const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
-               RsShared<FP32> &sh, double &en_tot) {
+               RsShared<FP32, VJP> &sh, double &en_tot, const double4 *__restrict__ gsorted = nullptr,
+               double *__restrict__ dq = nullptr) {
+    static_assert(!(VJP && FP32), "the adjoint is FP64");
     const int lane = threadIdx.x;
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
     const double side = L / nc;
@@ -488,6 +544,10 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     sh.facc[0][lane] = 0.0;
     sh.facc[1][lane] = 0.0;
     sh.facc[2][lane] = 0.0;
+    if constexpr (VJP) {
+        sh.facc[3][lane] = 0.0;
+        sh.home_g[lane] = active ? gsorted[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
     fill_shifts(sh, L);
     int count = 0; // wave-uniform queue length
     __syncthreads();
@@ -507,16 +567,24 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
             // 32-bit byte offset (the particle count is bounded on the host)
             const double4 pj = *reinterpret_cast<const double4 *>(
                 reinterpret_cast<const char *>(sorted) + (j << 5));
-            PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
-            pair<FP32>(*reinterpret_cast<const double4 *>(lds_home + ho), pj, s3.x, s3.y, s3.z, rcut,
-                       alpha, two_a_spi, sh.tab, pa);
-            atomicAdd(&sh.facc[0][o], pa.f0);
-            atomicAdd(&sh.facc[1][o], pa.f1);
-            atomicAdd(&sh.facc[2][o], pa.f2);
-            if (FP32)
-                en32 += pa.enf;
-            else
-                en_tot += pa.en;
+            if constexpr (VJP) {
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                pair_vjp(*reinterpret_cast<const double4 *>(lds_home + ho), pj, s3.x, s3.y, s3.z,
+                         rcut, alpha, two_a_spi, sh.tab, sh.home_g[o], gsorted[j], acc);
+                for (int d = 0; d < 4; d++)
+                    atomicAdd(&sh.facc[d][o], acc[d]);
+            } else {
+                PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
+                pair<FP32>(*reinterpret_cast<const double4 *>(lds_home + ho), pj, s3.x, s3.y, s3.z,
+                           rcut, alpha, two_a_spi, sh.tab, pa);
+                atomicAdd(&sh.facc[0][o], pa.f0);
+                atomicAdd(&sh.facc[1][o], pa.f1);
+                atomicAdd(&sh.facc[2][o], pa.f2);
+                if (FP32)
+                    en32 += pa.enf;
+                else
+                    en_tot += pa.en;
+            }
         }
         count = base;
         __syncthreads();
@@ -678,6 +746,8 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
         atomicAdd(&fr[p], sh.facc[0][lane]);
         atomicAdd(&fr[stride + p], sh.facc[1][lane]);
         atomicAdd(&fr[2 * stride + p], sh.facc[2][lane]);
+        if constexpr (VJP)
+            atomicAdd(&dq[p], sh.facc[3][lane]);
     }
     __syncthreads();
 }
@@ -719,6 +789,35 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
     }
 }
 
+// Adjoint of the real-space force over the same pair set (FP64), persistent
+// grid as k_realspace: dpos (3 n) += d pos, dq (n) += d q.
+template <bool SMALL_BOX>
+__global__ void __launch_bounds__(kWave)
+k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
+                const float *__restrict__ norm2, const int *__restrict__ perm,
+                const int *__restrict__ start, const int *__restrict__ istart,
+                const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L,
+                double rcut, double alpha, int *__restrict__ counter,
+                const double4 *__restrict__ gsorted, double *__restrict__ dpos,
+                double *__restrict__ dq) {
+    __shared__ RsShared<false, true> sh;
+    const int total = istart[ncell];
+    double en_unused = 0.0;
+    load_erfcx_table(sh);
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        const int home = (int)prefix_find(istart, ncell, item);
+        realspace_item<false, SMALL_BOX, true>(home, item - istart[home], 0, sorted, sorted32, norm2,
+                                               perm, start, istart, bbox, nc, reach, L, rcut, alpha,
+                                               n, dpos, sh, en_unused, gsorted, dq);
+    }
+}
+
 }  // namespace
 
 
@@ -726,7 +825,7 @@ namespace p3m {
 
 void cells_release(CellArrays &a) {
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox};
+                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox, a.gsorted};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -843,11 +942,10 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     return P3MK_OK;
 }
 
-extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, const double *y,
-                              const double *z, const double *q, double rcut, double alpha,
-                              int fp32, double *fr, double *energy, p3mk_queue *qu) {
-    if (n <= 0)
-        return P3MK_OK;
+// sort the particles into the cell list (with g: also the cotangent)
+static int build_cells(p3mk_cells *c, int n, double L, const double *x, const double *y,
+                       const double *z, const double *q, double rcut, const double *g,
+                       p3mk_queue *qu) {
     if (n > c->nmax || L != c->L || rcut != c->rcut)
         return p3mk_fail(P3MK_EINVAL, "cell list was built for another system size, box or cutoff", 0);
     CellArrays &a = c->a;
@@ -859,9 +957,42 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.norm2);
+                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.norm2, g, a.gsorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
-    P3MK_TRY(p3m::cells_end(a, c->ncell, qu));
+    return p3m::cells_end(a, c->ncell, qu);
+}
+
+extern "C" int p3mk_realspace_vjp(p3mk_cells *c, int n, double L, const double *x, const double *y,
+                                  const double *z, const double *q, double rcut, double alpha,
+                                  const double *g, double *dpos, double *dq, p3mk_queue *qu) {
+    if (n <= 0)
+        return P3MK_OK;
+    CellArrays &a = c->a;
+    if (a.gsorted == nullptr)
+        P3MK_TRY(p3mk_alloc((void **)&a.gsorted, sizeof(double4) * ((size_t)c->nmax + kPad)));
+    P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, g, qu));
+    const bool all_cells = (2 * c->reach + 1 > c->nc);
+    const dim3 grid((unsigned)c->grid);
+    hipStream_t s = p3mk_stream(qu);
+    if (all_cells)
+        hipLaunchKernelGGL(k_realspace_vjp<true>, grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,
+                           a.norm2, a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                           rcut, alpha, a.counter, a.gsorted, dpos, dq);
+    else
+        hipLaunchKernelGGL(k_realspace_vjp<false>, grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,
+                           a.norm2, a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                           rcut, alpha, a.counter, a.gsorted, dpos, dq);
+    return p3mk_launch_check("real-space adjoint");
+}
+
+extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, const double *y,
+                              const double *z, const double *q, double rcut, double alpha,
+                              int fp32, double *fr, double *energy, p3mk_queue *qu) {
+    if (n <= 0)
+        return P3MK_OK;
+    CellArrays &a = c->a;
+    hipStream_t s = p3mk_stream(qu);
+    P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
     const dim3 grid((unsigned)c->grid);
 #define LAUNCH(F32, SB)                                                                            \

@@ -179,6 +179,11 @@ void p3mk_cells_destroy(p3mk_cells *c);
 int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, const double *y,
                    const double *z, const double *q, double rcut, double alpha, int fp32,
                    double *fr, double *energy, p3mk_queue *qu);
+/* adjoint over the same pairs (FP64): for the cotangent g (3n) of the
+ * real-space force, dpos (3n) += d/d pos and dq (n) += d/d q of sum g . f_r */
+int p3mk_realspace_vjp(p3mk_cells *c, int n, double L, const double *x, const double *y,
+                       const double *z, const double *q, double rcut, double alpha,
+                       const double *g, double *dpos, double *dq, p3mk_queue *qu);
 
 /* ---- batch of independent systems ------------------------------------ */
 typedef struct {

@@ -189,3 +189,124 @@ def test_vjp_against_ewald(pkg, orc):
     err = rel_err(dpos, ref)
     print("P3M adjoint vs Ewald derivative: %.2e" % err)
     assert err < 1e-4
+
+
+# ---- real-space adjoint (p3m_hip_realteil_vjp*) ----------------------------
+
+def pair_distances(s):
+    P = np.stack([s["x"], s["y"], s["z"]], 1)
+    d = P[:, None, :] - P[None, :, :]
+    d -= np.round(d / s["L"]) * s["L"]
+    r = np.sqrt((d ** 2).sum(-1))
+    return r[np.triu_indices(len(P), 1)]
+
+
+def away_from_cutoff(n, L, rcut, seed, gap):
+    """A uniform system none of whose pair distances is within `gap` of rcut
+    (the real-space force jumps there, so finite differences must not cross
+    it)."""
+    for k in range(100):
+        s = interior_system(n, L, seed + 17 * k, 1e-3)
+        if np.min(np.abs(pair_distances(s) - rcut)) > gap:
+            return s
+    raise RuntimeError("no system away from the cutoff")
+
+
+def fd_realspace(orc, s, rcut, alpha, gvec, idx, h=1e-6, hq=1e-3):
+    """Central differences of J = sum g . f_r (oracle Realteil, O(N^2))."""
+    J = lambda t: np.sum(gvec * orc.realteil(t, rcut, alpha)[0])
+    dpos, dq = np.zeros((3, len(idx))), np.zeros(len(idx))
+    for c, j in enumerate(idx):
+        for d, k in enumerate("xyz"):
+            sp, sm = dict(s), dict(s)
+            sp[k] = np.array(s[k], copy=True)
+            sm[k] = np.array(s[k], copy=True)
+            sp[k][j] += h
+            sm[k][j] -= h
+            dpos[d, c] = (J(sp) - J(sm)) / (2 * h)
+        sp, sm = dict(s), dict(s)
+        sp["q"] = np.array(s["q"], copy=True)
+        sm["q"] = np.array(s["q"], copy=True)
+        sp["q"][j] += hq
+        sm["q"][j] -= hq
+        dq[c] = (J(sp) - J(sm)) / (2 * hq)
+    return dpos, dq
+
+
+@pytest.mark.parametrize("n,L,rcut,alpha", [(300, 8.0, 3.0, 1.0), (200, 5.0, 2.4, 1.4)])
+def test_realteil_vjp_matches_finite_differences(pkg, orc, n, L, rcut, alpha):
+    """The real-space adjoint against central differences of the oracle's
+    Realteil, for a random cotangent, on particles sampled over the box (the
+    second system has 2 cells per side: the all-cells path)."""
+    s = away_from_cutoff(n, L, rcut, 41, 1e-4)
+    rng = np.random.default_rng(5)
+    gvec = rng.normal(size=(3, n))
+    dpos, dq = pkg.realteil_vjp(pkg.HostSystem.from_dict(s), rcut, alpha, gvec)
+    idx = rng.choice(n, 10, replace=False)
+    fd_pos, fd_q = fd_realspace(orc, s, rcut, alpha, gvec, idx)
+    print("real-space adjoint vs FD: pos %.2e q %.2e" % (rel_err(dpos[:, idx], fd_pos),
+                                                        rel_err(dq[idx], fd_q)))
+    assert rel_err(dpos[:, idx], fd_pos) < 1e-7
+    assert rel_err(dq[idx], fd_q) < 1e-9
+
+
+def test_realteil_vjp_device_form_accumulates(pkg):
+    """The device entry point equals the host one and adds to its outputs."""
+    import torch
+
+    n, L, rcut, alpha = 500, 9.0, 3.0, 1.1
+    s = random_system(n, L, seed=44, layout="gaussian")
+    g = np.random.default_rng(6).normal(size=(3, n))
+    want_pos, want_q = pkg.realteil_vjp(pkg.HostSystem.from_dict(s), rcut, alpha, g)
+    solver = pkg.Solver(0, pkg.HostSystem.from_dict(s), pkg.parameters(16, 5, alpha))
+    dev = {k: torch.tensor(np.asarray(s[k]), dtype=torch.float64, device="cuda") for k in "xyzq"}
+    gd = torch.tensor(g.reshape(-1), dtype=torch.float64, device="cuda")
+    dpos = torch.ones(3 * n, dtype=torch.float64, device="cuda")
+    dq = torch.ones(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    solver.realteil_vjp_device(n, L, *[dev[k].data_ptr() for k in "xyzq"], rcut, alpha,
+                               gd.data_ptr(), dpos.data_ptr(), dq.data_ptr())
+    solver.synchronize()
+    assert rel_err(dpos.cpu().numpy().reshape(3, n) - 1.0, want_pos) < 1e-12
+    assert rel_err(dq.cpu().numpy() - 1.0, want_q) < 1e-12
+    solver.close()
+
+
+def test_total_force_autograd(pkg, orc):
+    """loss = sum |F - F_target|^2 over total forces (k space + real space)
+    through torch.autograd, against central differences of the oracle's total
+    force; positions at bin centres (k space) and away from the cutoff (real
+    space)."""
+    import importlib
+
+    import torch
+
+    ag = importlib.import_module("p3m_standalone_amd.autograd")
+    M, cao, alpha, L, n, rcut = 16, 5, 1.2, 8.0, 12, 3.0
+    for k in range(200):
+        s = snap_to_bins(interior_system(n, L, 60 + k, 1e-3), M)
+        if np.min(np.abs(pair_distances(s) - rcut)) > 1e-3:
+            break
+    rng = np.random.default_rng(9)
+    target = rng.normal(size=(3, n))
+    solver = pkg.Solver(0, pkg.HostSystem.from_dict(s), pkg.parameters(M, cao, alpha))
+    pos = torch.tensor(np.stack([s["x"], s["y"], s["z"]]), dtype=torch.float64, device="cuda",
+                       requires_grad=True)
+    q = torch.tensor(np.asarray(s["q"]), dtype=torch.float64, device="cuda", requires_grad=True)
+    F = ag.total_force(pos, q, solver, rcut, alpha)
+    G = orc.influence(0, M, cao, L, alpha)
+    total = lambda t: oracle_force(orc, 0, t, M, cao, alpha, G, None) + orc.realteil(t, rcut, alpha)[0]
+    Fo = total(s)
+    assert rel_err(F.detach().cpu().numpy(), Fo) < 1e-12
+    loss = ((F - torch.tensor(target, device="cuda")) ** 2).sum()
+    loss.backward()
+    gvec = 2.0 * (Fo - target)
+    # k-space part by whole quantisation bins, real-space part with a small step
+    fdk_pos, fdk_q = fd_gradient(orc, 0, s, M, cao, alpha, gvec)
+    fdr_pos, fdr_q = fd_realspace(orc, s, rcut, alpha, gvec, np.arange(n))
+    fd_pos, fd_q = fdk_pos + fdr_pos, fdk_q + fdr_q
+    print("total-force adjoint vs FD: pos %.2e q %.2e" % (rel_err(pos.grad.cpu().numpy(), fd_pos),
+                                                         rel_err(q.grad.cpu().numpy(), fd_q)))
+    assert rel_err(pos.grad.cpu().numpy(), fd_pos) < 2e-7
+    assert rel_err(q.grad.cpu().numpy(), fd_q) < 1e-9
+    solver.close()
