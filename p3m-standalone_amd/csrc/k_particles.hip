@@ -119,6 +119,25 @@ k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ p
                                           fk, f, n);
 }
 
+template <int P, typename T, bool R2C, bool IL>
+__global__ void __launch_bounds__(kThreads)
+k_interp_ad_tile(const double4 *__restrict__ sorted, const int *__restrict__ perm,
+                 const int *__restrict__ tstart, const int *__restrict__ cstart, long ntiles, int nt,
+                 int Tt, int M, double Hi, double Leni, const T *__restrict__ phi, double prefac,
+                 const double *__restrict__ selfb, double h, double *__restrict__ fk,
+                 double *__restrict__ f, long stride) {
+    extern __shared__ unsigned char smem_raw[];
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
+        return;
+    const long tile = item_tile(cstart, ntiles, item);
+    const int chunk = item - cstart[tile];
+    const int pb = tstart[tile] + chunk * kChunk;
+    const int pe = min(tstart[tile + 1], pb + kChunk);
+    interp_ad_item<P, T, R2C, IL>(sorted, perm, pb, pe, tile, nt, Tt, M, Hi, Leni, phi, prefac,
+                                  selfb, h, fk, f, stride, reinterpret_cast<T *>(smem_raw));
+}
+
 // ---- adjoint (vector-Jacobian product) of the k-space force ---------------
 //
 // ik: f_{i,d} = -s q_i sum_m W_i(m) phi_d(m), phi_d = K_d Q, Q = sum_j q_j W_j,
@@ -413,21 +432,27 @@ int launch_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void 
                      double prefac, const double *selfb, double *fk, double *f, p3mk_queue *qu) {
     const double Hi = (double)g->M / g->L;
     const double h = g->L / g->M;
-    dim3 grid((n + kThreads - 1) / kThreads);
     const T *ph = (const T *)phi;
     hipStream_t st = p3mk_stream(qu);
     const bool il = g->interlaced != 0;
+    // the item's halo region of phi in LDS (one plane per mesh)
+    const int NM = il ? 2 : 1;
+    const int R = s->T + g->cao - 1 + (il ? 1 : 0);
+    const size_t lds = sizeof(T) * NM * (size_t)R * R * R;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
+    const dim3 grid((unsigned)((long)(n + kChunk - 1) / kChunk + s->ntiles));
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n,    \
-                               s->sorted, s->perm, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, \
-                               f);                                                                 \
+            hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
+                               s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T,    \
+                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
         else                                                                                       \
-            hipLaunchKernelGGL((k_interp_ad<P, T, R2C, false>), grid, dim3(kThreads), 0, st, n,   \
-                               s->sorted, s->perm, Hi, 1.0 / g->L, g->M, ph, prefac, selfb, h, fk, \
-                               f);                                                                 \
+            hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
+                               st, s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T, \
+                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE

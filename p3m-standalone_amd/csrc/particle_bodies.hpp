@@ -268,5 +268,106 @@ interp_ad_particle(int t, int n, const double4 *__restrict__ sorted, const int *
     }
 }
 
+// ---- tile-staged ad back-interpolation ------------------------------------------
+//
+// A work item is a chunk of the particles of one mesh tile (as in assignment).
+// The item's halo region of phi -- (Tt + P - 1 + IL)^3 points per mesh from
+// the tile origin, periodic wrap -- is copied to LDS once (plane ii holds mesh
+// ii), and every particle of the item gathers its P^3 stencil from LDS.  The
+// sums factorise over the axes:
+//   F_x = sum_a d0[a] sum_b w1[b] S1(a,b),  F_y = sum_a w0[a] sum_b d1[b] S1(a,b),
+//   F_z = sum_a w0[a] sum_b w1[b] S2(a,b),
+//   S1 = sum_c w2[c] phi(a,b,c),  S2 = sum_c d2[c] phi(a,b,c),
+// two multiply-adds per stencil point; the sums over a run in FP64.  The force
+// is that of interp_ad_particle: f_k -= prefac q / L sum phi W^(d) (x 0.5 per
+// mesh when interlaced), and the self-force subtraction for ad / ad-r.
+template <int P, typename T, bool R2C, bool IL>
+__device__ __forceinline__ void
+interp_ad_item(const double4 *__restrict__ sorted, const int *__restrict__ perm, int pb, int pe,
+               long tl, int nt, int Tt, int M, double Hi, double Leni, const T *__restrict__ phi,
+               double prefac, const double *__restrict__ selfb, double h, double *__restrict__ fk,
+               double *__restrict__ f, long stride, T *__restrict__ lds) {
+    constexpr int NM = IL ? 2 : 1;
+    const int R = Tt + P - 1 + (IL ? 1 : 0);
+    const int npts = R * R * R;
+    const int tz = (int)(tl % nt), ty = (int)((tl / nt) % nt), tx = (int)(tl / ((long)nt * nt));
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+    for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+        const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
+        const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
+        const long g = ((long)gx * M + gy) * M + gz;
+        if (R2C) {
+            lds[k] = phi[g];
+        } else {
+            lds[k] = phi[2 * g];
+            if (IL)
+                lds[npts + k] = phi[2 * g + 1];
+        }
+    }
+    __syncthreads();
+    for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+        const double4 r = sorted[p];
+        double F[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ii = 0; ii < NM; ii++) {
+            int b0, b1, b2;
+            T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
+            const double half = 0.5 * ii;
+            axis_weights_d<P, T>(r.x, Hi, half, M, b0, w0, d0);
+            axis_weights_d<P, T>(r.y, Hi, half, M, b1, w1, d1);
+            axis_weights_d<P, T>(r.z, Hi, half, M, b2, w2, d2);
+            int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+            o0 += (o0 < 0) ? M : 0;
+            o1 += (o1 < 0) ? M : 0;
+            o2 += (o2 < 0) ? M : 0;
+            const T *pl = lds + ii * npts;
+            for (int a = 0; a < P; a++) {
+                T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+                for (int b = 0; b < P; b++) {
+                    const T *row = pl + ((o0 + a) * R + (o1 + b)) * R + o2;
+                    T s1 = T(0), s2 = T(0);
+#pragma unroll
+                    for (int c = 0; c < P; c++) {
+                        const T v = row[c];
+                        s1 = fma(w2[c], v, s1);
+                        s2 = fma(d2[c], v, s2);
+                    }
+                    sx = fma(w1[b], s1, sx);
+                    sy = fma(d1[b], s1, sy);
+                    sz = fma(w1[b], s2, sz);
+                }
+                F[0] += (double)d0[a] * (double)sx;
+                F[1] += (double)w0[a] * (double)sy;
+                F[2] += (double)w0[a] * (double)sz;
+            }
+        }
+        const double scale = (IL ? 0.5 * prefac : prefac) * (r.w * Leni);
+        const int i = perm[p];
+        fk[i] -= scale * F[0];
+        fk[stride + i] -= scale * F[1];
+        fk[2 * stride + i] -= scale * F[2];
+        if (selfb != nullptr) {
+            // f -= sum_m' b(m') q^2 sin(2 pi m'.r / h)  (p3m-ad-self-forces.c:122-135)
+            const double q2 = r.w * r.w;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            int ind = 0;
+            for (int m0 = -1; m0 <= 1; m0++)
+                for (int m1 = -1; m1 <= 1; m1++)
+                    for (int m2 = -1; m2 <= 1; m2++) {
+                        const double st = q2 * sin(2 * 3.14159265358979323846 *
+                                                   (m0 * r.x / h + m1 * r.y / h + m2 * r.z / h));
+                        s0 += selfb[ind] * st;
+                        s1 += selfb[ind + 1] * st;
+                        s2 += selfb[ind + 2] * st;
+                        ind += 3;
+                    }
+            f[i] -= s0;
+            f[stride + i] -= s1;
+            f[2 * stride + i] -= s2;
+        }
+    }
+}
+
 }  // namespace part
 }  // namespace p3m
