@@ -64,8 +64,10 @@ int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, in
 int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
                  long max_items, size_t lds, const int *start, const int *cstart,
                  const double4 *sorted, p3mk_queue *qu);
-int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int ntot,
-                 const double4 *sorted, const int *perm, double *fk, double *ftot, p3mk_queue *qu);
+int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
+                 long max_items, size_t lds, const int *start, const int *cstart,
+                 const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
+                 p3mk_queue *qu);
 
 // real space: cell-list arrays (octant-sorted cells, work items of 64
 // particles with bounding boxes) and one persistent launch over all items

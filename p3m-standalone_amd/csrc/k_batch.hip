@@ -428,7 +428,8 @@ extern "C" int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y
         else
             P3MK_TRY(bfft_pass(f, b->dsys, b->blk_z, ns, b->nblk_z, b->lds_z, kPassZ, kBufMesh, 1, qu));
     }
-    return bpart_interp(f, b->dsys, b->p0s, ns, b->ntot, b->sorted, b->perm, fk, ftot, qu);
+    return bpart_interp(f, b->dsys, b->tile0s, ns, b->ntiles, b->max_items, b->lds_assign, b->start,
+                        b->cstart, b->sorted, b->perm, b->ntot, fk, ftot, qu);
 }
 
 // real-space force of every system with rcut > 0: fr += (3 blocks of ntot);

@@ -99,24 +99,20 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
 
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
-k_interp_ik(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
-            int M, const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
-            double prefac, double *__restrict__ fk) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n)
-        interp_ik_particle<P, T, R2C, IL>(t, n, sorted, perm, Hi, M, F0, F1, F2, prefac, fk, n);
-}
-
-template <int P, typename T, bool R2C, bool IL>
-__global__ void __launch_bounds__(kThreads)
-k_interp_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
-            double Leni, int M, const T *__restrict__ phi, double prefac,
-            const double *__restrict__ selfb, double h, double *__restrict__ fk,
-            double *__restrict__ f) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n)
-        interp_ad_particle<P, T, R2C, IL>(t, n, sorted, perm, Hi, Leni, M, phi, prefac, selfb, h,
-                                          fk, f, n);
+k_interp_ik_tile(const double4 *__restrict__ sorted, const int *__restrict__ perm,
+                 const int *__restrict__ tstart, const int *__restrict__ cstart, long ntiles, int nt,
+                 int Tt, int M, double Hi, const T *__restrict__ F0, const T *__restrict__ F1,
+                 const T *__restrict__ F2, double prefac, double *__restrict__ fk, long stride) {
+    extern __shared__ unsigned char smem_raw[];
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
+        return;
+    const long tile = item_tile(cstart, ntiles, item);
+    const int chunk = item - cstart[tile];
+    const int pb = tstart[tile] + chunk * kChunk;
+    const int pe = min(tstart[tile + 1], pb + kChunk);
+    interp_ik_item<P, T, R2C, IL>(sorted, perm, pb, pe, tile, nt, Tt, M, Hi, F0, F1, F2, prefac, fk,
+                                  stride, reinterpret_cast<T *>(smem_raw));
 }
 
 template <int P, typename T, bool R2C, bool IL>
@@ -405,19 +401,27 @@ template <typename T, bool R2C>
 int launch_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const F[3],
                      double prefac, double *fk, p3mk_queue *qu) {
     const double Hi = (double)g->M / g->L;
-    dim3 grid((n + kThreads - 1) / kThreads);
     const T *F0 = (const T *)F[0], *F1 = (const T *)F[1], *F2 = (const T *)F[2];
     hipStream_t st = p3mk_stream(qu);
     const bool il = g->interlaced != 0;
+    // one field component's halo region in LDS at a time (one plane per mesh)
+    const int NM = il ? 2 : 1;
+    const int R = s->T + g->cao - 1 + (il ? 1 : 0);
+    const size_t lds = sizeof(T) * NM * (size_t)R * R * R;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
+    const dim3 grid((unsigned)((long)(n + kChunk - 1) / kChunk + s->ntiles));
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, true>), grid, dim3(kThreads), 0, st, n,    \
-                               s->sorted, s->perm, Hi, g->M, F0, F1, F2, prefac, fk);            \
+            hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
+                               s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T,    \
+                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
         else                                                                                       \
-            hipLaunchKernelGGL((k_interp_ik<P, T, R2C, false>), grid, dim3(kThreads), 0, st, n,   \
-                               s->sorted, s->perm, Hi, g->M, F0, F1, F2, prefac, fk);            \
+            hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
+                               st, s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T, \
+                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -663,22 +667,32 @@ k_bassign(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsy
     }
 }
 
+// back-interpolation of every system: work items as in assignment (a chunk
+// of one tile's particles), the tile's halo region staged in LDS
 template <typename T, bool R2C, bool IL, bool AD>
 __global__ void __launch_bounds__(kThreads)
-k_binterp(const BSys *__restrict__ sys, const int *__restrict__ p0s, int nsys, int ntot,
-          const double4 *__restrict__ sorted, const int *__restrict__ perm, double *__restrict__ fk,
-          double *__restrict__ ftot) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntot)
+k_binterp(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsys, long ntiles,
+          const int *__restrict__ start, const int *__restrict__ cstart,
+          const double4 *__restrict__ sorted, const int *__restrict__ perm, int ntot,
+          double *__restrict__ fk, double *__restrict__ ftot) {
+    extern __shared__ unsigned char smem_raw[];
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
         return;
-    const BSys &S = sys[fft::prefix_slot(p0s, nsys, t)];
+    const long tile = item_tile(cstart, ntiles, item);
+    const BSys &S = sys[fft::prefix_slot(tile0s, nsys, tile)];
+    const int chunk = item - cstart[tile];
+    const int pb = start[tile] + chunk * kChunk;
+    const int pe = min(start[tile + 1], pb + kChunk);
+    const long tl = tile - S.tile0;
+    T *lds = reinterpret_cast<T *>(smem_raw);
     if (AD) {
         const T *phi = (const T *)S.buf[kBufMesh];
         switch (S.cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
-        interp_ad_particle<P, T, R2C, IL>(t, ntot, sorted, perm, S.Hi, 1.0 / S.L, S.M, phi,         \
-                                          S.prefac, S.selfb, S.h, fk, ftot, ntot);                \
+        interp_ad_item<P, T, R2C, IL>(sorted, perm, pb, pe, tl, S.nt, S.T, S.M, S.Hi, 1.0 / S.L,    \
+                                      phi, S.prefac, S.selfb, S.h, fk, ftot, ntot, lds);          \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -690,8 +704,8 @@ k_binterp(const BSys *__restrict__ sys, const int *__restrict__ p0s, int nsys, i
         switch (S.cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
-        interp_ik_particle<P, T, R2C, IL>(t, ntot, sorted, perm, S.Hi, S.M, F0, F1, F2, S.prefac,   \
-                                          fk, ntot);                                              \
+        interp_ik_item<P, T, R2C, IL>(sorted, perm, pb, pe, tl, S.nt, S.T, S.M, S.Hi, F0, F1, F2,   \
+                                      S.prefac, fk, ntot, lds);                                   \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -709,16 +723,17 @@ void launch_bassign(const BSys *sys, const long *tile0s, int nsys, long ntiles, 
 }
 
 template <typename T, bool R2C, bool IL>
-void launch_binterp(bool ad, const BSys *sys, const int *p0s, int nsys, int ntot,
-                    const double4 *sorted, const int *perm, double *fk, double *ftot,
+void launch_binterp(bool ad, const BSys *sys, const long *tile0s, int nsys, long ntiles,
+                    long max_items, size_t lds, const int *start, const int *cstart,
+                    const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
                     hipStream_t st) {
-    const dim3 grid((ntot + kThreads - 1) / kThreads);
+    const dim3 grid((unsigned)max_items);
     if (ad)
-        hipLaunchKernelGGL((k_binterp<T, R2C, IL, true>), grid, dim3(kThreads), 0, st, sys, p0s,
-                           nsys, ntot, sorted, perm, fk, ftot);
+        hipLaunchKernelGGL((k_binterp<T, R2C, IL, true>), grid, dim3(kThreads), lds, st, sys, tile0s,
+                           nsys, ntiles, start, cstart, sorted, perm, ntot, fk, ftot);
     else
-        hipLaunchKernelGGL((k_binterp<T, R2C, IL, false>), grid, dim3(kThreads), 0, st, sys, p0s,
-                           nsys, ntot, sorted, perm, fk, ftot);
+        hipLaunchKernelGGL((k_binterp<T, R2C, IL, false>), grid, dim3(kThreads), lds, st, sys,
+                           tile0s, nsys, ntiles, start, cstart, sorted, perm, ntot, fk, ftot);
 }
 
 }  // namespace
@@ -769,13 +784,17 @@ int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, 
     return p3mk_launch_check("batched assignment");
 }
 
-int bpart_interp(const BForm &f, const BSys *sys, const int *p0s, int nsys, int ntot,
-                 const double4 *sorted, const int *perm, double *fk, double *ftot, p3mk_queue *qu) {
-    if (ntot <= 0)
+int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
+                 long max_items, size_t lds, const int *start, const int *cstart,
+                 const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
+                 p3mk_queue *qu) {
+    if (ntot <= 0 || max_items <= 0)
         return P3MK_OK;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
     hipStream_t st = p3mk_stream(qu);
     const bool ad = f.ad != 0;
-#define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, p0s, nsys, ntot, sorted, perm, fk, ftot, st)
+#define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, tile0s, nsys, ntiles, max_items, lds, start, cstart, sorted, perm, ntot, fk, ftot, st)
     if (f.fp32) {
         if (f.r2c) { if (f.interlaced) GO(float, true, true); else GO(float, true, false); }
         else { if (f.interlaced) GO(float, false, true); else GO(float, false, false); }

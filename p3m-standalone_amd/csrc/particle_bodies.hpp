@@ -157,114 +157,72 @@ template <> struct Acc<float> {
     __device__ __forceinline__ double value() const { return s.value(); }
 };
 
+// ---- tile-staged ik back-interpolation ------------------------------------------
+//
+// As interp_ad_item, one field component at a time: F_d's halo region (one
+// plane per mesh) is staged in LDS and every particle of the item adds
+// f_k,d -= s q sum_a w0[a] sum_b w1[b] sum_c w2[c] F_d(a,b,c) (s = prefac,
+// x 0.5 per mesh when interlaced), the sums over a in FP64.
 template <int P, typename T, bool R2C, bool IL>
 __device__ __forceinline__ void
-interp_ik_particle(int t, int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
-            int M, const T *__restrict__ F0, const T *__restrict__ F1, const T *__restrict__ F2,
-            double prefac, double *__restrict__ fk, long stride) {
+interp_ik_item(const double4 *__restrict__ sorted, const int *__restrict__ perm, int pb, int pe,
+               long tl, int nt, int Tt, int M, double Hi, const T *__restrict__ F0,
+               const T *__restrict__ F1, const T *__restrict__ F2, double prefac,
+               double *__restrict__ fk, long stride, T *__restrict__ lds) {
     constexpr int NM = IL ? 2 : 1;
-    Acc<T> acc[3];
-    const double4 r = sorted[t];
-    // interlaced: each mesh contributes with 0.5 (charge-assign.c:674-675)
-    const T scale = (T)(IL ? 0.5 * prefac : prefac) * (T)r.w;
-    for (int ii = 0; ii < NM; ii++) {
-        int b0, b1, b2;
-        T w0[P], w1[P], w2[P];
-        const double half = 0.5 * ii;
-        axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
-        axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
-        axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
-        int kk[P];
-#pragma unroll
-        for (int c = 0; c < P; c++)
-            kk[c] = wrap1(b2 + c, M);
-        for (int a = 0; a < P; a++) {
-            const int ia = wrap1(b0 + a, M);
-#pragma unroll
-            for (int b = 0; b < P; b++) {
-                const int jb = wrap1(b1 + b, M);
-                const T t1 = w0[a] * w1[b];
-                const long row = ((long)ia * M + jb) * M;
-#pragma unroll
-                for (int c = 0; c < P; c++) {
-                    const long idx = row + kk[c];
-                    const T B = scale * (t1 * w2[c]);
-                    acc[0].add(-mesh_val<T, R2C>(F0, idx, ii) * B);
-                    acc[1].add(-mesh_val<T, R2C>(F1, idx, ii) * B);
-                    acc[2].add(-mesh_val<T, R2C>(F2, idx, ii) * B);
-                }
+    const int R = Tt + P - 1 + (IL ? 1 : 0);
+    const int npts = R * R * R;
+    const int tz = (int)(tl % nt), ty = (int)((tl / nt) % nt), tx = (int)(tl / ((long)nt * nt));
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+    const double sc = IL ? 0.5 * prefac : prefac;
+    for (int d = 0; d < 3; d++) {
+        const T *F = d == 0 ? F0 : (d == 1 ? F1 : F2);
+        for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+            const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
+            const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
+            const long g = ((long)gx * M + gy) * M + gz;
+            if (R2C) {
+                lds[k] = F[g];
+            } else {
+                lds[k] = F[2 * g];
+                if (IL)
+                    lds[npts + k] = F[2 * g + 1];
             }
         }
-    }
-    const int i = perm[t];
-    fk[i] += acc[0].value();
-    fk[stride + i] += acc[1].value();
-    fk[2 * stride + i] += acc[2].value();
-}
-
-template <int P, typename T, bool R2C, bool IL>
-__device__ __forceinline__ void
-interp_ad_particle(int t, int n, const double4 *__restrict__ sorted, const int *__restrict__ perm, double Hi,
-            double Leni, int M, const T *__restrict__ phi, double prefac,
-            const double *__restrict__ selfb, double h, double *__restrict__ fk,
-            double *__restrict__ f, long stride) {
-    constexpr int NM = IL ? 2 : 1;
-    Acc<T> acc[3];
-    const double4 r = sorted[t];
-    const double xi = r.x, yi = r.y, zi = r.z, qi = r.w;
-    // dQ_d = W'.W.W q/L; force_d -= prefac * phi * dQ_d, interlaced x 0.5
-    // (charge-assign.c:1123-1125, 1374-1377, 758-760)
-    const T scale = (T)((IL ? 0.5 * prefac : prefac) * (qi * Leni));
-    for (int ii = 0; ii < NM; ii++) {
-        int b0, b1, b2;
-        T w0[P], w1[P], w2[P], d0[P], d1[P], d2[P];
-        const double half = 0.5 * ii;
-        axis_weights_d<P, T>(xi, Hi, half, M, b0, w0, d0);
-        axis_weights_d<P, T>(yi, Hi, half, M, b1, w1, d1);
-        axis_weights_d<P, T>(zi, Hi, half, M, b2, w2, d2);
-        int kk[P];
+        __syncthreads();
+        for (int p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+            const double4 r = sorted[p];
+            double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < P; c++)
-            kk[c] = wrap1(b2 + c, M);
-        for (int a = 0; a < P; a++) {
-            const int ia = wrap1(b0 + a, M);
+            for (int ii = 0; ii < NM; ii++) {
+                int b0, b1, b2;
+                T w0[P], w1[P], w2[P];
+                const double half = 0.5 * ii;
+                axis_weights<P, T>(r.x, Hi, half, M, b0, w0);
+                axis_weights<P, T>(r.y, Hi, half, M, b1, w1);
+                axis_weights<P, T>(r.z, Hi, half, M, b2, w2);
+                int o0 = b0 - t0x, o1 = b1 - t0y, o2 = b2 - t0z;
+                o0 += (o0 < 0) ? M : 0;
+                o1 += (o1 < 0) ? M : 0;
+                o2 += (o2 < 0) ? M : 0;
+                const T *pl = lds + ii * npts;
+                for (int a = 0; a < P; a++) {
+                    T sa = T(0);
 #pragma unroll
-            for (int b = 0; b < P; b++) {
-                const int jb = wrap1(b1 + b, M);
-                const long row = ((long)ia * M + jb) * M;
-                const T wx = d0[a] * w1[b], wy = w0[a] * d1[b], wz = w0[a] * w1[b];
+                    for (int b = 0; b < P; b++) {
+                        const T *row = pl + ((o0 + a) * R + (o1 + b)) * R + o2;
+                        T sb = T(0);
 #pragma unroll
-                for (int c = 0; c < P; c++) {
-                    const T B = scale * mesh_val<T, R2C>(phi, row + kk[c], ii);
-                    acc[0].add(-B * (wx * w2[c]));
-                    acc[1].add(-B * (wy * w2[c]));
-                    acc[2].add(-B * (wz * d2[c]));
+                        for (int c = 0; c < P; c++)
+                            sb = fma(w2[c], row[c], sb);
+                        sa = fma(w1[b], sb, sa);
+                    }
+                    acc += (double)w0[a] * (double)sa;
                 }
             }
+            fk[d * stride + perm[p]] -= sc * r.w * acc;
         }
-    }
-    const int i = perm[t];
-    fk[i] += acc[0].value();
-    fk[stride + i] += acc[1].value();
-    fk[2 * stride + i] += acc[2].value();
-    if (selfb != nullptr) {
-        // f -= sum_m' b(m') q^2 sin(2 pi m'.r / h)  (p3m-ad-self-forces.c:122-135)
-        const double q2 = qi * qi;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        int ind = 0;
-        for (int m0 = -1; m0 <= 1; m0++)
-            for (int m1 = -1; m1 <= 1; m1++)
-                for (int m2 = -1; m2 <= 1; m2++) {
-                    const double st =
-                        q2 * sin(2 * 3.14159265358979323846 * (m0 * xi / h + m1 * yi / h + m2 * zi / h));
-                    s0 += selfb[ind] * st;
-                    s1 += selfb[ind + 1] * st;
-                    s2 += selfb[ind + 2] * st;
-                    ind += 3;
-                }
-        f[i] -= s0;
-        f[stride + i] -= s1;
-        f[2 * stride + i] -= s2;
+        __syncthreads();
     }
 }
 
@@ -279,8 +237,9 @@ interp_ad_particle(int t, int n, const double4 *__restrict__ sorted, const int *
 //   F_z = sum_a w0[a] sum_b w1[b] S2(a,b),
 //   S1 = sum_c w2[c] phi(a,b,c),  S2 = sum_c d2[c] phi(a,b,c),
 // two multiply-adds per stencil point; the sums over a run in FP64.  The force
-// is that of interp_ad_particle: f_k -= prefac q / L sum phi W^(d) (x 0.5 per
-// mesh when interlaced), and the self-force subtraction for ad / ad-r.
+// is f_k -= prefac q / L sum phi W^(d) (x 0.5 per mesh when interlaced;
+// charge-assign.c:1352-1428, 727-806), and for ad / ad-r the self force is
+// subtracted from f.
 template <int P, typename T, bool R2C, bool IL>
 __device__ __forceinline__ void
 interp_ad_item(const double4 *__restrict__ sorted, const int *__restrict__ perm, int pb, int pe,
