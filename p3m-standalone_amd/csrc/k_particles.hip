@@ -22,6 +22,8 @@
 // of caching cf / dQ / ca_ind per particle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <new>
 
 #include "batch_desc.hpp"
@@ -69,6 +71,65 @@ __global__ void k_tile_scatter(int n, const double *__restrict__ x, const double
     const int p = start[k] + atomicAdd(&cursor[k], 1);
     perm[p] = i;
     sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+}
+
+// The same sort with per-block aggregation, for tile counts that fit in
+// LDS: a block takes a contiguous range of particles, counts them per tile in
+// LDS and adds each nonzero count to the global counter once; the scatter
+// reserves the block's range of every tile with one global atomic and ranks
+// its particles inside it with LDS atomics.  (Device-scope atomics are the
+// cost of the plain sort: one per particle.)
+constexpr int kSortThreads = 1024;
+constexpr long kSortMaxTiles = 8192; // two int arrays of this size in LDS
+
+__global__ void __launch_bounds__(kSortThreads)
+k_tile_count_agg(int n, int per_block, const double *__restrict__ x, const double *__restrict__ y,
+                 const double *__restrict__ z, double Hi, int P, int M, int T, int nt, int ntiles,
+                 int *__restrict__ key, int *__restrict__ count) {
+    extern __shared__ int hist[];
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x)
+        hist[k] = 0;
+    __syncthreads();
+    const int b = blockIdx.x * per_block, e = min(n, b + per_block);
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const int tx = axis_base(x[i], Hi, P, M) / T, ty = axis_base(y[i], Hi, P, M) / T,
+                  tz = axis_base(z[i], Hi, P, M) / T;
+        const int k = (tx * nt + ty) * nt + tz;
+        key[i] = k;
+        atomicAdd(&hist[k], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x)
+        if (hist[k] != 0)
+            atomicAdd(&count[k], hist[k]);
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_tile_scatter_agg(int n, int per_block, const double *__restrict__ x, const double *__restrict__ y,
+                   const double *__restrict__ z, const double *__restrict__ q, int ntiles,
+                   const int *__restrict__ key, const int *__restrict__ start,
+                   int *__restrict__ cursor, int *__restrict__ perm, double4 *__restrict__ sorted) {
+    extern __shared__ int lds_i[];
+    int *hist = lds_i, *base = lds_i + ntiles;
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x)
+        hist[k] = 0;
+    __syncthreads();
+    const int b = blockIdx.x * per_block, e = min(n, b + per_block);
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x)
+        atomicAdd(&hist[key[i]], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x) {
+        if (hist[k] != 0)
+            base[k] = start[k] + atomicAdd(&cursor[k], hist[k]);
+        hist[k] = 0;
+    }
+    __syncthreads();
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const int k = key[i];
+        const int p = base[k] + atomicAdd(&hist[k], 1);
+        perm[p] = i;
+        sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+    }
 }
 
 // chunks per tile: ceil(count / kChunk)
@@ -544,13 +605,29 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
     const size_t nt1 = (size_t)s->ntiles + 1;
     P3MK_TRY(p3mk_fill_zero(s->count, sizeof(int) * nt1, qu));
     P3MK_TRY(p3mk_fill_zero(s->cursor, sizeof(int) * nt1, qu));
-    hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
-                       s->nt, s->key, s->count);
-    P3MK_TRY(p3mk_launch_check("tile count"));
-    P3MK_TRY(p3mk_exclusive_scan(s->count, s->ntiles, s->start, s->scan_scratch, qu));
-    hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key, s->start,
-                       s->cursor, s->perm, s->sorted);
-    P3MK_TRY(p3mk_launch_check("tile scatter"));
+    if (s->ntiles <= kSortMaxTiles) {
+        // about 64 blocks: one global atomic per (block, tile) instead of per particle
+        const int per_block = std::max(4096, (n + 63) / 64);
+        const int blocks = (n + per_block - 1) / per_block;
+        const int ntiles = (int)s->ntiles;
+        hipLaunchKernelGGL(k_tile_count_agg, dim3(blocks), dim3(kSortThreads),
+                           sizeof(int) * (size_t)ntiles, st, n, per_block, x, y, z, Hi, g->cao,
+                           g->M, s->T, s->nt, ntiles, s->key, s->count);
+        P3MK_TRY(p3mk_launch_check("tile count"));
+        P3MK_TRY(p3mk_exclusive_scan(s->count, s->ntiles, s->start, s->scan_scratch, qu));
+        hipLaunchKernelGGL(k_tile_scatter_agg, dim3(blocks), dim3(kSortThreads),
+                           2 * sizeof(int) * (size_t)ntiles, st, n, per_block, x, y, z, q, ntiles,
+                           s->key, s->start, s->cursor, s->perm, s->sorted);
+        P3MK_TRY(p3mk_launch_check("tile scatter"));
+    } else {
+        hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M,
+                           s->T, s->nt, s->key, s->count);
+        P3MK_TRY(p3mk_launch_check("tile count"));
+        P3MK_TRY(p3mk_exclusive_scan(s->count, s->ntiles, s->start, s->scan_scratch, qu));
+        hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key,
+                           s->start, s->cursor, s->perm, s->sorted);
+        P3MK_TRY(p3mk_launch_check("tile scatter"));
+    }
     hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
                        s->count, s->ntiles, s->chunks);
     P3MK_TRY(p3mk_launch_check("tile chunks"));
