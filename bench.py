@@ -160,11 +160,14 @@ def ewald_gates(pkg, torch, w, dev, results, nsample=1000, precision=1e-6, rcut_
     return out
 
 
-# Algorithmic flops of one ordered real-space pair within rcut (Realteil,
+# Algorithmic flops of one real-space pair within rcut (Realteil,
 # realpart.c:124-167): displacement and r^2 8, 1/r, r and alpha r 3,
 # exp(-x^2) 11 and erfc(x) 10 (each function counted as a 10-flop polynomial),
-# the bracket q_i q_j (erfc/r + c exp)/r^2 6, the force update 6.
+# the bracket q_i q_j (erfc/r + c exp)/r^2 6, the force update 6.  The
+# kernel evaluates each unordered pair once and adds its reaction to the
+# partner (3 more flops), so its work is (ordered pairs / 2) x 47 flops.
 PAIR_FLOPS = 44
+HALF_PAIR_FLOPS = PAIR_FLOPS + 3
 
 
 def pair_count(w, sysd, samples=20000):
@@ -209,16 +212,16 @@ def roofline(w, res, s, pairs):
                            "flops": int(flops[k])})
     rs_ms = stages["realspace"]
     if rs_ms > 0:
-        tf = pairs * PAIR_FLOPS / (rs_ms * 1e-3) / 1e12
+        fl = 0.5 * pairs * HALF_PAIR_FLOPS
+        tf = fl / (rs_ms * 1e-3) / 1e12
         per["realspace"].update({"pairs": int(pairs), "TFLOP/s": round(tf, 3),
-                                 "frac_valu": round(tf / vpeak, 4),
-                                 "flops": int(pairs * PAIR_FLOPS)})
+                                 "frac_valu": round(tf / vpeak, 4), "flops": int(fl)})
     return dom, per
 
 
 # Kernel symbol of each stage's dominant launch in the mixed form, as the
 # profiler names it.
-STAGE_KERNEL = {"realspace": "k_realspace<true, false>", "interp": "k_interp_ad<7, float, false, true>",
+STAGE_KERNEL = {"realspace": "k_realspace_half<true>", "interp": "k_interp_ad_tile<7, float, false, true>",
                 "assign": "k_assign_halo<7, float, false, true, 0>", "fft_multiply": "k_fft_pass<float>"}
 
 
@@ -247,12 +250,13 @@ def roofline_object(dom, per, workload):
     kernel = STAGE_KERNEL.get(dom, dom)
     traffic = measured_traffic(workload, kernel)
     if dom == "realspace":
-        return {"bound": "mfma", "kernel": kernel + " (cell lists + pair kernel, FP32 pairs)",
+        return {"bound": "mfma", "kernel": kernel + " (cell lists + half-shell pair kernel, FP32 pairs)",
                 "achieved": st["TFLOP/s"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": st["frac_valu"], "traffic": traffic,
-                "note": "vector-ALU compute bound; achieved = %d ordered pairs within rcut x %d "
-                        "flops / median stage time; traffic = HBM bytes per launch from "
-                        "profiles/traffic_%s.md" % (st["pairs"], PAIR_FLOPS, workload)}
+                "note": "compute bound (vector ALUs and LDS atomics); achieved = %d ordered pairs "
+                        "within rcut / 2 (each unordered pair once) x %d flops / median stage time; "
+                        "traffic = HBM bytes per launch from profiles/traffic_%s.md"
+                        % (st["pairs"], HALF_PAIR_FLOPS, workload)}
     return {"bound": "hbm", "kernel": kernel, "achieved": st["GB/s"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": st["frac_hbm"], "traffic": traffic}
 

@@ -86,6 +86,7 @@ struct CellArrays {
     float4 *sorted32 = nullptr;
     float *norm2 = nullptr;
     double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
+    double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n (made on first use)
     float4 *bbox = nullptr;     // 2 per work item
     long max_items = 0;
 };

@@ -818,6 +818,375 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
     }
 }
 
+
+// ---- half shell (Newton's third law) ---------------------------------------
+//
+// Each unordered pair is evaluated once: the item of a home cell visits the
+// neighbour cells of lexicographically positive offset, its own chunk and the
+// later chunks of its cell (in its own chunk only the neighbours after the
+// home particle).  The owner receives f and the neighbour -f.  The reverse
+// pair's difference is -dx exactly, so its r, its inclusion and its force
+// factor are the same, and -f is the force the reference computes for it bit
+// for bit (realpart.c:134-148).  Forces are summed in sorted order
+// (fs[d stride + i]) and added to the caller's array by perm afterwards.
+//
+// Neighbour particles are staged in a window of 64 slots (several chunks, of
+// several cells), in FP32 relative to the HOME cell centre; the reactions on
+// them are summed per slot in LDS.  The candidate search of a dense item
+// pairs the window with its home particles on the matrix cores in 4 x 4
+// blocks (v_mfma_f32_4x4x1f32: lane l = 4 b + col of block b holds A[row =
+// l % 4], B[col = l % 4], and result register r is D[row r][col l % 4]),
+// K = 4 as four chained products of A = (-2 n', 1) and B = (h, |h|^2 -
+// rc2f), C = |n'|^2.  Issue s pairs block b with home group b mod G (G the
+// item's group count rounded up to a power of two) and window group (b + s)
+// mod 16, and each lane queues its results in the rotated order (col + q)
+// mod 4: the entries of one append then have distinct window slots and, for
+// a full item, distinct owners.  LDS atomics of one drained round that hit
+// one address from several lanes serialise: a chunk tiled 16 x 16 puts 16
+// lanes on each neighbour, a plain 4 x 4 order four, and both were measured
+// several times slower in the reaction sums.
+
+constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
+constexpr int kHalfQueue = 2 * kWave;              // 63 pending + one result register's appends
+constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
+
+template <bool FP32> struct RsHalfShared {
+    unsigned int qj[kHalfQueue]; // neighbour sorted index | shift index << 27
+    unsigned int qo[kHalfQueue]; // owner | window slot << 6
+    double4 shift[27];
+    double facc[3][kWave];
+    double facc_nb[3][kWave];    // reactions on the window's particles
+    float4 win_a[kWave];         // (-2 n', 1), n' relative to the home cell centre
+    float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
+    unsigned int win_js[kWave];  // sorted index | shift index << 27
+    double tab[FP32 ? 1 : kErfcxTab];
+};
+
+template <bool FP32>
+__device__ __forceinline__ void
+realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
+                    const float4 *__restrict__ sorted32, const int *__restrict__ start,
+                    const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc,
+                    int reach, double L, double rcut, double alpha, long stride,
+                    double *__restrict__ fs, RsHalfShared<FP32> &sh, double &en_tot) {
+    const int lane = threadIdx.x;
+    const double two_a_spi = 2.0 * alpha / sqrt_pi();
+    const double side = L / nc;
+    const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
+    const float rskip2 = rskip * rskip;
+    const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
+    const int hb = start[home], he = start[home + 1];
+    const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
+    const int i0 = hb + chunk * kWave;
+    const int nh = min(kWave, he - i0);
+    const int i = i0 + lane;
+    const bool active = lane < nh;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        sh.facc[d][lane] = 0.0;
+        sh.facc_nb[d][lane] = 0.0;
+    }
+    sh.win_a[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    sh.win_n2[lane] = 1e30f;
+    sh.win_js[lane] = 0u;
+    if (lane < 27)
+        sh.shift[lane] = make_double4((lane / 9 - 1) * L, ((lane / 3) % 3 - 1) * L, (lane % 3 - 1) * L, 0.0);
+    const bool tiles = he - hb > kMfmaMinCell;
+    // home groups of four rounded up to a power of two (2 ... 16)
+    const int gh = (nh + 3) >> 2;
+    const int G = gh <= 2 ? 2 : (gh <= 4 ? 4 : (gh <= 8 ? 8 : 16));
+    // the home particle of this lane's matrix column (tiles) or of the lane
+    const int hidx = tiles ? 4 * ((lane >> 2) & (G - 1)) + (lane & 3) : lane;
+    const bool hval = hidx < nh;
+    const unsigned int i_own = (unsigned)(i0 + hidx);
+    float4 bop = make_float4(0.0f, 0.0f, 0.0f, 1e30f);
+    float hxf = 2e30f, hyf = 2e30f, hzf = 2e30f; // vector search: far away unless valid
+    if (hval) {
+        const float4 v = sorted32[i_own];
+        hxf = -0.5f * v.x;
+        hyf = -0.5f * v.y;
+        hzf = -0.5f * v.z;
+        bop = make_float4(hxf, hyf, hzf, hxf * hxf + hyf * hyf + hzf * hzf - rc2f);
+    }
+    int count = 0;        // wave-uniform queue length
+    int w = 0;            // filled window slots
+    bool selfwin = false; // the window holds the item's own chunk
+    float en32 = 0.0f;
+    __syncthreads();
+
+    auto drain = [&](int m) {
+        const int base = count - m;
+        if (lane < m) {
+            const unsigned int js = sh.qj[base + lane], oq = sh.qo[base + lane];
+            const int o = (int)(oq & 63u), slot = (int)(oq >> 6);
+            // both positions from global memory (L1): LDS bandwidth bounds this kernel
+            const double4 pj = sorted[js & kIdxMask];
+            const double4 ph = sorted[i0 + o];
+            // the image shift, read only by the lanes whose pair wraps
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const unsigned int sidx = js >> 27;
+            if (sidx != kNoShift) {
+                const double4 s3 = sh.shift[sidx];
+                sx = s3.x;
+                sy = s3.y;
+                sz = s3.z;
+            }
+            PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
+            pair<FP32>(ph, pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+            atomicAdd(&sh.facc[0][o], pa.f0);
+            atomicAdd(&sh.facc[1][o], pa.f1);
+            atomicAdd(&sh.facc[2][o], pa.f2);
+            atomicAdd(&sh.facc_nb[0][slot], -pa.f0);
+            atomicAdd(&sh.facc_nb[1][slot], -pa.f1);
+            atomicAdd(&sh.facc_nb[2][slot], -pa.f2);
+            if (FP32)
+                en32 += pa.enf;
+            else
+                en_tot += 2.0 * pa.en;
+        }
+        count = base;
+        __syncthreads();
+    };
+    auto append = [&](bool ok, unsigned int js, unsigned int oq) {
+        const unsigned long long mask = __ballot(ok);
+        if (mask) {
+            if (ok) {
+                const unsigned int slot = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned int)(mask >> 32),
+                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, (unsigned int)count));
+                sh.qj[slot] = js;
+                sh.qo[slot] = oq;
+            }
+            count += __popcll(mask);
+        }
+    };
+    auto flush = [&]() {
+        if (count >= kWave) {
+            __syncthreads();
+            do
+                drain(kWave);
+            while (count >= kWave);
+        }
+    };
+    // the reference's exclusion in the own chunk: keep j > i only
+    auto keep = [&](unsigned int js, unsigned int iown) {
+        const unsigned int j = js & kIdxMask;
+        return !(j - (unsigned)i0 < (unsigned)nh && j <= iown);
+    };
+    // candidate search of the window against the home particles, then all
+    // queued pairs, then the reactions to fs; the window is emptied
+    auto process_window = [&]() {
+        __syncthreads();
+        if (tiles) {
+            // two issues at a time (G is even), so that their product chains overlap
+            auto issues = [&](auto self_tag) {
+                constexpr bool SELFW = decltype(self_tag)::value;
+                // append q takes result register (col + q) mod 4 of each lane:
+                // the 64 entries of one append then have 64 distinct window
+                // slots and (G = 16) 64 distinct owners
+                const int col = lane & 3;
+                auto emit = [&](const floatx4 &d, const uint4 &js4, int base) {
+                    float v[4] = {d[0], d[1], d[2], d[3]};
+                    unsigned int jr[4] = {js4.x, js4.y, js4.z, js4.w};
+                    if (col & 1) {
+                        const float v0 = v[0];
+                        const unsigned int j0 = jr[0];
+                        v[0] = v[1]; v[1] = v[2]; v[2] = v[3]; v[3] = v0;
+                        jr[0] = jr[1]; jr[1] = jr[2]; jr[2] = jr[3]; jr[3] = j0;
+                    }
+                    if (col & 2) {
+                        float t = v[0]; v[0] = v[2]; v[2] = t;
+                        t = v[1]; v[1] = v[3]; v[3] = t;
+                        unsigned int u = jr[0]; jr[0] = jr[2]; jr[2] = u;
+                        u = jr[1]; jr[1] = jr[3]; jr[3] = u;
+                    }
+                    const unsigned int oq0 = (unsigned)hidx | (unsigned)base << 6;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        bool ok = v[q] <= 0.0f;
+                        if (SELFW)
+                            ok = ok && keep(jr[q], i_own);
+                        append(ok, jr[q], oq0 + ((unsigned)((col + q) & 3) << 6));
+                        flush();
+                    }
+                };
+                for (int s = 0; s < G; s += 2) {
+                    const int base0 = ((lane & ~3) + 4 * s) & 63, base1 = (base0 + 4) & 63;
+                    const float4 a0 = sh.win_a[(lane + 4 * s) & 63];
+                    const float4 a1 = sh.win_a[(lane + 4 * s + 4) & 63];
+                    const float4 c0 = *reinterpret_cast<const float4 *>(&sh.win_n2[base0]);
+                    const float4 c1 = *reinterpret_cast<const float4 *>(&sh.win_n2[base1]);
+                    const uint4 js0 = *reinterpret_cast<const uint4 *>(&sh.win_js[base0]);
+                    const uint4 js1 = *reinterpret_cast<const uint4 *>(&sh.win_js[base1]);
+                    floatx4 d0 = {c0.x, c0.y, c0.z, c0.w}, d1 = {c1.x, c1.y, c1.z, c1.w};
+                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bop.x, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bop.x, d1, 0, 0, 0);
+                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, bop.y, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, bop.y, d1, 0, 0, 0);
+                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.z, bop.z, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.z, bop.z, d1, 0, 0, 0);
+                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.w, bop.w, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.w, bop.w, d1, 0, 0, 0);
+                    emit(d0, js0, base0);
+                    emit(d1, js1, base1);
+                }
+            };
+            if (selfwin)
+                issues(std::true_type{});
+            else
+                issues(std::false_type{});
+        } else {
+            for (int k = 0; k < w; k++) {
+                const float4 a = sh.win_a[k];
+                const unsigned int js = sh.win_js[k];
+                const float dx = fmaf(0.5f, a.x, hxf), dy = fmaf(0.5f, a.y, hyf), dz = fmaf(0.5f, a.z, hzf);
+                bool ok = dx * dx + dy * dy + dz * dz <= rc2f;
+                if (selfwin)
+                    ok = ok && keep(js, i_own);
+                append(ok, js, (unsigned)lane | (unsigned)k << 6);
+                flush();
+            }
+        }
+        __syncthreads();
+        if (count > 0)
+            drain(count);
+        if (lane < w) {
+            const unsigned int j = sh.win_js[lane] & kIdxMask;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = sh.facc_nb[d][lane];
+                if (v != 0.0)
+                    atomicAdd(&fs[d * stride + j], v);
+                sh.facc_nb[d][lane] = 0.0;
+            }
+        }
+        sh.win_n2[lane] = 1e30f;
+        w = 0;
+        selfwin = false;
+        __syncthreads();
+    };
+
+    const int hitem = istart[home] + chunk;
+    const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
+    for (int a = 0; a <= reach; a++) {
+        int cx = hx + a, wx = 0;
+        if (cx >= nc) { cx -= nc; wx = 1; }
+        const float gx = gap(hlo.x, hhi.x, (float)((hx + a) * side), (float)((hx + a + 1) * side));
+        for (int b = a == 0 ? 0 : -reach; b <= reach; b++) {
+            int cy = hy + b, wy = 0;
+            if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
+            const float gy = gap(hlo.y, hhi.y, (float)((hy + b) * side), (float)((hy + b + 1) * side));
+            if (gx * gx + gy * gy > rskip2)
+                continue;
+            for (int c = (a == 0 && b == 0) ? 0 : -reach; c <= reach; c++) {
+                int cz = hz + c, wz = 0;
+                if (cz < 0) { cz += nc; wz = -1; } else if (cz >= nc) { cz -= nc; wz = 1; }
+                const float gz = gap(hlo.z, hhi.z, (float)((hz + c) * side), (float)((hz + c + 1) * side));
+                if (gx * gx + gy * gy + gz * gz > rskip2)
+                    continue;
+                const int cell = (cx * nc + cy) * nc + cz;
+                const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
+                const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+                if (jb == je)
+                    continue;
+                const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
+                const unsigned int sidx = shift_index(wx, wy, wz) << 27;
+                const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
+                // window rows: A' = (-2 (n + o), 1), o the cell centre relative to the home's
+                const float ox2 = (float)(2.0 * a * side), oy2 = (float)(2.0 * b * side),
+                            oz2 = (float)(2.0 * c * side);
+                const bool own = a == 0 && b == 0 && c == 0;
+                for (int cb = own ? i0 : jb; cb < je; cb += kWave) {
+                    const int cnt = min(kWave, je - cb);
+                    const int nitem = item0 + (cb - jb) / kWave;
+                    float4 nlo = bbox[2 * nitem], nhi = bbox[2 * nitem + 1];
+                    nlo.x += fsx; nlo.y += fsy; nlo.z += fsz;
+                    nhi.x += fsx; nhi.y += fsy; nhi.z += fsz;
+                    if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
+                        continue;
+                    if (own && cb == i0)
+                        selfwin = true; // the first chunk staged: w == 0 here
+                    for (int k0 = 0; k0 < cnt;) {
+                        const int t = min(kWave - w, cnt - k0);
+                        if (lane < t) {
+                            const int j = cb + k0 + lane;
+                            const float4 v = sorted32[j];
+                            const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
+                            const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
+                            sh.win_a[w + lane] = make_float4(ax, ay, az, 1.0f);
+                            sh.win_n2[w + lane] = nx * nx + ny * ny + nz * nz;
+                            sh.win_js[w + lane] = (unsigned)j | sidx;
+                        }
+                        w += t;
+                        k0 += t;
+                        if (w == kWave)
+                            process_window();
+                    }
+                }
+            }
+        }
+    }
+    if (w > 0)
+        process_window();
+    if (FP32)
+        en_tot += (double)en32;
+    if (active) {
+        atomicAdd(&fs[i], sh.facc[0][lane]);
+        atomicAdd(&fs[stride + i], sh.facc[1][lane]);
+        atomicAdd(&fs[2 * stride + i], sh.facc[2][lane]);
+    }
+    __syncthreads();
+}
+
+// Persistent grid as k_realspace over the half shell; fs (3 n, sorted order)
+// is zeroed by the caller and added to the forces by k_unsort_add.
+template <bool FP32>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
+k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
+                 const int *__restrict__ start, const int *__restrict__ istart,
+                 const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L,
+                 double rcut, double alpha, int *__restrict__ counter, long stride,
+                 double *__restrict__ fs, double *__restrict__ energy) {
+    __shared__ RsHalfShared<FP32> sh;
+    const int total = istart[ncell];
+    double en_tot = 0.0;
+    if (!FP32) {
+        for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
+            sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
+    }
+    __syncthreads();
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        const int home = (int)prefix_find(istart, ncell, item);
+        realspace_item_half<FP32>(home, item - istart[home], sorted, sorted32, start, istart, bbox, nc,
+                                  reach, L, rcut, alpha, stride, fs, sh, en_tot);
+    }
+    if (energy != nullptr) {
+        for (int off = 32; off > 0; off >>= 1)
+            en_tot += __shfl_down(en_tot, off, kWave);
+        if (threadIdx.x == 0 && en_tot != 0.0)
+            atomicAdd(energy, en_tot);
+    }
+}
+
+// fr[d stride + perm[i]] += fs[d n + i]: sorted-order forces to the caller's
+// order (perm is a permutation, so every thread owns its three entries)
+__global__ void k_unsort_add(int n, const int *__restrict__ perm, const double *__restrict__ fs,
+                             long stride, double *__restrict__ fr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int p = perm[i];
+    fr[p] += fs[i];
+    fr[stride + p] += fs[(long)n + i];
+    fr[2 * stride + p] += fs[2 * (long)n + i];
+}
+
 }  // namespace
 
 
@@ -825,7 +1194,7 @@ namespace p3m {
 
 void cells_release(CellArrays &a) {
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox, a.gsorted};
+                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox, a.gsorted, a.fsorted};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -992,9 +1361,28 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
         return P3MK_OK;
     CellArrays &a = c->a;
     hipStream_t s = p3mk_stream(qu);
-    P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
     const bool all_cells = (2 * c->reach + 1 > c->nc);
     const dim3 grid((unsigned)c->grid);
+    if (!all_cells) {
+        // half shell: forces summed in sorted order, then added by perm
+        if (a.fsorted == nullptr)
+            P3MK_TRY(p3mk_alloc((void **)&a.fsorted, sizeof(double) * 3 * (size_t)c->nmax));
+        P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
+        P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)n, qu));
+        if (fp32)
+            hipLaunchKernelGGL(k_realspace_half<true>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
+                               a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                               a.counter, (long)n, a.fsorted, energy);
+        else
+            hipLaunchKernelGGL(k_realspace_half<false>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
+                               a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                               a.counter, (long)n, a.fsorted, energy);
+        P3MK_TRY(p3mk_launch_check("realspace (half shell)"));
+        hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a.perm,
+                           a.fsorted, (long)n, fr);
+        return p3mk_launch_check("realspace forces to caller order");
+    }
+    P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
 #define LAUNCH(F32, SB)                                                                            \
     hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,   \
                        a.norm2, a.perm,                                                            \
