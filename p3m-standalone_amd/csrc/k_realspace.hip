@@ -290,11 +290,13 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
         const float x = (float)alpha * (r2f * rinv);
         const float e = __expf(-x * x);
         const float ec = e * erfcx_f(x);
-        const float fak = inside ? (float)qq * (ec * rinv + (float)two_a_spi * e) * (rinv * rinv) : 0.0f;
+        const float qqm = inside ? (float)qq : 0.0f;
+        const float ecr = ec * rinv;
+        const float fak = qqm * (ecr + (float)two_a_spi * e) * (rinv * rinv);
         acc.f0 = (double)(fak * fx);
         acc.f1 = (double)(fak * fy);
         acc.f2 = (double)(fak * fz);
-        acc.enf = inside ? (float)qq * (ec * rinv) : 0.0f; // twice the energy
+        acc.enf = qqm * ecr; // twice the energy
     } else {
         if (!within_cutoff(dx, dy, dz, rcut))
             return;
@@ -847,15 +849,14 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // several times slower in the reaction sums.
 
 constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
-constexpr int kHalfQueue = 2 * kWave;              // 63 pending + one result register's appends
+constexpr int kHalfQueue = 2 * kWave;              // 63 pending + one append
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
 
 template <bool FP32> struct RsHalfShared {
-    unsigned int qj[kHalfQueue]; // neighbour sorted index | shift index << 27
-    unsigned int qo[kHalfQueue]; // owner | window slot << 6
+    unsigned int qs[kHalfQueue]; // owner | window slot << 6
     double4 shift[27];
     double facc[3][kWave];
-    double facc_nb[3][kWave];    // reactions on the window's particles
+    double facc_nb[3][kWave];    // the window particles' share, f summed (the reaction is -f)
     float4 win_a[kWave];         // (-2 n', 1), n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
@@ -917,8 +918,9 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
     auto drain = [&](int m) {
         const int base = count - m;
         if (lane < m) {
-            const unsigned int js = sh.qj[base + lane], oq = sh.qo[base + lane];
-            const int o = (int)(oq & 63u), slot = (int)(oq >> 6);
+            const unsigned int e = sh.qs[base + lane];
+            const int o = (int)(e & 63u), slot = (int)(e >> 6);
+            const unsigned int js = sh.win_js[slot];
             // both positions from global memory (L1): LDS bandwidth bounds this kernel
             const double4 pj = sorted[js & kIdxMask];
             const double4 ph = sorted[i0 + o];
@@ -936,9 +938,9 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
             atomicAdd(&sh.facc[0][o], pa.f0);
             atomicAdd(&sh.facc[1][o], pa.f1);
             atomicAdd(&sh.facc[2][o], pa.f2);
-            atomicAdd(&sh.facc_nb[0][slot], -pa.f0);
-            atomicAdd(&sh.facc_nb[1][slot], -pa.f1);
-            atomicAdd(&sh.facc_nb[2][slot], -pa.f2);
+            atomicAdd(&sh.facc_nb[0][slot], pa.f0);
+            atomicAdd(&sh.facc_nb[1][slot], pa.f1);
+            atomicAdd(&sh.facc_nb[2][slot], pa.f2);
             if (FP32)
                 en32 += pa.enf;
             else
@@ -947,15 +949,14 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
         count = base;
         __syncthreads();
     };
-    auto append = [&](bool ok, unsigned int js, unsigned int oq) {
+    auto append = [&](bool ok, unsigned int entry) {
         const unsigned long long mask = __ballot(ok);
         if (mask) {
             if (ok) {
                 const unsigned int slot = __builtin_amdgcn_mbcnt_hi(
                     (unsigned int)(mask >> 32),
                     __builtin_amdgcn_mbcnt_lo((unsigned int)mask, (unsigned int)count));
-                sh.qj[slot] = js;
-                sh.qo[slot] = oq;
+                sh.qs[slot] = entry;
             }
             count += __popcll(mask);
         }
@@ -985,28 +986,24 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                 // the 64 entries of one append then have 64 distinct window
                 // slots and (G = 16) 64 distinct owners
                 const int col = lane & 3;
-                auto emit = [&](const floatx4 &d, const uint4 &js4, int base) {
+                auto emit = [&](const floatx4 &d, int base) {
                     float v[4] = {d[0], d[1], d[2], d[3]};
-                    unsigned int jr[4] = {js4.x, js4.y, js4.z, js4.w};
                     if (col & 1) {
                         const float v0 = v[0];
-                        const unsigned int j0 = jr[0];
                         v[0] = v[1]; v[1] = v[2]; v[2] = v[3]; v[3] = v0;
-                        jr[0] = jr[1]; jr[1] = jr[2]; jr[2] = jr[3]; jr[3] = j0;
                     }
                     if (col & 2) {
                         float t = v[0]; v[0] = v[2]; v[2] = t;
                         t = v[1]; v[1] = v[3]; v[3] = t;
-                        unsigned int u = jr[0]; jr[0] = jr[2]; jr[2] = u;
-                        u = jr[1]; jr[1] = jr[3]; jr[3] = u;
                     }
-                    const unsigned int oq0 = (unsigned)hidx | (unsigned)base << 6;
+                    const unsigned int e0 = (unsigned)hidx | (unsigned)base << 6;
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
+                        const unsigned int rq = (unsigned)((col + q) & 3);
                         bool ok = v[q] <= 0.0f;
                         if (SELFW)
-                            ok = ok && keep(jr[q], i_own);
-                        append(ok, jr[q], oq0 + ((unsigned)((col + q) & 3) << 6));
+                            ok = ok && keep(sh.win_js[base + rq], i_own);
+                        append(ok, e0 + (rq << 6));
                         flush();
                     }
                 };
@@ -1016,8 +1013,6 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                     const float4 a1 = sh.win_a[(lane + 4 * s + 4) & 63];
                     const float4 c0 = *reinterpret_cast<const float4 *>(&sh.win_n2[base0]);
                     const float4 c1 = *reinterpret_cast<const float4 *>(&sh.win_n2[base1]);
-                    const uint4 js0 = *reinterpret_cast<const uint4 *>(&sh.win_js[base0]);
-                    const uint4 js1 = *reinterpret_cast<const uint4 *>(&sh.win_js[base1]);
                     floatx4 d0 = {c0.x, c0.y, c0.z, c0.w}, d1 = {c1.x, c1.y, c1.z, c1.w};
                     d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bop.x, d0, 0, 0, 0);
                     d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bop.x, d1, 0, 0, 0);
@@ -1027,8 +1022,8 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                     d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.z, bop.z, d1, 0, 0, 0);
                     d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.w, bop.w, d0, 0, 0, 0);
                     d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.w, bop.w, d1, 0, 0, 0);
-                    emit(d0, js0, base0);
-                    emit(d1, js1, base1);
+                    emit(d0, base0);
+                    emit(d1, base1);
                 }
             };
             if (selfwin)
@@ -1043,7 +1038,7 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                 bool ok = dx * dx + dy * dy + dz * dz <= rc2f;
                 if (selfwin)
                     ok = ok && keep(js, i_own);
-                append(ok, js, (unsigned)lane | (unsigned)k << 6);
+                append(ok, (unsigned)lane | (unsigned)k << 6);
                 flush();
             }
         }
@@ -1056,7 +1051,7 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
             for (int d = 0; d < 3; d++) {
                 const double v = sh.facc_nb[d][lane];
                 if (v != 0.0)
-                    atomicAdd(&fs[d * stride + j], v);
+                    atomicAdd(&fs[d * stride + j], -v);
                 sh.facc_nb[d][lane] = 0.0;
             }
         }
