@@ -986,16 +986,12 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                 // the 64 entries of one append then have 64 distinct window
                 // slots and (G = 16) 64 distinct owners
                 const int col = lane & 3;
+                const bool c1 = col & 1, c2 = col & 2;
                 auto emit = [&](const floatx4 &d, int base) {
-                    float v[4] = {d[0], d[1], d[2], d[3]};
-                    if (col & 1) {
-                        const float v0 = v[0];
-                        v[0] = v[1]; v[1] = v[2]; v[2] = v[3]; v[3] = v0;
-                    }
-                    if (col & 2) {
-                        float t = v[0]; v[0] = v[2]; v[2] = t;
-                        t = v[1]; v[1] = v[3]; v[3] = t;
-                    }
+                    // v[q] = d[(col + q) mod 4] by two selects per value
+                    const float u0 = c1 ? d[1] : d[0], u1 = c1 ? d[2] : d[1], u2 = c1 ? d[3] : d[2],
+                                u3 = c1 ? d[0] : d[3];
+                    const float v[4] = {c2 ? u2 : u0, c2 ? u3 : u1, c2 ? u0 : u2, c2 ? u1 : u3};
                     const unsigned int e0 = (unsigned)hidx | (unsigned)base << 6;
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
