@@ -1,0 +1,120 @@
+"""Real space on cell grids (the half-shell kernel: each unordered pair once,
+the reaction added to the partner) against the oracle's Realteil
+(realpart.c:109-152): dense cells of several 64-particle chunks next to
+sparse ones, clusters that wrap across the periodic boundary, the device
+entry point accumulating into the caller's arrays, and the mixed form."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def corner_cluster(n_cluster, n_background, L, sigma, seed, centre=(0.0, 0.0, 0.0)):
+    """A Gaussian cluster about `centre` (folded into the box, so at a corner
+    it spans the periodic boundary on all three axes) plus a uniform
+    background; charges +-1, neutral."""
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0.0, sigma, (n_cluster, 3)) + np.asarray(centre)
+    b = rng.uniform(0.0, L, (n_background, 3))
+    p = np.mod(np.concatenate([c, b]), L)
+    p[p >= L] = 0.0
+    n = n_cluster + n_background
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    if n % 2:
+        q[-1] = 0.0
+    return {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": float(L)}
+
+
+def cell_counts(s, rcut):
+    """Particles per cell of the library's grid (side >= rcut / 2)."""
+    L = s["L"]
+    nc = int(np.floor(L * 2 / rcut * (1 - 1e-12)))
+    c = np.minimum((np.stack([s["x"], s["y"], s["z"]], 1) * nc / L).astype(int), nc - 1)
+    return nc, np.bincount((c[:, 0] * nc + c[:, 1]) * nc + c[:, 2], minlength=nc ** 3)
+
+
+@pytest.mark.parametrize("centre", [(0.0, 0.0, 0.0), (7.0, 0.3, 13.9)])
+def test_wrapped_dense_cluster(pkg, orc, centre):
+    """A cluster across the box corner / faces: cells of up to several
+    hundred particles (several chunks; the candidate search on the matrix
+    cores), sparse background cells (the vector search), image shifts inside
+    dense cell pairs."""
+    L, rcut, alpha = 14.0, 3.0, 1.0
+    s = corner_cluster(2500, 700, L, 0.9, seed=31, centre=centre)
+    nc, cnt = cell_counts(s, rcut)
+    assert nc >= 5 and cnt.max() > 128 and np.sum((cnt > 0) & (cnt <= 4)) > 50
+    system = pkg.HostSystem.from_dict(s)
+    forces = pkg.realteil(system, rcut, alpha)
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    err = rel_err(forces.real(), fr_ref)
+    print("max cell %d, rel err %.2e, energy %.15e vs %.15e" % (cnt.max(), err, system.c.energy, e_ref))
+    assert err < 1e-13
+    assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+def test_single_dense_cell_of_many_chunks(pkg, orc):
+    """One cell of ~600 particles (ten chunks: the item's own chunk, later
+    chunks of its cell, earlier ones left to their own items) in an
+    otherwise sparse box."""
+    L, rcut, alpha = 12.0, 3.0, 1.2
+    rng = np.random.default_rng(5)
+    side = L / int(np.floor(L * 2 / rcut * (1 - 1e-12)))
+    cell = rng.uniform(3 * side + 1e-3, 4 * side - 1e-3, (600, 3))
+    rest = rng.uniform(0.0, L, (300, 3))
+    p = np.concatenate([cell, rest])
+    q = np.where(np.arange(900) % 2 == 0, 1.0, -1.0)
+    s = {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": L}
+    _, cnt = cell_counts(s, rcut)
+    assert cnt.max() >= 600
+    system = pkg.HostSystem.from_dict(s)
+    forces = pkg.realteil(system, rcut, alpha)
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    assert rel_err(forces.real(), fr_ref) < 1e-13
+    assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+def test_newton_third_law(pkg):
+    """Each unordered pair enters both partners with opposite sign, so the
+    real-space forces of a periodic system sum to zero (to rounding)."""
+    L, rcut, alpha = 14.0, 3.0, 1.0
+    s = corner_cluster(2000, 500, L, 1.2, seed=8, centre=(7.0, 7.0, 7.0))
+    forces = pkg.realteil(pkg.HostSystem.from_dict(s), rcut, alpha)
+    f = forces.real()
+    assert np.abs(f.sum(axis=1)).max() < 1e-10 * np.abs(f).sum()
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+def test_device_form_accumulates_and_mixed_is_close(pkg, orc, prec_name):
+    """The device entry point (both precision forms) adds the real-space
+    forces to the caller's array and the energy to its scalar; the mixed form
+    has the FP64 form's pair set (its energy agrees to FP32 rounding) and
+    forces within 1e-6 of the force norm (its pair energies are summed in
+    FP32 per lane, hence 1e-5 on the energy)."""
+    import torch
+
+    L, rcut, alpha = 14.0, 3.0, 1.0
+    s = corner_cluster(2500, 700, L, 0.9, seed=12)
+    n = len(s["q"])
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    prec = getattr(pkg, prec_name)
+    solver = pkg.Solver(3, pkg.HostSystem.from_dict(s), pkg.parameters(16, 5, alpha, rcut), prec)
+    dev = {k: torch.tensor(np.asarray(s[k]), dtype=torch.float64, device="cuda") for k in "xyzq"}
+    fr = torch.full((3 * n,), 2.0, dtype=torch.float64, device="cuda")
+    en = torch.full((1,), 5.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    solver.realteil_device(n, L, *[dev[k].data_ptr() for k in "xyzq"], rcut, alpha, fr.data_ptr(),
+                           en.data_ptr())
+    solver.synchronize()
+    got = fr.cpu().numpy().reshape(3, n) - 2.0
+    e = float(en.item()) - 5.0
+    solver.close()
+    err = rel_err(got, fr_ref)
+    print("%s: rel err %.2e, energy rel %.2e" % (prec_name, err, abs(e - e_ref) / abs(e_ref)))
+    if prec_name == "FP64":
+        assert err < 1e-13
+        assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+    else:
+        assert err < 1e-6
+        assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
