@@ -1059,6 +1059,11 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
 
     const int hitem = istart[home] + chunk;
     const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
+    // the item's box relative to the home cell centre (the window's frame)
+    const float hcx = (float)((hx + 0.5) * side), hcy = (float)((hy + 0.5) * side),
+                hcz = (float)((hz + 0.5) * side);
+    const float4 hlo_r = make_float4(hlo.x - hcx, hlo.y - hcy, hlo.z - hcz, 0.0f);
+    const float4 hhi_r = make_float4(hhi.x - hcx, hhi.y - hcy, hhi.z - hcz, 0.0f);
     for (int a = 0; a <= reach; a++) {
         int cx = hx + a, wx = 0;
         if (cx >= nc) { cx -= nc; wx = 1; }
@@ -1097,19 +1102,34 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                         continue;
                     if (own && cb == i0)
                         selfwin = true; // the first chunk staged: w == 0 here
-                    for (int k0 = 0; k0 < cnt;) {
-                        const int t = min(kWave - w, cnt - k0);
-                        if (lane < t) {
-                            const int j = cb + k0 + lane;
-                            const float4 v = sorted32[j];
-                            const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
-                            const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
-                            sh.win_a[w + lane] = make_float4(ax, ay, az, 1.0f);
-                            sh.win_n2[w + lane] = nx * nx + ny * ny + nz * nz;
-                            sh.win_js[w + lane] = (unsigned)j | sidx;
+                    // the chunk's particles within rcut of the item's box (all of
+                    // the own chunk), compacted into the window
+                    bool keep_l = false;
+                    float ax = 0.0f, ay = 0.0f, az = 0.0f, n2 = 0.0f;
+                    if (lane < cnt) {
+                        const float4 v = sorted32[cb + lane];
+                        ax = v.x - ox2;
+                        ay = v.y - oy2;
+                        az = v.z - oz2;
+                        const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
+                        n2 = nx * nx + ny * ny + nz * nz;
+                        const float gxx = gap(hlo_r.x, hhi_r.x, nx, nx), gyy = gap(hlo_r.y, hhi_r.y, ny, ny),
+                                    gzz = gap(hlo_r.z, hhi_r.z, nz, nz);
+                        keep_l = gxx * gxx + gyy * gyy + gzz * gzz <= rskip2;
+                    }
+                    unsigned long long mask = __ballot(keep_l);
+                    while (mask) {
+                        const int take = min(kWave - w, __popcll(mask));
+                        const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+                            (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+                        if (keep_l && rank < take) {
+                            sh.win_a[w + rank] = make_float4(ax, ay, az, 1.0f);
+                            sh.win_n2[w + rank] = n2;
+                            sh.win_js[w + rank] = (unsigned)(cb + lane) | sidx;
                         }
-                        w += t;
-                        k0 += t;
+                        w += take;
+                        keep_l = keep_l && rank >= take;
+                        mask = __ballot(keep_l);
                         if (w == kWave)
                             process_window();
                     }
