@@ -86,7 +86,7 @@ struct CellArrays {
     float4 *sorted32 = nullptr;
     float *norm2 = nullptr;
     double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
-    double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n (made on first use)
+    double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n
     float4 *bbox = nullptr;     // 2 per work item
     long max_items = 0;
 };
@@ -94,7 +94,7 @@ int cells_alloc(CellArrays &a, long n, long ncell);
 void cells_release(CellArrays &a);
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
-               const double *q, CellArrays &a, int grid, double *fr, double *energy,
-               p3mk_queue *qu);
+               const double *q, CellArrays &a, int grid, bool any_small, bool any_cells,
+               double *fr, double *energy, p3mk_queue *qu);
 
 }  // namespace p3m

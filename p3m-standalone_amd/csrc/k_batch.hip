@@ -49,6 +49,7 @@ struct p3mk_batch {
     long ntiles = 0, max_items = 0, ncell = 0;
     int tile_side = 8; // requested assignment tile side (autotuned)
     int rs_grid = 0;   // persistent real-space grid (autotuned)
+    bool rs_small = false, rs_cells = false; // systems with nearest-image / cell-grid real space
     int cus = 256;
     // arenas
     void *mesh_arena = nullptr;
@@ -234,6 +235,10 @@ extern "C" int p3mk_batch_create(p3mk_batch **out, int r2c, int interlaced, int 
         S.nc = nc;
         S.reach = reach;
         S.small = (2 * reach + 1 > nc) ? 1 : 0;
+        if (I.rcut > 0.0) {
+            b->rs_small = b->rs_small || S.small;
+            b->rs_cells = b->rs_cells || !S.small;
+        }
         S.cell0 = (int)ncell;
         cell0s[s] = (int)ncell;
         ncell += (long)nc * nc * nc;
@@ -438,5 +443,5 @@ extern "C" int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double
                                     const double *z, const double *q, double *fr, double *energy,
                                     p3mk_queue *qu) {
     return brs_forces(b->form, b->dsys, b->psys, b->cell0s, b->nsys, b->ncell, b->ntot, x, y, z, q,
-                      b->cells, b->rs_grid, fr, energy, qu);
+                      b->cells, b->rs_grid, b->rs_small, b->rs_cells, fr, energy, qu);
 }

@@ -865,7 +865,7 @@ template <bool FP32> struct RsHalfShared {
 
 template <bool FP32>
 __device__ __forceinline__ void
-realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
+realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
                     const float4 *__restrict__ sorted32, const int *__restrict__ start,
                     const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc,
                     int reach, double L, double rcut, double alpha, long stride,
@@ -876,7 +876,7 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
     const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
     const float rskip2 = rskip * rskip;
     const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
-    const int hb = start[home], he = start[home + 1];
+    const int hb = start[cell0 + home], he = start[cell0 + home + 1];
     const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
     const int i0 = hb + chunk * kWave;
     const int nh = min(kWave, he - i0);
@@ -1057,7 +1057,7 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
         __syncthreads();
     };
 
-    const int hitem = istart[home] + chunk;
+    const int hitem = istart[cell0 + home] + chunk;
     const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
     // the item's box relative to the home cell centre (the window's frame)
     const float hcx = (float)((hx + 0.5) * side), hcy = (float)((hy + 0.5) * side),
@@ -1080,7 +1080,7 @@ realspace_item_half(int home, int chunk, const double4 *__restrict__ sorted,
                 const float gz = gap(hlo.z, hhi.z, (float)((hz + c) * side), (float)((hz + c + 1) * side));
                 if (gx * gx + gy * gy + gz * gz > rskip2)
                     continue;
-                const int cell = (cx * nc + cy) * nc + cz;
+                const int cell = cell0 + (cx * nc + cy) * nc + cz;
                 const int jb = __builtin_amdgcn_readfirstlane(start[cell]);
                 const int je = __builtin_amdgcn_readfirstlane(start[cell + 1]);
                 if (jb == je)
@@ -1174,7 +1174,7 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item_half<FP32>(home, item - istart[home], sorted, sorted32, start, istart, bbox, nc,
+        realspace_item_half<FP32>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
                                   reach, L, rcut, alpha, stride, fs, sh, en_tot);
     }
     if (energy != nullptr) {
@@ -1231,6 +1231,7 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.norm2, sizeof(float) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fsorted, sizeof(double) * 3 * nn);
     if (rc != P3MK_OK)
         cells_release(a);
     return rc;
@@ -1376,8 +1377,6 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     const dim3 grid((unsigned)c->grid);
     if (!all_cells) {
         // half shell: forces summed in sorted order, then added by perm
-        if (a.fsorted == nullptr)
-            P3MK_TRY(p3mk_alloc((void **)&a.fsorted, sizeof(double) * 3 * (size_t)c->nmax));
         P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
         P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)n, qu));
         if (fp32)
@@ -1475,17 +1474,55 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
         const int home = (int)prefix_find(istart, ncell, item);
         const int s = p3m::fft::prefix_slot(cell0s, nsys, (long)home);
         const BSys &S = sys[s];
-        if (!(S.rcut > 0.0))
-            continue; // no real-space part for this system
+        if (!(S.rcut > 0.0) || !S.small)
+            continue; // no real-space part, or a cell grid (k_brealspace_half)
         double en = 0.0;
-        if (S.small)
-            realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
-                                       norm2, perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
-                                       ntot, fr, sh, en);
-        else
-            realspace_item<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
-                                        norm2, perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha,
-                                        ntot, fr, sh, en);
+        realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, norm2,
+                                   perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot,
+                                   fr, sh, en);
+        if (energy != nullptr) {
+            for (int off = 32; off > 0; off >>= 1)
+                en += __shfl_down(en, off, kWave);
+            if (threadIdx.x == 0 && en != 0.0)
+                atomicAdd(&energy[s], en);
+        }
+    }
+}
+
+// Cell-grid systems of a batch over the half shell: forces to fs (3 ntot,
+// sorted order, zeroed by the caller), energy[s] += system s's energy.
+// FP64 form held to four waves per SIMD: the batch's sparse cells are
+// latency bound (measured 21.1 ms against 29.1 ms unconstrained at the
+// 256-system batch; the single-system kernel is faster unconstrained).
+template <bool FP32>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 4)))
+k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
+                  const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
+                  const int *__restrict__ start, const int *__restrict__ istart,
+                  const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
+                  double *__restrict__ fs, double *__restrict__ energy) {
+    __shared__ RsHalfShared<FP32> sh;
+    const int total = istart[ncell];
+    if (!FP32) {
+        for (int k = threadIdx.x; k < kErfcxTab; k += blockDim.x)
+            sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
+    }
+    __syncthreads();
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0)
+            item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total)
+            break;
+        const int home = (int)prefix_find(istart, ncell, item);
+        const int s = p3m::fft::prefix_slot(cell0s, nsys, (long)home);
+        const BSys &S = sys[s];
+        if (!(S.rcut > 0.0) || S.small)
+            continue; // no real-space part, or nearest images (k_brealspace)
+        double en = 0.0;
+        realspace_item_half<FP32>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
+                                  istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fs, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
                 en += __shfl_down(en, off, kWave);
@@ -1501,8 +1538,8 @@ namespace p3m {
 
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
-               const double *q, CellArrays &a, int grid, double *fr, double *energy,
-               p3mk_queue *qu) {
+               const double *q, CellArrays &a, int grid, bool any_small, bool any_cells,
+               double *fr, double *energy, p3mk_queue *qu) {
     if (ntot <= 0)
         return P3MK_OK;
     hipStream_t s = p3mk_stream(qu);
@@ -1517,15 +1554,35 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
                        a.norm2);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
-    if (f.fp32)
-        hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart, a.bbox,
-                           ncell, a.counter, fr, energy);
-    else
-        hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                           nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart, a.bbox,
-                           ncell, a.counter, fr, energy);
-    return p3mk_launch_check("batched real space");
+    if (any_small) {
+        // nearest-image systems: forces straight to fr
+        if (f.fp32)
+            hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
+                               nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart,
+                               a.bbox, ncell, a.counter, fr, energy);
+        else
+            hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
+                               nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart,
+                               a.bbox, ncell, a.counter, fr, energy);
+        P3MK_TRY(p3mk_launch_check("batched real space (nearest images)"));
+    }
+    if (any_cells) {
+        // cell-grid systems: half shell in sorted order, then added by perm
+        P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)ntot, qu));
+        if (f.fp32)
+            hipLaunchKernelGGL(k_brealspace_half<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
+                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.start, a.istart, a.bbox, ncell,
+                               a.counter + 1, a.fsorted, energy);
+        else
+            hipLaunchKernelGGL(k_brealspace_half<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
+                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.start, a.istart, a.bbox, ncell,
+                               a.counter + 1, a.fsorted, energy);
+        P3MK_TRY(p3mk_launch_check("batched real space (half shell)"));
+        hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, ntot,
+                           a.perm, a.fsorted, (long)ntot, fr);
+        P3MK_TRY(p3mk_launch_check("batched real-space forces to caller order"));
+    }
+    return P3MK_OK;
 }
 
 }  // namespace p3m
