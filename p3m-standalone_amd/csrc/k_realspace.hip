@@ -91,7 +91,7 @@ __device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int n
     const float lx = (float)(r.x - (cx + 0.5) * side), ly = (float)(r.y - (cy + 0.5) * side),
                 lz = (float)(r.z - (cz + 0.5) * side);
     sorted[pos] = r;
-    sorted32[pos] = make_float4(-2.0f * lx, -2.0f * ly, -2.0f * lz, 1.0f);
+    sorted32[pos] = make_float4(-2.0f * lx, -2.0f * ly, -2.0f * lz, (float)r.w);
     norm2[pos] = lx * lx + ly * ly + lz * lz;
 }
 
@@ -99,7 +99,7 @@ __device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int n
 __device__ __forceinline__ void store_pad(int pos, double4 *__restrict__ sorted,
                                           float4 *__restrict__ sorted32, float *__restrict__ norm2) {
     sorted[pos] = make_double4(0.0, 0.0, 0.0, 0.0);
-    sorted32[pos] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    sorted32[pos] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     norm2[pos] = 1e30f;
 }
 
@@ -267,6 +267,24 @@ __device__ __forceinline__ bool within_cutoff(double dx, double dy, double dz, d
 //  - FP32: d = dx rounded to FP32 gives r2f within 5 2^-24 of r2, so the
 //    inclusion test is decided on r2f except within 1e-6 of rcut^2, where
 //    the FP64 test runs.
+// FP32 pair force from the FP32 difference (fx, fy, fz), r2f = |f|^2, the
+// exact inclusion decision and the charge product
+__device__ __forceinline__ void pair_force_f32(float fx, float fy, float fz, float r2f, bool inside,
+                                               float qq, double alpha, double two_a_spi,
+                                               PairAcc &acc) {
+    const float rinv = __builtin_amdgcn_rsqf(r2f);
+    const float x = (float)alpha * (r2f * rinv);
+    const float e = __expf(-x * x);
+    const float ec = e * erfcx_f(x);
+    const float qqm = inside ? qq : 0.0f;
+    const float ecr = ec * rinv;
+    const float fak = qqm * (ecr + (float)two_a_spi * e) * (rinv * rinv);
+    acc.f0 = (double)(fak * fx);
+    acc.f1 = (double)(fak * fy);
+    acc.f2 = (double)(fak * fz);
+    acc.enf = qqm * ecr; // twice the energy
+}
+
 template <bool FP32>
 __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, double sx, double sy,
                                      double sz, double rcut, double alpha, double two_a_spi,
@@ -286,17 +304,7 @@ __device__ __forceinline__ void pair(const double4 &pi, const double4 &pj, doubl
         bool inside = r2f <= (float)(rc2 * (1.0 - 1e-6));
         if (__builtin_expect(!inside && r2f <= (float)(rc2 * (1.0 + 1e-6)), 0))
             inside = within_cutoff(dx, dy, dz, rcut);
-        const float rinv = __builtin_amdgcn_rsqf(r2f);
-        const float x = (float)alpha * (r2f * rinv);
-        const float e = __expf(-x * x);
-        const float ec = e * erfcx_f(x);
-        const float qqm = inside ? (float)qq : 0.0f;
-        const float ecr = ec * rinv;
-        const float fak = qqm * (ecr + (float)two_a_spi * e) * (rinv * rinv);
-        acc.f0 = (double)(fak * fx);
-        acc.f1 = (double)(fak * fy);
-        acc.f2 = (double)(fak * fz);
-        acc.enf = qqm * ecr; // twice the energy
+        pair_force_f32(fx, fy, fz, r2f, inside, (float)qq, alpha, two_a_spi, acc);
     } else {
         if (!within_cutoff(dx, dy, dz, rcut))
             return;
@@ -415,53 +423,6 @@ __device__ __forceinline__ float box_dist2(float4 alo, float4 ahi, float4 blo, f
 // distance along one axis between [alo, ahi] and [blo, bhi]
 __device__ __forceinline__ float gap(float alo, float ahi, float blo, float bhi) {
     return fmaxf(0.0f, fmaxf(blo - ahi, alo - bhi));
-}
-
-// One neighbour chunk [cb, cb + cnt) against the item's home particles on
-// the matrix cores (v_mfma_f32_16x16x4f32: lane l holds A[l % 16][l / 16]
-// and B[l / 16][l % 16]; result register r of lane l is
-// D[4 (l / 16) + r][l % 16]).  Rows are neighbours, columns home particles:
-// A = (-2 n, 1) from sorted32, B = bop (h', |h'|^2 - rc2f) per row tile of
-// home particles, C = |n|^2.  D <= 0 queues the pair (neighbour, owner);
-// the queue is drained after every tile.  SELF: the chunk is the item's
-// own, whose diagonal is excluded.
-template <bool SELF, typename Append, typename Flush>
-__device__ __forceinline__ void scan_chunk(Append &append, Flush &flush,
-                                           const float4 *__restrict__ sorted32,
-                                           const float *__restrict__ norm2, int cb, int cnt,
-                                           int ntr, int i0, int col, int kc, floatx4 bop,
-                                           unsigned int shp) {
-    const int je = cb + cnt;
-    const int ntc = (cnt + 15) >> 4;
-    for (int tc = 0; tc < ntc; tc++) {
-        const int row0 = cb + 16 * tc;
-        const float av = reinterpret_cast<const float *>(sorted32)[4 * (row0 + col) + kc];
-        const int rb = row0 + 4 * kc; // neighbour rows of this lane's results
-        floatx4 cv;
-        cv[0] = norm2[rb];
-        cv[1] = norm2[rb + 1];
-        cv[2] = norm2[rb + 2];
-        cv[3] = norm2[rb + 3];
-        if (row0 + 16 > je) {
-            // rows past the chunk's end never pass
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (rb + r >= je)
-                    cv[r] = 1e30f;
-        }
-        for (int tr = 0; tr < ntr; tr++) {
-            const floatx4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bop[tr], cv, 0, 0, 0);
-            const unsigned int os = pack_pair(16 * tr + col, 0) | shp;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                bool ok = d[r] <= 0.0f;
-                if (SELF)
-                    ok = ok && (rb + r != i0 + 16 * tr + col);
-                append(ok, (unsigned)(rb + r), os);
-            }
-            flush();
-        }
-    }
 }
 
 // One neighbour chunk [cb, cb + cnt) against the item's home particles on
@@ -648,23 +609,10 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     } else {
         const int hitem = istart[hcell] + chunk;
         const float4 hlo = bbox[2 * hitem], hhi = bbox[2 * hitem + 1];
-        const int kc = lane >> 4, col = lane & 15;
-        const int ntr = (nh + 15) >> 4; // row tiles of home particles
         const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
-        const bool tiles = FP32 && he - hb > kMfmaMinCell;
-        // this lane's home particle relative to the home cell centre (vector
-        // search; inactive lanes far away)
-        const float4 hl = active ? sorted32[i] : make_float4(-2e30f, -2e30f, -2e30f, 1.0f);
-        // component kc < 3 of home particle 16 tr + col relative to the home
-        // cell centre (0 in the lanes kc = 3, which sum the squares)
-        floatx4 hk;
-        bool hv[4];
-#pragma unroll
-        for (int tr = 0; tr < 4; tr++) {
-            hv[tr] = 16 * tr + col < nh;
-            const float *s = reinterpret_cast<const float *>(sorted32 + i0 + 16 * tr + col);
-            hk[tr] = (tiles && hv[tr] && kc < 3) ? -0.5f * s[kc] : 0.0f;
-        }
+        // this lane's home particle relative to the home cell centre (inactive
+        // lanes far away)
+        const float4 hl = active ? sorted32[i] : make_float4(-2e30f, -2e30f, -2e30f, 0.0f);
         for (int a = -reach; a <= reach; a++) {
             int cx = hx + a, wx = 0;
             if (cx < 0) { cx += nc; wx = -1; } else if (cx >= nc) { cx -= nc; wx = 1; }
@@ -689,29 +637,6 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                     if (jb == je)
                         continue;
                     const int item0 = __builtin_amdgcn_readfirstlane(istart[cell]);
-                    // B operand: h' = h - o (o the neighbour cell's centre
-                    // relative to the home cell's) in the lanes kc < 3, and
-                    // |h'|^2 - rc2f in the lanes kc = 3, summed over the four
-                    // lanes of a column (lanes col + 16 k) by two lane swaps
-                    floatx4 bop;
-                    if (tiles) {
-                        const int ok = kc == 0 ? a : (kc == 1 ? b : (kc == 2 ? c : 0));
-                        const float oc = (float)(ok * side);
-#pragma unroll
-                        for (int tr = 0; tr < 4; tr++) {
-                            if (tr >= ntr)
-                                break;
-                            const float p = hk[tr] - oc;
-                            const float q = p * p;
-                            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(q),
-                                                                              __float_as_uint(q), false, false);
-                            const float q2 = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
-                            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(q2),
-                                                                              __float_as_uint(q2), false, false);
-                            const float n2 = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
-                            bop[tr] = kc < 3 ? p : (hv[tr] ? n2 - rc2f : 1e30f);
-                        }
-                    }
                     const unsigned int shp = pack_pair(0, shift_index(wx, wy, wz));
                     const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
                     for (int cb = jb; cb < je; cb += kWave) {
@@ -722,17 +647,8 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
                         nhi.x += fsx; nhi.y += fsy; nhi.z += fsz;
                         if (box_dist2(hlo, hhi, nlo, nhi) > rskip2)
                             continue;
-                        if (tiles) {
-                            if (cb == i0)
-                                scan_chunk<true>(append, flush, sorted32, norm2, cb, cnt, ntr, i0, col,
-                                                 kc, bop, shp);
-                            else
-                                scan_chunk<false>(append, flush, sorted32, norm2, cb, cnt, ntr, i0, col,
-                                                  kc, bop, shp);
-                        } else {
-                            scan_chunk_scalar(append, flush, sorted32, cb, cnt, i0, lane, hl, a * side,
-                                              b * side, c * side, rc2f, pack_pair(lane, 0) | shp);
-                        }
+                        scan_chunk_scalar(append, flush, sorted32, cb, cnt, i0, lane, hl, a * side,
+                                          b * side, c * side, rc2f, pack_pair(lane, 0) | shp);
                     }
                 }
             }
@@ -860,6 +776,9 @@ template <bool FP32> struct RsHalfShared {
     float4 win_a[kWave];         // (-2 n', 1), n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
+    // FP32 form: the neighbour cell's offset from the home cell, (a + 2) |
+    // (b + 2) << 3 | (c + 2) << 6
+    unsigned short win_off[FP32 ? kWave : 1];
     double tab[FP32 ? 1 : kErfcxTab];
 };
 
@@ -915,26 +834,77 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     float en32 = 0.0f;
     __syncthreads();
 
+    // band of the FP32 inclusion test (mixed form): r2f is within 1.5e-6
+    // rcut^2 of r^2 (side <= 4 rcut / 3)
+    const double rc2 = rcut * rcut;
+    const float rin2 = (float)(rc2 * (1.0 - 4e-6)), rout2 = (float)(rc2 * (1.0 + 4e-6));
+    const float rnear2 = (float)(rc2 / 64.0);
+    const float sidef = (float)side;
     auto drain = [&](int m) {
         const int base = count - m;
         if (lane < m) {
             const unsigned int e = sh.qs[base + lane];
             const int o = (int)(e & 63u), slot = (int)(e >> 6);
             const unsigned int js = sh.win_js[slot];
-            // both positions from global memory (L1): LDS bandwidth bounds this kernel
-            const double4 pj = sorted[js & kIdxMask];
-            const double4 ph = sorted[i0 + o];
-            // the image shift, read only by the lanes whose pair wraps
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            const unsigned int sidx = js >> 27;
-            if (sidx != kNoShift) {
-                const double4 s3 = sh.shift[sidx];
-                sx = s3.x;
-                sy = s3.y;
-                sz = s3.z;
-            }
             PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
-            pair<FP32>(ph, pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+            if constexpr (FP32) {
+                // the difference from the FP32 coordinates relative to the cell
+                // centres, rows of (-2 x, -2 y, -2 z, q) (32 bytes per pair where
+                // the FP64 positions take 64; the loads through L1 bound this
+                // loop), error below 4.5e-7 side; pairs nearer than rcut / 8,
+                // whose relative error that would raise, and pairs within 4e-6
+                // of the cutoff, whose inclusion it could flip, take the exact
+                // FP64 difference instead
+                const float4 vn = sorted32[js & kIdxMask];
+                const float4 vh = sorted32[i0 + o];
+                const unsigned int off = sh.win_off[slot];
+                const float ox = (float)((int)(off & 7u) - 2) * sidef,
+                            oy = (float)((int)((off >> 3) & 7u) - 2) * sidef,
+                            oz = (float)((int)(off >> 6) - 2) * sidef;
+                float fx = fmaf(0.5f, vn.x - vh.x, -ox), fy = fmaf(0.5f, vn.y - vh.y, -oy),
+                      fz = fmaf(0.5f, vn.z - vh.z, -oz);
+                float r2f = fx * fx + fy * fy + fz * fz;
+                bool inside = r2f <= rin2;
+                if (__builtin_expect(r2f < rnear2 || (!inside && r2f <= rout2), 0)) {
+                    const double4 pj = sorted[js & kIdxMask];
+                    const double4 ph = sorted[i0 + o];
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    const unsigned int sidx = js >> 27;
+                    if (sidx != kNoShift) {
+                        const double4 s3 = sh.shift[sidx];
+                        sx = s3.x;
+                        sy = s3.y;
+                        sz = s3.z;
+                    }
+                    double dx, dy, dz;
+                    {
+#pragma clang fp contract(off)
+                        dx = (ph.x - pj.x) - sx;
+                        dy = (ph.y - pj.y) - sy;
+                        dz = (ph.z - pj.z) - sz;
+                    }
+                    fx = (float)dx;
+                    fy = (float)dy;
+                    fz = (float)dz;
+                    r2f = fx * fx + fy * fy + fz * fz;
+                    inside = within_cutoff(dx, dy, dz, rcut);
+                }
+                pair_force_f32(fx, fy, fz, r2f, inside, vh.w * vn.w, alpha, two_a_spi, pa);
+            } else {
+                // both positions from global memory (L1): LDS bandwidth bounds this kernel
+                const double4 pj = sorted[js & kIdxMask];
+                const double4 ph = sorted[i0 + o];
+                // the image shift, read only by the lanes whose pair wraps
+                double sx = 0.0, sy = 0.0, sz = 0.0;
+                const unsigned int sidx = js >> 27;
+                if (sidx != kNoShift) {
+                    const double4 s3 = sh.shift[sidx];
+                    sx = s3.x;
+                    sy = s3.y;
+                    sz = s3.z;
+                }
+                pair<false>(ph, pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+            }
             atomicAdd(&sh.facc[0][o], pa.f0);
             atomicAdd(&sh.facc[1][o], pa.f1);
             atomicAdd(&sh.facc[2][o], pa.f2);
@@ -1126,6 +1096,8 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                             sh.win_a[w + rank] = make_float4(ax, ay, az, 1.0f);
                             sh.win_n2[w + rank] = n2;
                             sh.win_js[w + rank] = (unsigned)(cb + lane) | sidx;
+                            if (FP32)
+                                sh.win_off[w + rank] = (unsigned short)((a + 2) | (b + 2) << 3 | (c + 2) << 6);
                         }
                         w += take;
                         keep_l = keep_l && rank >= take;
@@ -1398,11 +1370,10 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
                        a.norm2, a.perm,                                                            \
                        a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
                        a.counter, fr, energy)
-    if (fp32) {
-        if (all_cells) LAUNCH(true, true); else LAUNCH(true, false);
-    } else {
-        if (all_cells) LAUNCH(false, true); else LAUNCH(false, false);
-    }
+    if (fp32)
+        LAUNCH(true, true);
+    else
+        LAUNCH(false, true);
 #undef LAUNCH
     return p3mk_launch_check("realspace");
 }
