@@ -917,7 +917,11 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 en_tot += 2.0 * pa.en;
         }
         count = base;
-        __syncthreads();
+        // no wait here: the workgroup is this one wavefront, whose LDS
+        // accesses are performed in program order, and the queue entries just
+        // read were consumed above; the accumulators are read only after the
+        // barrier that ends the window
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
     };
     auto append = [&](bool ok, unsigned int entry) {
         const unsigned long long mask = __ballot(ok);
@@ -933,7 +937,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     };
     auto flush = [&]() {
         if (count >= kWave) {
-            __syncthreads();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             do
                 drain(kWave);
             while (count >= kWave);
