@@ -765,7 +765,7 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // several times slower in the reaction sums.
 
 constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
-constexpr int kHalfQueue = 2 * kWave;              // 63 pending + one append
+constexpr int kHalfQueue = 3 * kWave;              // 127 pending (FP64 form) + one append
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
 
 template <bool FP32> struct RsHalfShared {
@@ -840,61 +840,51 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     const float rin2 = (float)(rc2 * (1.0 - 4e-6)), rout2 = (float)(rc2 * (1.0 + 4e-6));
     const float rnear2 = (float)(rc2 / 64.0);
     const float sidef = (float)side;
-    auto drain = [&](int m) {
-        const int base = count - m;
-        if (lane < m) {
-            const unsigned int e = sh.qs[base + lane];
-            const int o = (int)(e & 63u), slot = (int)(e >> 6);
-            const unsigned int js = sh.win_js[slot];
-            PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
-            if constexpr (FP32) {
-                // the difference from the FP32 coordinates relative to the cell
-                // centres, rows of (-2 x, -2 y, -2 z, q) (32 bytes per pair where
-                // the FP64 positions take 64; the loads through L1 bound this
-                // loop), error below 4.5e-7 side; pairs nearer than rcut / 8,
-                // whose relative error that would raise, and pairs within 4e-6
-                // of the cutoff, whose inclusion it could flip, take the exact
-                // FP64 difference instead
-                const float4 vn = sorted32[js & kIdxMask];
-                const float4 vh = sorted32[i0 + o];
-                const unsigned int off = sh.win_off[slot];
-                const float ox = (float)((int)(off & 7u) - 2) * sidef,
-                            oy = (float)((int)((off >> 3) & 7u) - 2) * sidef,
-                            oz = (float)((int)(off >> 6) - 2) * sidef;
-                float fx = fmaf(0.5f, vn.x - vh.x, -ox), fy = fmaf(0.5f, vn.y - vh.y, -oy),
-                      fz = fmaf(0.5f, vn.z - vh.z, -oz);
-                float r2f = fx * fx + fy * fy + fz * fz;
-                bool inside = r2f <= rin2;
-                if (__builtin_expect(r2f < rnear2 || (!inside && r2f <= rout2), 0)) {
-                    const double4 pj = sorted[js & kIdxMask];
-                    const double4 ph = sorted[i0 + o];
-                    double sx = 0.0, sy = 0.0, sz = 0.0;
-                    const unsigned int sidx = js >> 27;
-                    if (sidx != kNoShift) {
-                        const double4 s3 = sh.shift[sidx];
-                        sx = s3.x;
-                        sy = s3.y;
-                        sz = s3.z;
-                    }
-                    double dx, dy, dz;
-                    {
-#pragma clang fp contract(off)
-                        dx = (ph.x - pj.x) - sx;
-                        dy = (ph.y - pj.y) - sy;
-                        dz = (ph.z - pj.z) - sz;
-                    }
-                    fx = (float)dx;
-                    fy = (float)dy;
-                    fz = (float)dz;
-                    r2f = fx * fx + fy * fy + fz * fz;
-                    inside = within_cutoff(dx, dy, dz, rcut);
-                }
-                pair_force_f32(fx, fy, fz, r2f, inside, vh.w * vn.w, alpha, two_a_spi, pa);
-            } else {
-                // both positions from global memory (L1): LDS bandwidth bounds this kernel
+    // one queued pair: what a round reads before its arithmetic
+    struct Fetched {
+        int o, slot;
+        unsigned int js, off;
+        float4 vn, vh;   // FP32 form: cell-relative rows (-2 x, -2 y, -2 z, q)
+        double4 pj, ph;  // FP64 form: positions
+    };
+    auto fetch = [&](int base, Fetched &f) {
+        const unsigned int e = sh.qs[base + lane];
+        f.o = (int)(e & 63u);
+        f.slot = (int)(e >> 6);
+        f.js = sh.win_js[f.slot];
+        if constexpr (FP32) {
+            f.vn = sorted32[f.js & kIdxMask];
+            f.vh = sorted32[i0 + f.o];
+            f.off = sh.win_off[f.slot];
+        } else {
+            // both positions from global memory (L1): LDS bandwidth bounds this kernel
+            f.pj = sorted[f.js & kIdxMask];
+            f.ph = sorted[i0 + f.o];
+        }
+    };
+    auto evaluate = [&](const Fetched &f) {
+        const int o = f.o, slot = f.slot;
+        const unsigned int js = f.js;
+        PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
+        if constexpr (FP32) {
+            // the difference from the FP32 coordinates relative to the cell
+            // centres (16 bytes per particle where the FP64 positions take 32;
+            // the loads through L1 are a bound of this loop), error below
+            // 4.5e-7 side; pairs nearer than rcut / 8, whose relative error
+            // that would raise, and pairs within 4e-6 of the cutoff, whose
+            // inclusion it could flip, take the exact FP64 difference instead
+            const float4 vn = f.vn, vh = f.vh;
+            const unsigned int off = f.off;
+            const float ox = (float)((int)(off & 7u) - 2) * sidef,
+                        oy = (float)((int)((off >> 3) & 7u) - 2) * sidef,
+                        oz = (float)((int)(off >> 6) - 2) * sidef;
+            float fx = fmaf(0.5f, vn.x - vh.x, -ox), fy = fmaf(0.5f, vn.y - vh.y, -oy),
+                  fz = fmaf(0.5f, vn.z - vh.z, -oz);
+            float r2f = fx * fx + fy * fy + fz * fz;
+            bool inside = r2f <= rin2;
+            if (__builtin_expect(r2f < rnear2 || (!inside && r2f <= rout2), 0)) {
                 const double4 pj = sorted[js & kIdxMask];
                 const double4 ph = sorted[i0 + o];
-                // the image shift, read only by the lanes whose pair wraps
                 double sx = 0.0, sy = 0.0, sz = 0.0;
                 const unsigned int sidx = js >> 27;
                 if (sidx != kNoShift) {
@@ -903,24 +893,67 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     sy = s3.y;
                     sz = s3.z;
                 }
-                pair<false>(ph, pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+                double dx, dy, dz;
+                {
+#pragma clang fp contract(off)
+                    dx = (ph.x - pj.x) - sx;
+                    dy = (ph.y - pj.y) - sy;
+                    dz = (ph.z - pj.z) - sz;
+                }
+                fx = (float)dx;
+                fy = (float)dy;
+                fz = (float)dz;
+                r2f = fx * fx + fy * fy + fz * fz;
+                inside = within_cutoff(dx, dy, dz, rcut);
             }
-            atomicAdd(&sh.facc[0][o], pa.f0);
-            atomicAdd(&sh.facc[1][o], pa.f1);
-            atomicAdd(&sh.facc[2][o], pa.f2);
-            atomicAdd(&sh.facc_nb[0][slot], pa.f0);
-            atomicAdd(&sh.facc_nb[1][slot], pa.f1);
-            atomicAdd(&sh.facc_nb[2][slot], pa.f2);
-            if (FP32)
-                en32 += pa.enf;
-            else
-                en_tot += 2.0 * pa.en;
+            pair_force_f32(fx, fy, fz, r2f, inside, vh.w * vn.w, alpha, two_a_spi, pa);
+        } else {
+            // the image shift, read only by the lanes whose pair wraps
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const unsigned int sidx = js >> 27;
+            if (sidx != kNoShift) {
+                const double4 s3 = sh.shift[sidx];
+                sx = s3.x;
+                sy = s3.y;
+                sz = s3.z;
+            }
+            pair<false>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+        }
+        atomicAdd(&sh.facc[0][o], pa.f0);
+        atomicAdd(&sh.facc[1][o], pa.f1);
+        atomicAdd(&sh.facc[2][o], pa.f2);
+        atomicAdd(&sh.facc_nb[0][slot], pa.f0);
+        atomicAdd(&sh.facc_nb[1][slot], pa.f1);
+        atomicAdd(&sh.facc_nb[2][slot], pa.f2);
+        if (FP32)
+            en32 += pa.enf;
+        else
+            en_tot += 2.0 * pa.en;
+    };
+    // evaluate the last m <= 64 queued pairs
+    // (no LDS wait after a round: the workgroup is this one wavefront, whose
+    // LDS accesses are performed in program order, and the queue entries read
+    // were consumed; the accumulators are read only after the barrier that
+    // ends the window)
+    auto drain = [&](int m) {
+        const int base = count - m;
+        if (lane < m) {
+            Fetched f;
+            fetch(base, f);
+            evaluate(f);
         }
         count = base;
-        // no wait here: the workgroup is this one wavefront, whose LDS
-        // accesses are performed in program order, and the queue entries just
-        // read were consumed above; the accumulators are read only after the
-        // barrier that ends the window
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    };
+    // two full rounds, the second one's reads issued before the first one's
+    // arithmetic so that their latencies overlap
+    auto drain2 = [&]() {
+        Fetched fa, fb;
+        fetch(count - kWave, fa);
+        fetch(count - 2 * kWave, fb);
+        evaluate(fa);
+        evaluate(fb);
+        count -= 2 * kWave;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
     };
     auto append = [&](bool ok, unsigned int entry) {
@@ -935,12 +968,24 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
             count += __popcll(mask);
         }
     };
+    // FP64 form: two rounds at a time (11.4 ms against 12.0 at the Gaussian
+    // 1M workload); the FP32 form has no registers to spare for the second
+    // round's data (8.5 ms against 7.5 with it, from spills)
     auto flush = [&]() {
-        if (count >= kWave) {
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            do
-                drain(kWave);
-            while (count >= kWave);
+        if constexpr (FP32) {
+            if (count >= kWave) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                do
+                    drain(kWave);
+                while (count >= kWave);
+            }
+        } else {
+            if (count >= 2 * kWave) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                do
+                    drain2();
+                while (count >= 2 * kWave);
+            }
         }
     };
     // the reference's exclusion in the own chunk: keep j > i only
@@ -1013,6 +1058,8 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
             }
         }
         __syncthreads();
+        if (count >= kWave)
+            drain(kWave);
         if (count > 0)
             drain(count);
         if (lane < w) {
