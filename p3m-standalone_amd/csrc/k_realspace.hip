@@ -776,13 +776,10 @@ template <bool FP32> struct RsHalfShared {
     float4 win_a[kWave];         // (-2 n', 1), n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
-    // FP32 form: the neighbour cell's offset from the home cell, (a + 2) |
-    // (b + 2) << 3 | (c + 2) << 6
-    unsigned short win_off[FP32 ? kWave : 1];
     double tab[FP32 ? 1 : kErfcxTab];
 };
 
-template <bool FP32>
+template <bool FP32, bool PAIRED_ROUNDS>
 __device__ __forceinline__ void
 realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
                     const float4 *__restrict__ sorted32, const int *__restrict__ start,
@@ -834,91 +831,34 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     float en32 = 0.0f;
     __syncthreads();
 
-    // band of the FP32 inclusion test (mixed form): r2f is within 1.5e-6
-    // rcut^2 of r^2 (side <= 4 rcut / 3)
-    const double rc2 = rcut * rcut;
-    const float rin2 = (float)(rc2 * (1.0 - 4e-6)), rout2 = (float)(rc2 * (1.0 + 4e-6));
-    const float rnear2 = (float)(rc2 / 64.0);
-    const float sidef = (float)side;
-    // one queued pair: what a round reads before its arithmetic
+    // one queued pair: what a round reads before its arithmetic (both
+    // positions from global memory, through L1)
     struct Fetched {
         int o, slot;
-        unsigned int js, off;
-        float4 vn, vh;   // FP32 form: cell-relative rows (-2 x, -2 y, -2 z, q)
-        double4 pj, ph;  // FP64 form: positions
+        unsigned int js;
+        double4 pj, ph;
     };
     auto fetch = [&](int base, Fetched &f) {
         const unsigned int e = sh.qs[base + lane];
         f.o = (int)(e & 63u);
         f.slot = (int)(e >> 6);
         f.js = sh.win_js[f.slot];
-        if constexpr (FP32) {
-            f.vn = sorted32[f.js & kIdxMask];
-            f.vh = sorted32[i0 + f.o];
-            f.off = sh.win_off[f.slot];
-        } else {
-            // both positions from global memory (L1): LDS bandwidth bounds this kernel
-            f.pj = sorted[f.js & kIdxMask];
-            f.ph = sorted[i0 + f.o];
-        }
+        f.pj = sorted[f.js & kIdxMask];
+        f.ph = sorted[i0 + f.o];
     };
     auto evaluate = [&](const Fetched &f) {
         const int o = f.o, slot = f.slot;
-        const unsigned int js = f.js;
-        PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
-        if constexpr (FP32) {
-            // the difference from the FP32 coordinates relative to the cell
-            // centres (16 bytes per particle where the FP64 positions take 32;
-            // the loads through L1 are a bound of this loop), error below
-            // 4.5e-7 side; pairs nearer than rcut / 8, whose relative error
-            // that would raise, and pairs within 4e-6 of the cutoff, whose
-            // inclusion it could flip, take the exact FP64 difference instead
-            const float4 vn = f.vn, vh = f.vh;
-            const unsigned int off = f.off;
-            const float ox = (float)((int)(off & 7u) - 2) * sidef,
-                        oy = (float)((int)((off >> 3) & 7u) - 2) * sidef,
-                        oz = (float)((int)(off >> 6) - 2) * sidef;
-            float fx = fmaf(0.5f, vn.x - vh.x, -ox), fy = fmaf(0.5f, vn.y - vh.y, -oy),
-                  fz = fmaf(0.5f, vn.z - vh.z, -oz);
-            float r2f = fx * fx + fy * fy + fz * fz;
-            bool inside = r2f <= rin2;
-            if (__builtin_expect(r2f < rnear2 || (!inside && r2f <= rout2), 0)) {
-                const double4 pj = sorted[js & kIdxMask];
-                const double4 ph = sorted[i0 + o];
-                double sx = 0.0, sy = 0.0, sz = 0.0;
-                const unsigned int sidx = js >> 27;
-                if (sidx != kNoShift) {
-                    const double4 s3 = sh.shift[sidx];
-                    sx = s3.x;
-                    sy = s3.y;
-                    sz = s3.z;
-                }
-                double dx, dy, dz;
-                {
-#pragma clang fp contract(off)
-                    dx = (ph.x - pj.x) - sx;
-                    dy = (ph.y - pj.y) - sy;
-                    dz = (ph.z - pj.z) - sz;
-                }
-                fx = (float)dx;
-                fy = (float)dy;
-                fz = (float)dz;
-                r2f = fx * fx + fy * fy + fz * fz;
-                inside = within_cutoff(dx, dy, dz, rcut);
-            }
-            pair_force_f32(fx, fy, fz, r2f, inside, vh.w * vn.w, alpha, two_a_spi, pa);
-        } else {
-            // the image shift, read only by the lanes whose pair wraps
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            const unsigned int sidx = js >> 27;
-            if (sidx != kNoShift) {
-                const double4 s3 = sh.shift[sidx];
-                sx = s3.x;
-                sy = s3.y;
-                sz = s3.z;
-            }
-            pair<false>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+        // the image shift, read only by the lanes whose pair wraps
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        const unsigned int sidx = f.js >> 27;
+        if (sidx != kNoShift) {
+            const double4 s3 = sh.shift[sidx];
+            sx = s3.x;
+            sy = s3.y;
+            sz = s3.z;
         }
+        PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
+        pair<FP32>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
         atomicAdd(&sh.facc[0][o], pa.f0);
         atomicAdd(&sh.facc[1][o], pa.f1);
         atomicAdd(&sh.facc[2][o], pa.f2);
@@ -968,11 +908,13 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
             count += __popcll(mask);
         }
     };
-    // FP64 form: two rounds at a time (11.4 ms against 12.0 at the Gaussian
-    // 1M workload); the FP32 form has no registers to spare for the second
-    // round's data (8.5 ms against 7.5 with it, from spills)
+    // PAIRED_ROUNDS: two rounds at a time (the FP64 form of the single-system
+    // kernel: 11.4 ms against 12.0 at the Gaussian 1M workload); the FP32
+    // form has no registers to spare for the second round's data (8.5 ms
+    // against 7.5 with it, from spills), nor the batched FP64 kernel, which
+    // is held to four waves
     auto flush = [&]() {
-        if constexpr (FP32) {
+        if constexpr (!PAIRED_ROUNDS) {
             if (count >= kWave) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 do
@@ -1179,8 +1121,6 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                             sh.win_a[w + rank] = make_float4(ax, ay, az, 1.0f);
                             sh.win_n2[w + rank] = n2;
                             sh.win_js[w + rank] = (unsigned)(cb + lane) | sidx;
-                            if (FP32)
-                                sh.win_off[w + rank] = (unsigned short)((a + 2) | (b + 2) << 3 | (c + 2) << 6);
                         }
                         w += take;
                         keep_l = keep_l && rank >= take;
@@ -1231,7 +1171,7 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item_half<FP32>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
+        realspace_item_half<FP32, !FP32>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
                                   reach, L, rcut, alpha, stride, fs, sh, en_tot);
     }
     if (energy != nullptr) {
@@ -1577,7 +1517,7 @@ k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, 
         if (!(S.rcut > 0.0) || S.small)
             continue; // no real-space part, or nearest images (k_brealspace)
         double en = 0.0;
-        realspace_item_half<FP32>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
+        realspace_item_half<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
                                   istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fs, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
