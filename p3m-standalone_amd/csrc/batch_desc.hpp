@@ -81,10 +81,9 @@ struct CellArrays {
     int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
     int *perm = nullptr, *scan = nullptr, *counter = nullptr;
     double4 *sorted = nullptr;
-    // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, 1) and |x|^2;
-    // the three sorted arrays carry 64 padding entries past the last particle
+    // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, q); the sorted
+    // arrays carry 64 padding entries past the last particle
     float4 *sorted32 = nullptr;
-    float *norm2 = nullptr;
     double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
     double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n
     float4 *bbox = nullptr;     // 2 per work item

@@ -49,14 +49,15 @@ namespace {
 using p3m::CellArrays;
 
 constexpr int kWave = 64;
-// at most 63 pending pairs plus the appends of one 16 x 16 tile (4 x 64)
-constexpr int kQueue = 5 * kWave;
+// full-shell kernels (nearest images, adjoint): at most 63 pending pairs plus
+// one append of the vector search
+constexpr int kQueue = 2 * kWave;
 constexpr int kMaxCellsPerSide = 512;
-// sorted arrays carry kPad entries past the last particle: a 16-row tile of
-// a chunk's last rows may read up to 15 rows beyond it
+// sorted arrays carry kPad zero-charge entries past the last particle
 constexpr int kPad = 64;
 // home cells of more particles than this use the matrix-core candidate search
-// (FP32 form); sparser ones the vector search, whose per-cell cost is lower
+// of the half shell; sparser ones the vector search, whose per-cell cost is
+// lower
 constexpr int kMfmaMinCell = 4;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -81,26 +82,23 @@ __device__ __forceinline__ int fine_key(double xf, double yf, double zf, double 
 }
 
 // Sorted copies of one particle: FP64 (x, y, z, q), and relative to the
-// centre of its cell (cx, cy, cz) in FP32 the matrix row (-2 x, -2 y, -2 z, 1)
-// and |x|^2 of the candidate search.
+// centre of its cell (cx, cy, cz) in FP32 the row (-2 x, -2 y, -2 z, q) the
+// candidate search builds its matrix rows from.
 __device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int nc, double side,
                                              double4 *__restrict__ sorted,
-                                             float4 *__restrict__ sorted32,
-                                             float *__restrict__ norm2) {
+                                             float4 *__restrict__ sorted32) {
     const int cz = cell % nc, cy = (cell / nc) % nc, cx = cell / (nc * nc);
     const float lx = (float)(r.x - (cx + 0.5) * side), ly = (float)(r.y - (cy + 0.5) * side),
                 lz = (float)(r.z - (cz + 0.5) * side);
     sorted[pos] = r;
     sorted32[pos] = make_float4(-2.0f * lx, -2.0f * ly, -2.0f * lz, (float)r.w);
-    norm2[pos] = lx * lx + ly * ly + lz * lz;
 }
 
-// the kPad entries past the last particle: finite rows that never pass
+// the kPad entries past the last particle: finite rows of zero charge
 __device__ __forceinline__ void store_pad(int pos, double4 *__restrict__ sorted,
-                                          float4 *__restrict__ sorted32, float *__restrict__ norm2) {
+                                          float4 *__restrict__ sorted32) {
     sorted[pos] = make_double4(0.0, 0.0, 0.0, 0.0);
     sorted32[pos] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    norm2[pos] = 1e30f;
 }
 
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
@@ -121,13 +119,13 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
                                int nc, const int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
-                               float *__restrict__ norm2, const double *__restrict__ g,
+                               const double *__restrict__ g,
                                double4 *__restrict__ gsorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n + kPad)
         return;
     if (i >= n) {
-        store_pad(i, sorted, sorted32, norm2);
+        store_pad(i, sorted, sorted32);
         if (g != nullptr)
             gsorted[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
@@ -136,7 +134,7 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32, norm2);
+    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32);
     if (g != nullptr)
         gsorted[pos] = make_double4(g[i], g[n + i], g[2 * (long)n + i], 0.0);
 }
@@ -482,8 +480,7 @@ __device__ __forceinline__ void scan_chunk_scalar(Append &append, Flush &flush,
 template <bool FP32, bool SMALL_BOX, bool VJP = false>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-               const float4 *__restrict__ sorted32, const float *__restrict__ norm2,
-               const int *__restrict__ perm, const int *__restrict__ start,
+               const float4 *__restrict__ sorted32, const int *__restrict__ perm, const int *__restrict__ start,
                const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
                RsShared<FP32, VJP> &sh, double &en_tot, const double4 *__restrict__ gsorted = nullptr,
@@ -678,7 +675,7 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
 k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-            const float *__restrict__ norm2, const int *__restrict__ perm,
+            const int *__restrict__ perm,
             const int *__restrict__ start, const int *__restrict__ istart,
             const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L, double rcut,
             double alpha, int *__restrict__ counter, double *__restrict__ fr,
@@ -695,7 +692,7 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, norm2, perm,
+        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, perm,
                                         start, istart, bbox, nc, reach, L, rcut, alpha, n, fr, sh,
                                         en_tot);
     }
@@ -712,7 +709,7 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
 template <bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave)
 k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-                const float *__restrict__ norm2, const int *__restrict__ perm,
+                const int *__restrict__ perm,
                 const int *__restrict__ start, const int *__restrict__ istart,
                 const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L,
                 double rcut, double alpha, int *__restrict__ counter,
@@ -730,7 +727,7 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<false, SMALL_BOX, true>(home, item - istart[home], 0, sorted, sorted32, norm2,
+        realspace_item<false, SMALL_BOX, true>(home, item - istart[home], 0, sorted, sorted32,
                                                perm, start, istart, bbox, nc, reach, L, rcut, alpha,
                                                n, dpos, sh, en_unused, gsorted, dq);
     }
@@ -1202,7 +1199,7 @@ namespace p3m {
 
 void cells_release(CellArrays &a) {
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.sorted32, a.norm2, a.bbox, a.gsorted, a.fsorted};
+                    a.scan, a.counter, a.sorted, a.sorted32, a.bbox, a.gsorted, a.fsorted};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -1226,7 +1223,6 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     const size_t np = nn + kPad;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * np);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.norm2, sizeof(float) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fsorted, sizeof(double) * 3 * nn);
     if (rc != P3MK_OK)
@@ -1335,7 +1331,7 @@ static int build_cells(p3mk_cells *c, int n, double L, const double *x, const do
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.norm2, g, a.gsorted);
+                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, g, a.gsorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     return p3m::cells_end(a, c->ncell, qu);
 }
@@ -1354,11 +1350,11 @@ extern "C" int p3mk_realspace_vjp(p3mk_cells *c, int n, double L, const double *
     hipStream_t s = p3mk_stream(qu);
     if (all_cells)
         hipLaunchKernelGGL(k_realspace_vjp<true>, grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,
-                           a.norm2, a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                           a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
                            rcut, alpha, a.counter, a.gsorted, dpos, dq);
     else
         hipLaunchKernelGGL(k_realspace_vjp<false>, grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,
-                           a.norm2, a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                           a.perm, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
                            rcut, alpha, a.counter, a.gsorted, dpos, dq);
     return p3mk_launch_check("real-space adjoint");
 }
@@ -1392,7 +1388,7 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
     P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
 #define LAUNCH(F32, SB)                                                                            \
     hipLaunchKernelGGL((k_realspace<F32, SB>), grid, dim3(kWave), 0, s, n, a.sorted, a.sorted32,   \
-                       a.norm2, a.perm,                                                            \
+                       a.perm,                                                                     \
                        a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
                        a.counter, fr, energy)
     if (fp32)
@@ -1431,13 +1427,12 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
                                 const double *__restrict__ z, const double *__restrict__ q,
                                 const int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
-                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
-                                float *__restrict__ norm2) {
+                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot + kPad)
         return;
     if (i >= ntot) {
-        store_pad(i, sorted, sorted32, norm2);
+        store_pad(i, sorted, sorted32);
         return;
     }
     const BSys &S = sys[psys[i]];
@@ -1446,14 +1441,14 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32, norm2);
+    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32);
 }
 
 template <bool FP32>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
 k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
              const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-             const float *__restrict__ norm2, const int *__restrict__ perm,
+             const int *__restrict__ perm,
              const int *__restrict__ start, const int *__restrict__ istart,
              const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
              double *__restrict__ fr, double *__restrict__ energy) {
@@ -1473,7 +1468,7 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
         if (!(S.rcut > 0.0) || !S.small)
             continue; // no real-space part, or a cell grid (k_brealspace_half)
         double en = 0.0;
-        realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, norm2,
+        realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
                                    perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot,
                                    fr, sh, en);
         if (energy != nullptr) {
@@ -1546,19 +1541,18 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
     P3MK_TRY(p3mk_launch_check("batched cell count"));
     P3MK_TRY(cells_scan(a, ncell, qu));
     hipLaunchKernelGGL(k_bcell_scatter, dim3((ntot + kPad + 255) / 256), dim3(256), 0, s, ntot, sys,
-                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32,
-                       a.norm2);
+                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
     if (any_small) {
         // nearest-image systems: forces straight to fr
         if (f.fp32)
             hipLaunchKernelGGL(k_brealspace<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                               nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart,
+                               nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart,
                                a.bbox, ncell, a.counter, fr, energy);
         else
             hipLaunchKernelGGL(k_brealspace<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys, cell0s,
-                               nsys, ntot, a.sorted, a.sorted32, a.norm2, a.perm, a.start, a.istart,
+                               nsys, ntot, a.sorted, a.sorted32, a.perm, a.start, a.istart,
                                a.bbox, ncell, a.counter, fr, energy);
         P3MK_TRY(p3mk_launch_check("batched real space (nearest images)"));
     }
