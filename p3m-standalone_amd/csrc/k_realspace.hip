@@ -215,7 +215,12 @@ constexpr int kErfcxTab = p3m::kErfcxDoublePieces * (p3m::kErfcxDoubleDegree + 1
 // erfcx(x) = exp(x^2) erfc(x), x >= 0, from the generated piecewise FP64
 // tables (a copy in LDS: the piece index differs from lane to lane)
 __device__ __forceinline__ double erfcx_d(double x, const double *tab) {
-    const double y = 1.0 / (1.0 + 0.5 * x);
+    // y = 1 / (1 + x / 2): hardware reciprocal refined by two Newton steps
+    // (the piece index and argument need y to a few ulp, not correctly rounded)
+    const double den = fma(0.5, x, 1.0);
+    double y = __builtin_amdgcn_rcp(den);
+    y = fma(fma(-den, y, 1.0), y, y);
+    y = fma(fma(-den, y, 1.0), y, y);
     const int K = p3m::kErfcxDoublePieces;
     int k = (int)(y * K);
     k = k > K - 1 ? K - 1 : k;
