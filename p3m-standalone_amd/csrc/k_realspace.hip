@@ -15,23 +15,26 @@
 // and neighbour cells and neighbour chunks whose box is beyond rcut from the
 // item's box are skipped.
 //
-// Candidate search on the matrix cores: for a home chunk and a neighbour
-// chunk, r^2 - rcut^2 of all 64 x 64 pairs is one K = 4 product per 16 x 16
-// tile,
-//   [-2 n, 1] . [h', |h'|^2 - rcut^2] + |n|^2 = |h' - n|^2 - rcut^2,
-// with n the neighbour relative to its cell centre and h' the home particle
-// relative to the same centre (FP32, |error| < 5e-6 rcut^2 for pairs near
-// the cutoff; the test keeps a 2e-5 margin).  Accepted pairs of a tile go to
-// one queue per wavefront (rank by mbcnt over the ballot); every 64 queued
-// pairs are evaluated by the 64 lanes at once, whoever owns them, with the
-// exact FP64 difference and the reference's inclusion test, and the forces
-// are added to the owners' LDS accumulators.
+// Two kernels walk this layout.
+//  - Cell grids (at least 2 R + 1 cells per side): the half shell
+//    (k_realspace_half, further down), each unordered pair once with its
+//    reaction added to the partner; neighbour particles are staged into a
+//    64-slot LDS window and paired with the item's home particles on the
+//    matrix cores, K = 4 products of
+//      [-2 n', 1] . [h, |h|^2 - rcut^2] + |n'|^2 = |h - n'|^2 - rcut^2
+//    with n' and h relative to the home cell centre (FP32; the test keeps a
+//    2e-5 margin).
+//  - Boxes of fewer cells per side (nearest image per pair) and the adjoint:
+//    the full shell (k_realspace, k_realspace_vjp), every ordered pair for
+//    its home particle, candidates found by the vector ALUs.
+// Both queue the accepted pairs in LDS (rank by mbcnt over the ballot) and
+// evaluate every 64 queued pairs with the 64 lanes at once, whoever owns
+// them, with the exact FP64 difference and the reference's inclusion test.
 //
 // Periodic images are a per-neighbour-cell shift of 0 or +-L, which equals
 // the reference's round(dx/L)*L for every pair inside the cutoff
 // (rcut <= L/2), so dx = (x_i - x_j) - shift is the reference's dx bit for
-// bit.  (Newton's third law was tried and measured slower here: the
-// partner forces need global atomics per neighbour chunk.)
+// bit.
 
 #include <hip/hip_runtime.h>
 
@@ -472,8 +475,9 @@ __device__ __forceinline__ void scan_chunk_scalar(Append &append, Flush &flush,
         test(sorted32[cb + k], k);
 }
 
-// One work item (home cell, chunk of up to 64 home particles); every
-// ordered pair is evaluated for its home particle, by lane = owner.
+// One work item of the full shell (home cell, chunk of up to 64 home
+// particles; boxes of fewer than 2 reach + 1 cells per side and the
+// adjoint); every ordered pair is evaluated for its home particle.
 // Neighbour cells are the (2 reach + 1)^3 cells around the home cell, or,
 // in a box of fewer cells per side (SMALL_BOX), every cell once with the
 // nearest image taken per pair.  Neighbour cells and neighbour chunks whose
@@ -485,8 +489,8 @@ __device__ __forceinline__ void scan_chunk_scalar(Append &append, Flush &flush,
 template <bool FP32, bool SMALL_BOX, bool VJP = false>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-               const float4 *__restrict__ sorted32, const int *__restrict__ perm, const int *__restrict__ start,
-               const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
+               const float4 *__restrict__ sorted32, const int *__restrict__ perm,
+               const int *__restrict__ start, const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
                double L, double rcut, double alpha, long stride, double *__restrict__ fr,
                RsShared<FP32, VJP> &sh, double &en_tot, const double4 *__restrict__ gsorted = nullptr,
                double *__restrict__ dq = nullptr) {
@@ -672,11 +676,9 @@ realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorte
     __syncthreads();
 }
 
-// Persistent grid: each single-wavefront workgroup claims work items from an
-// atomic counter until none are left.  FP32 form: five waves per SIMD (at
-// most 102 VGPRs), measured faster than the unconstrained four and than
-// six, which spills; the FP64 form is left to the compiler (a four-wave
-// hint gave fewer registers but a slower kernel).
+// Persistent grid of the full shell (launched for boxes of fewer than
+// 2 reach + 1 cells per side): each single-wavefront workgroup claims work
+// items from an atomic counter until none are left.
 template <bool FP32, bool SMALL_BOX>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 1)))
 k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
