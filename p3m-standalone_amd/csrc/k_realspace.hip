@@ -490,8 +490,9 @@ template <bool FP32, bool SMALL_BOX, bool VJP = false>
 __device__ __forceinline__ void
 realspace_item(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
                const float4 *__restrict__ sorted32, const int *__restrict__ perm,
-               const int *__restrict__ start, const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc, int reach,
-               double L, double rcut, double alpha, long stride, double *__restrict__ fr,
+               const int *__restrict__ start, const int *__restrict__ istart,
+               const float4 *__restrict__ bbox, int nc, int reach, double L, double rcut,
+               double alpha, long stride, double *__restrict__ fr,
                RsShared<FP32, VJP> &sh, double &en_tot, const double4 *__restrict__ gsorted = nullptr,
                double *__restrict__ dq = nullptr) {
     static_assert(!(VJP && FP32), "the adjoint is FP64");
