@@ -64,6 +64,7 @@ constexpr int kPad = 64;
 constexpr int kMfmaMinCell = 4;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double fold(double r, double L) {
     // into [0, L); identity for positions already inside the box
@@ -778,7 +779,7 @@ template <bool FP32> struct RsHalfShared {
     double4 shift[27];
     double facc[3][kWave];
     double facc_nb[3][kWave];    // the window particles' share, f summed (the reaction is -f)
-    float4 win_a[kWave];         // (-2 n', 1), n' relative to the home cell centre
+    halfx4 win_a[kWave];         // (-2 n, 1) in FP16, n = n' / side, n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
     double tab[FP32 ? 1 : kErfcxTab];
@@ -796,7 +797,6 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     const double side = L / nc;
     const float rskip = (float)(rcut * (1.0 + 1e-5) + 1e-5 * L);
     const float rskip2 = rskip * rskip;
-    const float rc2f = (float)(rcut * rcut * (1.0 + 2e-5));
     const int hb = start[cell0 + home], he = start[cell0 + home + 1];
     const int hz = home % nc, hy = (home / nc) % nc, hx = home / (nc * nc);
     const int i0 = hb + chunk * kWave;
@@ -808,7 +808,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
         sh.facc[d][lane] = 0.0;
         sh.facc_nb[d][lane] = 0.0;
     }
-    sh.win_a[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    sh.win_a[lane] = halfx4{(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)1.0f};
     sh.win_n2[lane] = 1e30f;
     sh.win_js[lane] = 0u;
     if (lane < 27)
@@ -821,14 +821,21 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     const int hidx = tiles ? 4 * ((lane >> 2) & (G - 1)) + (lane & 3) : lane;
     const bool hval = hidx < nh;
     const unsigned int i_own = (unsigned)(i0 + hidx);
-    float4 bop = make_float4(0.0f, 0.0f, 0.0f, 1e30f);
-    float hxf = 2e30f, hyf = 2e30f, hzf = 2e30f; // vector search: far away unless valid
+    // candidate test in units of the cell side, FP16 operands: coordinates
+    // within 2.5 sides carry at most 1.3e-3, so |h - n|^2 is within 1.3e-2
+    // side^2 of its value; the test passes pairs up to rcut^2 + 0.02 side^2
+    const float inv_side = (float)(1.0 / side);
+    const float rho2m = (float)(rcut * rcut / (side * side)) + 0.02f;
+    halfx4 bop = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)30000.0f};
+    float hxf = 1e4f, hyf = 1e4f, hzf = 1e4f; // vector search: far away unless valid
     if (hval) {
         const float4 v = sorted32[i_own];
-        hxf = -0.5f * v.x;
-        hyf = -0.5f * v.y;
-        hzf = -0.5f * v.z;
-        bop = make_float4(hxf, hyf, hzf, hxf * hxf + hyf * hyf + hzf * hzf - rc2f);
+        const _Float16 hx16 = (_Float16)(-0.5f * v.x * inv_side), hy16 = (_Float16)(-0.5f * v.y * inv_side),
+                       hz16 = (_Float16)(-0.5f * v.z * inv_side);
+        hxf = (float)hx16;
+        hyf = (float)hy16;
+        hzf = (float)hz16;
+        bop = halfx4{hx16, hy16, hz16, (_Float16)(hxf * hxf + hyf * hyf + hzf * hzf - rho2m)};
     }
     int count = 0;        // wave-uniform queue length
     int w = 0;            // filled window slots
@@ -971,19 +978,13 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 };
                 for (int s = 0; s < G; s += 2) {
                     const int base0 = ((lane & ~3) + 4 * s) & 63, base1 = (base0 + 4) & 63;
-                    const float4 a0 = sh.win_a[(lane + 4 * s) & 63];
-                    const float4 a1 = sh.win_a[(lane + 4 * s + 4) & 63];
+                    const halfx4 a0 = sh.win_a[(lane + 4 * s) & 63];
+                    const halfx4 a1 = sh.win_a[(lane + 4 * s + 4) & 63];
                     const float4 c0 = *reinterpret_cast<const float4 *>(&sh.win_n2[base0]);
                     const float4 c1 = *reinterpret_cast<const float4 *>(&sh.win_n2[base1]);
                     floatx4 d0 = {c0.x, c0.y, c0.z, c0.w}, d1 = {c1.x, c1.y, c1.z, c1.w};
-                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bop.x, d0, 0, 0, 0);
-                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bop.x, d1, 0, 0, 0);
-                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, bop.y, d0, 0, 0, 0);
-                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, bop.y, d1, 0, 0, 0);
-                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.z, bop.z, d0, 0, 0, 0);
-                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.z, bop.z, d1, 0, 0, 0);
-                    d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.w, bop.w, d0, 0, 0, 0);
-                    d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.w, bop.w, d1, 0, 0, 0);
+                    d0 = __builtin_amdgcn_mfma_f32_4x4x4f16(a0, bop, d0, 0, 0, 0);
+                    d1 = __builtin_amdgcn_mfma_f32_4x4x4f16(a1, bop, d1, 0, 0, 0);
                     emit(d0, base0);
                     emit(d1, base1);
                 }
@@ -994,10 +995,11 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 issues(std::false_type{});
         } else {
             for (int k = 0; k < w; k++) {
-                const float4 a = sh.win_a[k];
+                const halfx4 a = sh.win_a[k];
                 const unsigned int js = sh.win_js[k];
-                const float dx = fmaf(0.5f, a.x, hxf), dy = fmaf(0.5f, a.y, hyf), dz = fmaf(0.5f, a.z, hzf);
-                bool ok = dx * dx + dy * dy + dz * dz <= rc2f;
+                const float dx = fmaf(0.5f, (float)a.x, hxf), dy = fmaf(0.5f, (float)a.y, hyf),
+                            dz = fmaf(0.5f, (float)a.z, hzf);
+                bool ok = dx * dx + dy * dy + dz * dz <= rho2m;
                 if (selfwin)
                     ok = ok && keep(js, i_own);
                 append(ok, (unsigned)lane | (unsigned)k << 6);
@@ -1105,17 +1107,22 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     // the chunk's particles within rcut of the item's box (all of
                     // the own chunk), compacted into the window
                     bool keep_l = false;
-                    float ax = 0.0f, ay = 0.0f, az = 0.0f, n2 = 0.0f;
+                    halfx4 arow = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)1.0f};
+                    float n2 = 0.0f;
                     if (lane < cnt) {
                         const float4 v = sorted32[cb + lane];
-                        ax = v.x - ox2;
-                        ay = v.y - oy2;
-                        az = v.z - oz2;
+                        const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
                         const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
-                        n2 = nx * nx + ny * ny + nz * nz;
                         const float gxx = gap(hlo_r.x, hhi_r.x, nx, nx), gyy = gap(hlo_r.y, hhi_r.y, ny, ny),
                                     gzz = gap(hlo_r.z, hhi_r.z, nz, nz);
                         keep_l = gxx * gxx + gyy * gyy + gzz * gzz <= rskip2;
+                        // the matrix row in side units, rounded to FP16, and |n|^2
+                        // of the rounded row
+                        arow = halfx4{(_Float16)(ax * inv_side), (_Float16)(ay * inv_side),
+                                      (_Float16)(az * inv_side), (_Float16)1.0f};
+                        const float rx = -0.5f * (float)arow.x, ry = -0.5f * (float)arow.y,
+                                    rz = -0.5f * (float)arow.z;
+                        n2 = rx * rx + ry * ry + rz * rz;
                     }
                     unsigned long long mask = __ballot(keep_l);
                     while (mask) {
@@ -1123,7 +1130,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                         const int rank = (int)__builtin_amdgcn_mbcnt_hi(
                             (unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
                         if (keep_l && rank < take) {
-                            sh.win_a[w + rank] = make_float4(ax, ay, az, 1.0f);
+                            sh.win_a[w + rank] = arow;
                             sh.win_n2[w + rank] = n2;
                             sh.win_js[w + rank] = (unsigned)(cb + lane) | sidx;
                         }

@@ -118,3 +118,27 @@ def test_device_form_accumulates_and_mixed_is_close(pkg, orc, prec_name):
     else:
         assert err < 1e-6
         assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
+
+
+def test_pairs_at_the_cutoff(pkg, orc):
+    """Pairs placed within 1e-9 of rcut on either side, in all directions and
+    across cell and box boundaries: the candidate search (FP16 operands in
+    units of the cell side, with a margin) passes all of them and the exact
+    FP64 test keeps exactly the reference's ones."""
+    L, rcut, alpha = 12.0, 3.0, 0.8
+    rng = np.random.default_rng(77)
+    m = 400
+    a = rng.uniform(0.0, L, (m, 3))
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    eps = np.where(np.arange(m) % 2 == 0, -1e-9, 1e-9)
+    b = a + (rcut * (1.0 + eps))[:, None] * u
+    p = np.mod(np.concatenate([a, b]), L)
+    p[p >= L] = 0.0
+    q = np.concatenate([np.ones(m), -np.ones(m)])
+    s = {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": L}
+    system = pkg.HostSystem.from_dict(s)
+    forces = pkg.realteil(system, rcut, alpha)
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    assert rel_err(forces.real(), fr_ref) < 1e-13
+    assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
