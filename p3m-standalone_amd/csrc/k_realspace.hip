@@ -920,11 +920,11 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
             count += __popcll(mask);
         }
     };
-    // PAIRED_ROUNDS: two rounds at a time (the FP64 form of the single-system
-    // kernel: 11.4 ms against 12.0 at the Gaussian 1M workload); the FP32
-    // form has no registers to spare for the second round's data (8.5 ms
-    // against 7.5 with it, from spills), nor the batched FP64 kernel, which
-    // is held to four waves
+    // PAIRED_ROUNDS: two rounds at a time (single-system kernel: FP64
+    // 11.4 ms against 12.0 at the Gaussian 1M workload, FP32 at four waves
+    // per SIMD 7.07 ms against 7.35 at five with single rounds); the batched
+    // FP64 kernel, held to 128 registers, has no room for the second round's
+    // data
     auto flush = [&]() {
         if constexpr (!PAIRED_ROUNDS) {
             if (count >= kWave) {
@@ -1157,11 +1157,13 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
 }
 
 // Persistent grid as k_realspace over the half shell; fs (3 n, sorted order)
-// is zeroed by the caller and added to the forces by k_unsort_add.  FP32:
-// five waves per SIMD; FP64: three (12.0 ms against 13.3 unconstrained and
-// 12.0 at four, at the Gaussian 1M workload).
+// is zeroed by the caller and added to the forces by k_unsort_add.  Queue
+// rounds are evaluated two at a time in both forms; FP32: four waves per
+// SIMD (7.07 ms at the Gaussian 1M workload; five with single rounds 7.35,
+// five with paired rounds 8.06 from spills); FP64: three (12.0 ms against
+// 13.3 unconstrained and 12.0 at four).
 template <bool FP32>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 3)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 4 : 3)))
 k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
                  const int *__restrict__ start, const int *__restrict__ istart,
                  const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L,
@@ -1183,7 +1185,7 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
         if (item >= total)
             break;
         const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item_half<FP32, !FP32>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
+        realspace_item_half<FP32, true>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
                                   reach, L, rcut, alpha, stride, fs, sh, en_tot);
     }
     if (energy != nullptr) {
@@ -1497,11 +1499,11 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
 
 // Cell-grid systems of a batch over the half shell: forces to fs (3 ntot,
 // sorted order, zeroed by the caller), energy[s] += system s's energy.
-// FP64 form held to four waves per SIMD: the batch's sparse cells are
-// latency bound (measured 21.1 ms against 29.1 ms unconstrained at the
-// 256-system batch; the single-system kernel is faster unconstrained).
+// Four waves per SIMD: the batch's sparse cells are latency bound (FP64:
+// 21.1 ms against 29.1 ms unconstrained at the 256-system batch); the FP32
+// form evaluates its queue rounds two at a time, the FP64 form one.
 template <bool FP32>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 5 : 4)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
                   const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
                   const int *__restrict__ start, const int *__restrict__ istart,
@@ -1527,7 +1529,7 @@ k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, 
         if (!(S.rcut > 0.0) || S.small)
             continue; // no real-space part, or nearest images (k_brealspace)
         double en = 0.0;
-        realspace_item_half<FP32, false>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
+        realspace_item_half<FP32, FP32>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
                                   istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fs, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
