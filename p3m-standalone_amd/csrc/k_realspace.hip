@@ -754,14 +754,15 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // for bit (realpart.c:134-148).  Forces are summed in sorted order
 // (fs[d stride + i]) and added to the caller's array by perm afterwards.
 //
-// Neighbour particles are staged in a window of 64 slots (several chunks, of
-// several cells), in FP32 relative to the HOME cell centre; the reactions on
-// them are summed per slot in LDS.  The candidate search of a dense item
-// pairs the window with its home particles on the matrix cores in 4 x 4
-// blocks (v_mfma_f32_4x4x1f32: lane l = 4 b + col of block b holds A[row =
-// l % 4], B[col = l % 4], and result register r is D[row r][col l % 4]),
-// K = 4 as four chained products of A = (-2 n', 1) and B = (h, |h|^2 -
-// rc2f), C = |n'|^2.  Issue s pairs block b with home group b mod G (G the
+// Neighbour particles within rcut of the item's box are staged in a window
+// of 64 slots (several chunks, of several cells), relative to the HOME cell
+// centre in units of the cell side; the reactions on them are summed per
+// slot in LDS.  The candidate search of a dense item pairs the window with
+// its home particles on the matrix cores in 4 x 4 blocks
+// (v_mfma_f32_4x4x4f16: lane l = 4 b + col of block b holds row l % 4 of A
+// and column col of B, K = 4 each, and result register r is D[row r][col]),
+// one product of A = (-2 n, 1) and B = (h, |h|^2 - rho^2 - margin) in FP16
+// with C = |n|^2 in FP32.  Issue s pairs block b with home group b mod G (G the
 // item's group count rounded up to a power of two) and window group (b + s)
 // mod 16, and each lane queues its results in the rotated order (col + q)
 // mod 4: the entries of one append then have distinct window slots and, for
@@ -771,7 +772,7 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // several times slower in the reaction sums.
 
 constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
-constexpr int kHalfQueue = 3 * kWave;              // 127 pending (FP64 form) + one append
+constexpr int kHalfQueue = 3 * kWave;              // 127 pending (paired rounds) + one append
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
 
 template <bool FP32> struct RsHalfShared {
