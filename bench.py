@@ -255,8 +255,9 @@ def roofline_object(dom, per, workload):
                 "frac": st["frac_valu"], "traffic": traffic,
                 "note": "compute bound (vector ALUs and LDS atomics); achieved = %d ordered pairs "
                         "within rcut / 2 (each unordered pair once) x %d flops / median stage time; "
-                        "traffic = HBM bytes per launch from profiles/traffic_%s.md"
-                        % (st["pairs"], HALF_PAIR_FLOPS, workload)}
+                        "peak = the published FP32 figure, which counts packed FP32 (built without "
+                        "packed FP32 here: 78.6 TFLOP/s unpacked); traffic = HBM bytes per launch "
+                        "from profiles/traffic_%s.md" % (st["pairs"], HALF_PAIR_FLOPS, workload)}
     return {"bound": "hbm", "kernel": kernel, "achieved": st["GB/s"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": st["frac_hbm"], "traffic": traffic}
 
