@@ -76,7 +76,7 @@ int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, 
 constexpr long kCellMaxParticles = (1L << 27) - 64;
 
 struct CellArrays {
-    int *key = nullptr;                               // fine key per particle
+    int *key = nullptr;                               // fine key per particle, then its sorted position
     int *fcount = nullptr, *fstart = nullptr, *fcursor = nullptr; // 8 ncell + 1
     int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
     int *perm = nullptr, *scan = nullptr, *counter = nullptr;

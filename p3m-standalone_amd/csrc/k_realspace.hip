@@ -116,11 +116,12 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
     atomicAdd(&count[k], 1);
 }
 
-// n + kPad threads; with g (3n, x/y/z blocks) also gsorted[pos] = (g_x, g_y,
+// n + kPad threads; key[i] is replaced by the particle's sorted position (the
+// inverse of perm); with g (3n, x/y/z blocks) also gsorted[pos] = (g_x, g_y,
 // g_z, 0) of the particle at pos (the adjoint's cotangent in sorted order)
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
-                               int nc, const int *__restrict__ key, const int *__restrict__ start,
+                               int nc, int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
                                const double *__restrict__ g,
@@ -137,6 +138,7 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
+    key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
     store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32);
     if (g != nullptr)
@@ -1197,17 +1199,18 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
     }
 }
 
-// fr[d stride + perm[i]] += fs[d n + i]: sorted-order forces to the caller's
-// order (perm is a permutation, so every thread owns its three entries)
-__global__ void k_unsort_add(int n, const int *__restrict__ perm, const double *__restrict__ fs,
+// fr[d stride + i] += fs[d n + pos[i]]: sorted-order forces to the caller's
+// order, read through the inverse permutation so that the caller's array is
+// updated in order
+__global__ void k_unsort_add(int n, const int *__restrict__ pos, const double *__restrict__ fs,
                              long stride, double *__restrict__ fr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int p = perm[i];
-    fr[p] += fs[i];
-    fr[stride + p] += fs[(long)n + i];
-    fr[2 * stride + p] += fs[2 * (long)n + i];
+    const int p = pos[i];
+    fr[i] += fs[p];
+    fr[stride + i] += fs[(long)n + p];
+    fr[2 * stride + i] += fs[2 * (long)n + p];
 }
 
 }  // namespace
@@ -1399,7 +1402,7 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
                                a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
                                a.counter, (long)n, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("realspace (half shell)"));
-        hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a.perm,
+        hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a.key,
                            a.fsorted, (long)n, fr);
         return p3mk_launch_check("realspace forces to caller order");
     }
@@ -1440,10 +1443,11 @@ __global__ void k_bcell_count(int ntot, const BSys *__restrict__ sys, const int 
     atomicAdd(&count[k], 1);
 }
 
+// as k_cell_scatter: key[i] becomes the particle's sorted position
 __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
                                 const double *__restrict__ x, const double *__restrict__ y,
                                 const double *__restrict__ z, const double *__restrict__ q,
-                                const int *__restrict__ key, const int *__restrict__ start,
+                                int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
                                 double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1458,6 +1462,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     const int k = key[i];
     const int pos = start[k] + atomicAdd(&cursor[k], 1);
     perm[pos] = i;
+    key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
     store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32);
 }
@@ -1587,7 +1592,7 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
                                a.counter + 1, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("batched real space (half shell)"));
         hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, ntot,
-                           a.perm, a.fsorted, (long)ntot, fr);
+                           a.key, a.fsorted, (long)ntot, fr);
         P3MK_TRY(p3mk_launch_check("batched real-space forces to caller order"));
     }
     return P3MK_OK;
