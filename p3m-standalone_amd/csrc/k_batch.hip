@@ -77,20 +77,6 @@ void release_all(p3mk_batch *b) {
     cells_release(b->cells);
 }
 
-// cells of side >= rcut / reach, as the single-system cell list
-void cell_geometry(double L, double rcut, int &nc, int &reach) {
-    reach = 2;
-    nc = (int)std::floor(L * reach / rcut * (1.0 - 1e-12));
-    if (nc < 2 * reach + 1) {
-        reach = 1;
-        nc = (int)std::floor(L / rcut * (1.0 - 1e-12));
-    }
-    if (nc < 1)
-        nc = 1;
-    if (nc > 512)
-        nc = 512;
-}
-
 // Build the tile-dependent descriptors (tile side T per system) and upload
 // every descriptor array.
 int build_descriptors(p3mk_batch *b) {
@@ -231,7 +217,7 @@ extern "C" int p3mk_batch_create(p3mk_batch **out, int r2c, int interlaced, int 
         mesh_off += align256(p3mk_mesh_bytes(&g64));
         int nc = 1, reach = 1;
         if (I.rcut > 0.0)
-            cell_geometry(I.L, I.rcut, nc, reach);
+            cell_geometry(I.n, I.L, I.rcut, nc, reach);
         S.nc = nc;
         S.reach = reach;
         S.small = (2 * reach + 1 > nc) ? 1 : 0;

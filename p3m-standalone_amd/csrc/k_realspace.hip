@@ -55,7 +55,6 @@ constexpr int kWave = 64;
 // full-shell kernels (nearest images, adjoint): at most 63 pending pairs plus
 // one append of the vector search
 constexpr int kQueue = 2 * kWave;
-constexpr int kMaxCellsPerSide = 512;
 // sorted arrays carry kPad zero-charge entries past the last particle
 constexpr int kPad = 64;
 // home cells of more particles than this use the matrix-core candidate search
@@ -780,7 +779,7 @@ constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
 template <bool FP32> struct RsHalfShared {
     unsigned int qs[kHalfQueue]; // owner | window slot << 6
     double4 shift[27];
-    double facc[3][kWave];
+    double facc[3][kWave];       // the home particles' forces, each in kWave / owner-slots replicas
     double facc_nb[3][kWave];    // the window particles' share, f summed (the reaction is -f)
     halfx4 win_a[kWave];         // (-2 n, 1) in FP16, n = n' / side, n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
@@ -824,6 +823,13 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     const int hidx = tiles ? 4 * ((lane >> 2) & (G - 1)) + (lane & 3) : lane;
     const bool hval = hidx < nh;
     const unsigned int i_own = (unsigned)(i0 + hidx);
+    // owner o of the lane evaluating a pair adds to facc[.][o + oslots r],
+    // r the lane's low bits: a round holds up to 64 / oslots pairs per owner
+    // (16 with the four home particles of the vector search), which the
+    // replicas spread over distinct LDS words
+    const int oslots = tiles ? 4 * G : 4;
+    const int orep = kWave / oslots;
+    const int oofs = oslots * (lane & (orep - 1));
     // candidate test in units of the cell side, FP16 operands: coordinates
     // within 2.5 sides carry at most 1.3e-3, so |h - n|^2 is within 1.3e-2
     // side^2 of its value; the test passes pairs up to rcut^2 + 0.02 side^2
@@ -874,9 +880,9 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
         }
         PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
         pair<FP32>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
-        atomicAdd(&sh.facc[0][o], pa.f0);
-        atomicAdd(&sh.facc[1][o], pa.f1);
-        atomicAdd(&sh.facc[2][o], pa.f2);
+        atomicAdd(&sh.facc[0][o + oofs], pa.f0);
+        atomicAdd(&sh.facc[1][o + oofs], pa.f1);
+        atomicAdd(&sh.facc[2][o + oofs], pa.f2);
         atomicAdd(&sh.facc_nb[0][slot], pa.f0);
         atomicAdd(&sh.facc_nb[1][slot], pa.f1);
         atomicAdd(&sh.facc_nb[2][slot], pa.f2);
@@ -997,16 +1003,24 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
             else
                 issues(std::false_type{});
         } else {
-            for (int k = 0; k < w; k++) {
-                const halfx4 a = sh.win_a[k];
-                const unsigned int js = sh.win_js[k];
-                const float dx = fmaf(0.5f, (float)a.x, hxf), dy = fmaf(0.5f, (float)a.y, hyf),
-                            dz = fmaf(0.5f, (float)a.z, hzf);
-                bool ok = dx * dx + dy * dy + dz * dz <= rho2m;
+            // vector search (at most four home particles): the lanes take
+            // the window slots, pass r tests slot l against home particle
+            // (l + r) mod nh, so that an append has distinct slots and spreads
+            // evenly over the home particles
+            const halfx4 a = sh.win_a[lane];
+            const unsigned int js = sh.win_js[lane];
+            const bool filled = lane < w;
+            int o = lane % nh;
+            for (int r = 0; r < nh; r++) {
+                const float dx = fmaf(0.5f, (float)a.x, __shfl(hxf, o, kWave)),
+                            dy = fmaf(0.5f, (float)a.y, __shfl(hyf, o, kWave)),
+                            dz = fmaf(0.5f, (float)a.z, __shfl(hzf, o, kWave));
+                bool ok = filled && dx * dx + dy * dy + dz * dz <= rho2m;
                 if (selfwin)
-                    ok = ok && keep(js, i_own);
-                append(ok, (unsigned)lane | (unsigned)k << 6);
+                    ok = ok && keep(js, (unsigned)(i0 + o));
+                append(ok, (unsigned)o | (unsigned)lane << 6);
                 flush();
+                o = (o + 1 == nh) ? 0 : o + 1;
             }
         }
         __syncthreads();
@@ -1152,9 +1166,15 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     if (FP32)
         en_tot += (double)en32;
     if (active) {
-        atomicAdd(&fs[i], sh.facc[0][lane]);
-        atomicAdd(&fs[stride + i], sh.facc[1][lane]);
-        atomicAdd(&fs[2 * stride + i], sh.facc[2][lane]);
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        for (int r = 0; r < orep; r++) {
+            f0 += sh.facc[0][lane + oslots * r];
+            f1 += sh.facc[1][lane + oslots * r];
+            f2 += sh.facc[2][lane + oslots * r];
+        }
+        atomicAdd(&fs[i], f0);
+        atomicAdd(&fs[stride + i], f1);
+        atomicAdd(&fs[2 * stride + i], f2);
     }
     __syncthreads();
 }
@@ -1303,18 +1323,8 @@ extern "C" int p3mk_cells_create(p3mk_cells **out, int n, double L, double rcut)
     if (!c)
         return p3mk_fail(P3MK_ENOMEM, "cell list", 0);
     c->nmax = n;
-    // cells of side >= rcut/reach (with a margin, so that rounding of
-    // x*nc/L never puts a pair inside the cutoff more than `reach` cells apart)
-    int reach = 2;
-    int nc = (int)std::floor(L * reach / rcut * (1.0 - 1e-12));
-    if (nc < 2 * reach + 1) {
-        reach = 1;
-        nc = (int)std::floor(L / rcut * (1.0 - 1e-12));
-    }
-    if (nc < 1)
-        nc = 1;
-    if (nc > kMaxCellsPerSide)
-        nc = kMaxCellsPerSide;
+    int nc, reach;
+    p3m::cell_geometry(n, L, rcut, nc, reach);
     c->nc = nc;
     c->reach = reach;
     c->ncell = (long)nc * nc * nc;

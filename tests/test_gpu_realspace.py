@@ -1,8 +1,9 @@
 """Real space on cell grids (the half-shell kernel: each unordered pair once,
 the reaction added to the partner) against the oracle's Realteil
 (realpart.c:109-152): dense cells of several 64-particle chunks next to
-sparse ones, clusters that wrap across the periodic boundary, the device
-entry point accumulating into the caller's arrays, and the mixed form."""
+sparse ones, clusters that wrap across the periodic boundary, both cell
+sizes (side rcut / 2, and side rcut for sparse systems), the device entry
+point accumulating into the caller's arrays, and the mixed form."""
 import numpy as np
 import pytest
 
@@ -27,29 +28,45 @@ def corner_cluster(n_cluster, n_background, L, sigma, seed, centre=(0.0, 0.0, 0.
     return {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": float(L)}
 
 
+def cell_geometry(n, L, rcut):
+    """The library's cell grid: (cells per side, reach).  Cells of side
+    >= rcut / 2 (reach 2) unless those would hold fewer than 6 particles on
+    average, or the box has fewer than 5 of them per side; then side >= rcut
+    (reach 1)."""
+    nc2 = min(512, int(np.floor(L * 2 / rcut * (1 - 1e-12))))
+    nc1 = min(512, int(np.floor(L / rcut * (1 - 1e-12))))
+    if nc2 < 5 or (nc1 >= 3 and n < 6.0 * nc2 ** 3):
+        return max(nc1, 1), 1
+    return nc2, 2
+
+
 def cell_counts(s, rcut):
-    """Particles per cell of the library's grid (side >= rcut / 2)."""
+    """Particles per cell of the library's grid."""
     L = s["L"]
-    nc = int(np.floor(L * 2 / rcut * (1 - 1e-12)))
+    nc, reach = cell_geometry(len(s["q"]), L, rcut)
     c = np.minimum((np.stack([s["x"], s["y"], s["z"]], 1) * nc / L).astype(int), nc - 1)
-    return nc, np.bincount((c[:, 0] * nc + c[:, 1]) * nc + c[:, 2], minlength=nc ** 3)
+    return nc, reach, np.bincount((c[:, 0] * nc + c[:, 1]) * nc + c[:, 2], minlength=nc ** 3)
 
 
-@pytest.mark.parametrize("centre", [(0.0, 0.0, 0.0), (7.0, 0.3, 13.9)])
-def test_wrapped_dense_cluster(pkg, orc, centre):
+@pytest.mark.parametrize("L,n_background,centre,reach", [
+    (14.0, 2500, (0.0, 0.0, 0.0), 2), (14.0, 2500, (7.0, 0.3, 13.9), 2),
+    (30.0, 1000, (0.0, 0.0, 0.0), 1), (30.0, 1000, (29.5, 15.0, 0.2), 1)])
+def test_wrapped_dense_cluster(pkg, orc, L, n_background, centre, reach):
     """A cluster across the box corner / faces: cells of up to several
     hundred particles (several chunks; the candidate search on the matrix
     cores), sparse background cells (the vector search), image shifts inside
-    dense cell pairs."""
-    L, rcut, alpha = 14.0, 3.0, 1.0
-    s = corner_cluster(2500, 700, L, 0.9, seed=31, centre=centre)
-    nc, cnt = cell_counts(s, rcut)
-    assert nc >= 5 and cnt.max() > 128 and np.sum((cnt > 0) & (cnt <= 4)) > 50
+    dense cell pairs; with cells of side rcut / 2 and of side rcut."""
+    rcut, alpha = 3.0, 1.0
+    s = corner_cluster(2500 if reach == 2 else 2000, n_background, L, 0.9, seed=31, centre=centre)
+    nc, got_reach, cnt = cell_counts(s, rcut)
+    assert got_reach == reach
+    assert nc >= 2 * reach + 1 and cnt.max() > 128 and np.sum((cnt > 0) & (cnt <= 4)) > 50
     system = pkg.HostSystem.from_dict(s)
     forces = pkg.realteil(system, rcut, alpha)
     fr_ref, e_ref = orc.realteil(s, rcut, alpha)
     err = rel_err(forces.real(), fr_ref)
-    print("max cell %d, rel err %.2e, energy %.15e vs %.15e" % (cnt.max(), err, system.c.energy, e_ref))
+    print("reach %d, max cell %d, rel err %.2e, energy %.15e vs %.15e" % (
+        reach, cnt.max(), err, system.c.energy, e_ref))
     assert err < 1e-13
     assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
 
@@ -60,13 +77,14 @@ def test_single_dense_cell_of_many_chunks(pkg, orc):
     otherwise sparse box."""
     L, rcut, alpha = 12.0, 3.0, 1.2
     rng = np.random.default_rng(5)
-    side = L / int(np.floor(L * 2 / rcut * (1 - 1e-12)))
-    cell = rng.uniform(3 * side + 1e-3, 4 * side - 1e-3, (600, 3))
+    nc, _ = cell_geometry(900, L, rcut)
+    side = L / nc
+    cell = rng.uniform(1 * side + 1e-3, 2 * side - 1e-3, (600, 3))
     rest = rng.uniform(0.0, L, (300, 3))
     p = np.concatenate([cell, rest])
     q = np.where(np.arange(900) % 2 == 0, 1.0, -1.0)
     s = {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": L}
-    _, cnt = cell_counts(s, rcut)
+    _, _, cnt = cell_counts(s, rcut)
     assert cnt.max() >= 600
     system = pkg.HostSystem.from_dict(s)
     forces = pkg.realteil(system, rcut, alpha)
@@ -120,14 +138,15 @@ def test_device_form_accumulates_and_mixed_is_close(pkg, orc, prec_name):
         assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
 
 
-def test_pairs_at_the_cutoff(pkg, orc):
+@pytest.mark.parametrize("m,reach", [(400, 1), (1100, 2)])
+def test_pairs_at_the_cutoff(pkg, orc, m, reach):
     """Pairs placed within 1e-9 of rcut on either side, in all directions and
     across cell and box boundaries: the candidate search (FP16 operands in
     units of the cell side, with a margin) passes all of them and the exact
-    FP64 test keeps exactly the reference's ones."""
+    FP64 test keeps exactly the reference's ones; both cell sizes."""
     L, rcut, alpha = 12.0, 3.0, 0.8
+    assert cell_geometry(2 * m, L, rcut)[1] == reach
     rng = np.random.default_rng(77)
-    m = 400
     a = rng.uniform(0.0, L, (m, 3))
     u = rng.normal(size=(m, 3))
     u /= np.linalg.norm(u, axis=1)[:, None]
