@@ -827,7 +827,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // r the lane's low bits: a round holds up to 64 / oslots pairs per owner
     // (16 with the four home particles of the vector search), which the
     // replicas spread over distinct LDS words
-    const int oslots = tiles ? 4 * G : 4;
+    const int oslots = (tiles || nh > 4) ? 4 * G : 4;
     const int orep = kWave / oslots;
     const int oofs = oslots * (lane & (orep - 1));
     // candidate test in units of the cell side, FP16 operands: coordinates
