@@ -699,19 +699,95 @@ extern "C" int p3mk_adjoint_gather(const p3mk_geom *g, int ad, int n, const p3mk
 
 namespace {
 
-__global__ void k_btile_count(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
-                              const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ z, int *__restrict__ key,
-                              int *__restrict__ count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot)
+// The batched sort with per-block aggregation (as k_tile_count_agg and
+// k_tile_scatter_agg).  A block's particles are a contiguous range, so they
+// belong to a run of neighbouring systems whose tiles are the key window
+// [lo, hi).  A block whose window fits in LDS counts and ranks there, with
+// one global atomic per (block, tile); any other block takes one global
+// atomic per particle.
+__device__ __forceinline__ void btile_window(const BSys *__restrict__ sys, const int *__restrict__ psys,
+                                             int b, int e, long &lo, long &hi) {
+    const BSys &S0 = sys[psys[b]], &S1 = sys[psys[e - 1]];
+    lo = S0.tile0;
+    hi = S1.tile0 + (long)S1.nt * S1.nt * S1.nt;
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_btile_count_agg(int ntot, int per_block, const BSys *__restrict__ sys, const int *__restrict__ psys,
+                  const double *__restrict__ x, const double *__restrict__ y,
+                  const double *__restrict__ z, int *__restrict__ key, int *__restrict__ count) {
+    extern __shared__ int hist[];
+    const int b = blockIdx.x * per_block, e = min(ntot, b + per_block);
+    if (b >= e)
         return;
-    const BSys &S = sys[psys[i]];
-    const int tx = axis_base(x[i], S.Hi, S.cao, S.M) / S.T, ty = axis_base(y[i], S.Hi, S.cao, S.M) / S.T,
-              tz = axis_base(z[i], S.Hi, S.cao, S.M) / S.T;
-    const int k = (int)(S.tile0 + (tx * S.nt + ty) * S.nt + tz);
-    key[i] = k;
-    atomicAdd(&count[k], 1);
+    long lo, hi;
+    btile_window(sys, psys, b, e, lo, hi);
+    const int w = (int)min(hi - lo, kSortMaxTiles + 1);
+    const bool agg = w <= kSortMaxTiles;
+    if (agg)
+        for (int k = threadIdx.x; k < w; k += blockDim.x)
+            hist[k] = 0;
+    __syncthreads();
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const BSys &S = sys[psys[i]];
+        const int tx = axis_base(x[i], S.Hi, S.cao, S.M) / S.T, ty = axis_base(y[i], S.Hi, S.cao, S.M) / S.T,
+                  tz = axis_base(z[i], S.Hi, S.cao, S.M) / S.T;
+        const int k = (int)(S.tile0 + (tx * S.nt + ty) * S.nt + tz);
+        key[i] = k;
+        if (agg)
+            atomicAdd(&hist[k - lo], 1);
+        else
+            atomicAdd(&count[k], 1);
+    }
+    if (agg) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < w; k += blockDim.x)
+            if (hist[k] != 0)
+                atomicAdd(&count[lo + k], hist[k]);
+    }
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_btile_scatter_agg(int ntot, int per_block, const BSys *__restrict__ sys, const int *__restrict__ psys,
+                    const double *__restrict__ x, const double *__restrict__ y,
+                    const double *__restrict__ z, const double *__restrict__ q,
+                    const int *__restrict__ key, const int *__restrict__ start,
+                    int *__restrict__ cursor, int *__restrict__ perm, double4 *__restrict__ sorted) {
+    extern __shared__ int lds_i[];
+    const int b = blockIdx.x * per_block, e = min(ntot, b + per_block);
+    if (b >= e)
+        return;
+    long lo, hi;
+    btile_window(sys, psys, b, e, lo, hi);
+    const int w = (int)min(hi - lo, kSortMaxTiles + 1);
+    if (w > kSortMaxTiles) {
+        for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+            const int k = key[i];
+            const int p = start[k] + atomicAdd(&cursor[k], 1);
+            perm[p] = i;
+            sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+        }
+        return;
+    }
+    int *hist = lds_i, *base = lds_i + kSortMaxTiles;
+    for (int k = threadIdx.x; k < w; k += blockDim.x)
+        hist[k] = 0;
+    __syncthreads();
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x)
+        atomicAdd(&hist[key[i] - lo], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < w; k += blockDim.x) {
+        if (hist[k] != 0)
+            base[k] = start[lo + k] + atomicAdd(&cursor[lo + k], hist[k]);
+        hist[k] = 0;
+    }
+    __syncthreads();
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const int k = key[i] - (int)lo;
+        const int p = base[k] + atomicAdd(&hist[k], 1);
+        perm[p] = i;
+        sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+    }
 }
 
 template <typename T, bool R2C, bool IL>
@@ -825,15 +901,20 @@ int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, in
     if (ntot <= 0)
         return P3MK_OK;
     hipStream_t st = p3mk_stream(qu);
-    const int nb = (ntot + 255) / 256;
     P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * (ntiles + 1), qu));
     P3MK_TRY(p3mk_fill_zero(cursor, sizeof(int) * (ntiles + 1), qu));
-    hipLaunchKernelGGL(k_btile_count, dim3(nb), dim3(256), 0, st, ntot, sys, psys, x, y, z, key,
-                       count);
+    // blocks of 4096 particles: one system or a few neighbouring ones, whose
+    // tiles fit in LDS unless the tiles are very small
+    constexpr int per_block = 4096;
+    const int blocks = (ntot + per_block - 1) / per_block;
+    hipLaunchKernelGGL(k_btile_count_agg, dim3(blocks), dim3(kSortThreads),
+                       sizeof(int) * (size_t)kSortMaxTiles, st, ntot, per_block, sys, psys, x, y, z,
+                       key, count);
     P3MK_TRY(p3mk_launch_check("batched tile count"));
     P3MK_TRY(p3mk_exclusive_scan(count, ntiles, start, scan_scratch, qu));
-    hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, ntot, x, y, z, q, key, start,
-                       cursor, perm, sorted);
+    hipLaunchKernelGGL(k_btile_scatter_agg, dim3(blocks), dim3(kSortThreads),
+                       2 * sizeof(int) * (size_t)kSortMaxTiles, st, ntot, per_block, sys, psys, x, y,
+                       z, q, key, start, cursor, perm, sorted);
     P3MK_TRY(p3mk_launch_check("batched tile scatter"));
     hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, st,
                        count, ntiles, chunks);
