@@ -151,9 +151,20 @@ extern "C" void p3mk_fft_destroy(p3mk_fft *f) {
 
 namespace {
 
+// Lines per workgroup of a launch over one mesh: tile_lines, halved (down to
+// 8) while the launch would have fewer workgroups than the GPU has CUs (256).
+// Small meshes need it: at M = 64 an FP32 z pass of 32 lines per workgroup
+// has 128 workgroups.
+template <typename T> int launch_lines(int n, int nbuf, long nlines) {
+    int lt = tile_lines<T>(n, nbuf);
+    while (lt > 8 && (nlines + lt - 1) / lt < 256)
+        lt /= 2;
+    return lt;
+}
+
 template <typename T>
 int pass(p3mk_fft *f, c2_t<T> *data, const LineMap &m, int inv, bool contig, p3mk_queue *q) {
-    const int lt = tile_lines<T>(m.n, 2);
+    const int lt = launch_lines<T>(m.n, 2, m.nlines);
     const size_t lds = 2 * (size_t)lt * (m.n + 1) * sizeof(c2_t<T>);
     const long blocks = (m.nlines + lt - 1) / lt;
     const c2_t<T> *tw = sizeof(T) == 8 ? (const c2_t<T> *)f->tw64 : (const c2_t<T> *)f->tw32;
@@ -174,7 +185,7 @@ template <typename T> int forward_zy(p3mk_fft *f, void *mesh, void *spec, p3mk_q
         return pass<T>(f, d, y, 0, false, q);
     }
     const long nl = (long)M * M;
-    const int lt = tile_lines<T>(M, 2);
+    const int lt = launch_lines<T>(M, 2, nl);
     const size_t lds = 2 * (size_t)lt * (M + 1) * sizeof(V);
     hipLaunchKernelGGL(k_fft_r2c_z<T>, dim3((unsigned)((nl + lt - 1) / lt)), dim3(kThreads), lds,
                        p3mk_stream(q), (const T *)mesh, (V *)spec, nl, M, f->rp, tw, lt);
@@ -197,7 +208,7 @@ template <typename T> int backward_yz(p3mk_fft *f, void *spec, void *mesh, p3mk_
     LineMap y = {M, H, H, (long)M * H, (long)M * H};
     P3MK_TRY(pass<T>(f, (V *)spec, y, 1, false, q));
     const long nl = (long)M * M;
-    const int lt = tile_lines<T>(M, 2);
+    const int lt = launch_lines<T>(M, 2, nl);
     const size_t lds = 2 * (size_t)lt * (M + 1) * sizeof(V);
     hipLaunchKernelGGL(k_fft_c2r_z<T>, dim3((unsigned)((nl + lt - 1) / lt)), dim3(kThreads), lds,
                        p3mk_stream(q), (const V *)spec, (T *)mesh, nl, M, f->rp, tw, lt);
@@ -213,7 +224,7 @@ int convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spe
     const long nl = (long)M * W;
     LineMap x = {M, nl, nl, 0, nl};
     const int nbuf = (nout == 3) ? 3 : 2;
-    const int lt = tile_lines<T>(M, nbuf);
+    const int lt = launch_lines<T>(M, nbuf, nl);
     const size_t lds = (size_t)nbuf * lt * (M + 1) * sizeof(V);
     const c2_t<T> *tw = sizeof(T) == 8 ? (const V *)f->tw64 : (const V *)f->tw32;
     const double twopi_L = 2.0 * 3.14159265358979323846 / f->g.L;
@@ -234,7 +245,7 @@ int divergence(p3mk_fft *f, const double *G, const double *Dn, void *const in[3]
     const int W = f->g.r2c ? f->H : M;
     const long nl = (long)M * W;
     LineMap x = {M, nl, nl, 0, nl};
-    const int lt = tile_lines<T>(M, 3);
+    const int lt = launch_lines<T>(M, 3, nl);
     const size_t lds = (size_t)3 * lt * (M + 1) * sizeof(V);
     const c2_t<T> *tw = sizeof(T) == 8 ? (const V *)f->tw64 : (const V *)f->tw32;
     const double twopi_L = 2.0 * 3.14159265358979323846 / f->g.L;
