@@ -6,8 +6,9 @@
 // and 0.5 q_i q_j erfc(alpha r)/r to the energy.
 //
 // Layout: particles are folded into [0, L) and counting-sorted into cells of
-// side >= rcut/R (R = 2 when the box holds >= 5 cells per side), and inside a
-// cell by octant, so that 64 consecutive particles of a cell are spatially
+// side >= rcut/R (R = 2, or R = 1 when cells of side rcut/2 would hold fewer
+// than 6 particles on average or the box has fewer than 5 of them per side;
+// cell_geometry), and inside a cell by octant, so that 64 consecutive particles of a cell are spatially
 // compact.  A work item is (home cell, chunk of 64 home particles); a
 // persistent grid of single-wavefront workgroups claims items from an
 // atomic counter, so the dense cells of a clustered system are spread over
@@ -21,9 +22,11 @@
 //    reaction added to the partner; neighbour particles are staged into a
 //    64-slot LDS window and paired with the item's home particles on the
 //    matrix cores, K = 4 products of
-//      [-2 n', 1] . [h, |h|^2 - rcut^2] + |n'|^2 = |h - n'|^2 - rcut^2
-//    with n' and h relative to the home cell centre (FP32; the test keeps a
-//    2e-5 margin).
+//      [-2 n', 1] . [h, |h|^2 - rho^2 - m] + |n'|^2 = |h - n'|^2 - rho^2 - m
+//    with n' and h relative to the home cell centre in units of the cell
+//    side, rho = rcut / side, FP16 operands and FP32 sums; the margin
+//    m = 0.02 covers the FP16 rounding, so no pair within rcut is lost.
+//    Home cells of at most four particles take a vector search instead.
 //  - Boxes of fewer cells per side (nearest image per pair) and the adjoint:
 //    the full shell (k_realspace, k_realspace_vjp), every ordered pair for
 //    its home particle, candidates found by the vector ALUs.
