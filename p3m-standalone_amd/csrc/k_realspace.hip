@@ -181,24 +181,36 @@ __device__ __forceinline__ float up(double v) {
 }
 
 // bounding box of every work item's particles (outward-rounded FP32):
-// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0)
+// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0); one
+// wavefront per item, a lane per particle, min / max across the lanes
 __global__ void k_item_bbox(const int *__restrict__ start, const int *__restrict__ istart, long ncell,
                             long max_items, const double4 *__restrict__ sorted,
                             float4 *__restrict__ bbox) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (t >= max_items || t >= istart[ncell])
-        return;
+        return; // the whole wavefront
     const long c = prefix_find(istart, ncell, (int)t);
-    const int p0 = start[c] + (int)(t - istart[c]) * kWave;
-    const int p1 = min(start[c + 1], p0 + kWave);
+    const int p = start[c] + (int)(t - istart[c]) * kWave + lane;
     double xl = 1e300, yl = 1e300, zl = 1e300, xh = -1e300, yh = -1e300, zh = -1e300;
-    for (int p = p0; p < p1; p++) {
+    if (p < start[c + 1]) {
         const double4 r = sorted[p];
-        xl = fmin(xl, r.x); yl = fmin(yl, r.y); zl = fmin(zl, r.z);
-        xh = fmax(xh, r.x); yh = fmax(yh, r.y); zh = fmax(zh, r.z);
+        xl = xh = r.x;
+        yl = yh = r.y;
+        zl = zh = r.z;
     }
-    bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), 0.0f);
-    bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), 0.0f);
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        xl = fmin(xl, __shfl_xor(xl, off, kWave));
+        yl = fmin(yl, __shfl_xor(yl, off, kWave));
+        zl = fmin(zl, __shfl_xor(zl, off, kWave));
+        xh = fmax(xh, __shfl_xor(xh, off, kWave));
+        yh = fmax(yh, __shfl_xor(yh, off, kWave));
+        zh = fmax(zh, __shfl_xor(zh, off, kWave));
+    }
+    if (lane == 0) {
+        bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), 0.0f);
+        bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), 0.0f);
+    }
 }
 
 // value of lane k (uniform) in every lane
@@ -1292,7 +1304,7 @@ static int cells_end(CellArrays &a, long ncell, p3mk_queue *qu) {
                        a.fstart, ncell, a.start, a.items);
     P3MK_TRY(p3mk_launch_check("cell starts"));
     P3MK_TRY(p3mk_exclusive_scan(a.items, ncell, a.istart, a.scan, qu));
-    hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items + 3) / 4)), dim3(4 * kWave), 0, s,
                        a.start, a.istart, ncell, a.max_items, a.sorted, a.bbox);
     P3MK_TRY(p3mk_launch_check("item boxes"));
     return p3mk_fill_zero(a.counter, sizeof(int) * 4, qu);
