@@ -1,6 +1,9 @@
 // Exclusive prefix sum of int counts, used by the counting sorts of the
-// particle <-> mesh and real-space kernels.  Three launches: per-block scans
-// of 1024 elements, a scan of the block sums, and the block offsets added.
+// particle <-> mesh and real-space kernels.  Up to 4096 counts: one launch
+// of one workgroup (runs of up to four counts per thread; longer serial runs
+// lose to the three launches below).  Beyond: three launches, per-block
+// scans of 1024 elements, a scan of the block sums, and the block offsets
+// added.
 #include <hip/hip_runtime.h>
 
 #include "k_internal.hpp"
@@ -78,6 +81,30 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ out, int 
         out[m] = sums[nb];
 }
 
+// one workgroup: thread t sums its run of ceil(m / 1024) consecutive counts,
+// the run totals are scanned across the workgroup, and each thread writes its
+// run's prefixes; total to out[m].  Each thread reads its run before writing
+// it, so in and out may be the same array.
+constexpr long kSmallScan = 4 * kBlock;
+
+__global__ void __launch_bounds__(kBlock) k_scan_small(const int *in, int m, int *out) {
+    __shared__ int sh[128];
+    const int k = (m + kBlock - 1) / kBlock;
+    const int b = min(m, (int)threadIdx.x * k), e = min(m, b + k);
+    int sum = 0;
+    for (int i = b; i < e; i++)
+        sum += in[i];
+    int total;
+    int run = block_exclusive_scan(sum, sh, total);
+    for (int i = b; i < e; i++) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0)
+        out[m] = total;
+}
+
 }  // namespace
 
 size_t p3mk_scan_scratch_ints(long m) { return (size_t)((m + kBlock - 1) / kBlock) + 1; }
@@ -87,6 +114,10 @@ int p3mk_exclusive_scan(const int *count, long m, int *start, int *scratch, p3mk
     const int nb = (int)((m + kBlock - 1) / kBlock);
     if (m <= 0)
         return p3mk_fill_zero(start, sizeof(int), q);
+    if (m <= kSmallScan) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kBlock), 0, s, count, (int)m, start);
+        return p3mk_launch_check("scan");
+    }
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kBlock), 0, s, count, (int)m, start, scratch);
     P3MK_TRY(p3mk_launch_check("scan blocks"));
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, scratch, nb);
