@@ -78,6 +78,25 @@ int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, 
 // sorted FP64 positions (and 64 padding entries) with 32-bit byte offsets.
 constexpr long kCellMaxParticles = (1L << 27) - 64;
 
+// Counts and cursors of a counting sort over m keys in one allocation
+// (cursor = count + m rounded up to a multiple of 4), cleared by one fill.
+inline int alloc_count_cursor(int **count, int **cursor, size_t m) {
+    const size_t m4 = (m + 3) & ~(size_t)3;
+    *cursor = nullptr;
+    const int rc = p3mk_alloc((void **)count, sizeof(int) * 2 * m4);
+    if (rc == P3MK_OK)
+        *cursor = *count + m4;
+    return rc;
+}
+
+inline int zero_count_cursor(int *count, int *cursor, size_t m, p3mk_queue *qu) {
+    const size_t m4 = (m + 3) & ~(size_t)3;
+    if (cursor == count + m4)
+        return p3mk_fill_zero(count, sizeof(int) * 2 * m4, qu);
+    P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * m, qu));
+    return p3mk_fill_zero(cursor, sizeof(int) * m, qu);
+}
+
 // Below this mean number of particles per cell of side rcut / 2, the cell
 // list takes cells of side rcut instead.  In a sparse system a cell of side
 // rcut / 2 holds one or two particles, and each work item's walk over its 63

@@ -68,10 +68,10 @@ struct p3mk_batch {
 namespace {
 
 void release_all(p3mk_batch *b) {
+    // cursor lives in the count allocation
     void *ptrs[] = {b->dsys,  b->psys,   b->p0s,        b->cell0s,    b->tile0s, b->blk_z,
                     b->blk_y, b->blk_x,  b->mesh_arena, b->aux_arena, b->key,    b->count,
-                    b->start, b->cursor, b->perm,       b->chunks,    b->cstart, b->scan,
-                    b->sorted};
+                    b->start, b->perm,   b->chunks,     b->cstart,    b->scan,   b->sorted};
     for (void *p : ptrs)
         p3mk_release(p);
     cells_release(b->cells);
@@ -137,16 +137,16 @@ int build_descriptors(p3mk_batch *b) {
     P3MK_TRY(upload(&b->blk_y, blk_y));
     P3MK_TRY(upload(&b->blk_x, blk_x));
     // particle sort over all tiles
-    void **sort_ptrs[] = {(void **)&b->count, (void **)&b->start, (void **)&b->cursor,
-                          (void **)&b->chunks, (void **)&b->cstart, (void **)&b->scan};
+    void **sort_ptrs[] = {(void **)&b->count, (void **)&b->start, (void **)&b->chunks,
+                          (void **)&b->cstart, (void **)&b->scan};
     for (void **p : sort_ptrs) {
         p3mk_release(*p);
         *p = nullptr;
     }
+    b->cursor = nullptr;
     const size_t nt1 = (size_t)tiles + 1;
-    P3MK_TRY(p3mk_alloc((void **)&b->count, sizeof(int) * nt1));
+    P3MK_TRY(alloc_count_cursor(&b->count, &b->cursor, nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->start, sizeof(int) * nt1));
-    P3MK_TRY(p3mk_alloc((void **)&b->cursor, sizeof(int) * nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->chunks, sizeof(int) * nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->cstart, sizeof(int) * nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->scan, sizeof(int) * p3mk_scan_scratch_ints(tiles)));

@@ -379,9 +379,8 @@ extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
     if (!s)
         return;
     p3mk_release(s->key);
-    p3mk_release(s->count);
+    p3mk_release(s->count); // cursor lives in this allocation
     p3mk_release(s->start);
-    p3mk_release(s->cursor);
     p3mk_release(s->perm);
     p3mk_release(s->chunks);
     p3mk_release(s->cstart);
@@ -404,9 +403,8 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
     const size_t nt1 = (size_t)s->ntiles + 1;
     int rc = P3MK_OK;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->key, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->count, sizeof(int) * nt1);
+    if (rc == P3MK_OK) rc = alloc_count_cursor(&s->count, &s->cursor, nt1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->start, sizeof(int) * nt1);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cursor, sizeof(int) * nt1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->chunks, sizeof(int) * nt1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cstart, sizeof(int) * nt1);
     if (rc == P3MK_OK)
@@ -603,8 +601,7 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
     const double Hi = (double)g->M / g->L;
     const int nb = (n + 255) / 256;
     const size_t nt1 = (size_t)s->ntiles + 1;
-    P3MK_TRY(p3mk_fill_zero(s->count, sizeof(int) * nt1, qu));
-    P3MK_TRY(p3mk_fill_zero(s->cursor, sizeof(int) * nt1, qu));
+    P3MK_TRY(zero_count_cursor(s->count, s->cursor, nt1, qu));
     if (s->ntiles <= kSortMaxTiles) {
         // about 64 blocks: one global atomic per (block, tile) instead of per particle
         const int per_block = std::max(4096, (n + 63) / 64);
@@ -901,8 +898,7 @@ int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, in
     if (ntot <= 0)
         return P3MK_OK;
     hipStream_t st = p3mk_stream(qu);
-    P3MK_TRY(p3mk_fill_zero(count, sizeof(int) * (ntiles + 1), qu));
-    P3MK_TRY(p3mk_fill_zero(cursor, sizeof(int) * (ntiles + 1), qu));
+    P3MK_TRY(zero_count_cursor(count, cursor, (size_t)ntiles + 1, qu));
     // blocks of 4096 particles: one system or a few neighbouring ones, whose
     // tiles fit in LDS unless the tiles are very small
     constexpr int per_block = 4096;

@@ -58,6 +58,8 @@ constexpr int kWave = 64;
 // full-shell kernels (nearest images, adjoint): at most 63 pending pairs plus
 // one append of the vector search
 constexpr int kQueue = 2 * kWave;
+// work-item counters of a cell list (one per kernel launched on it)
+constexpr int kCounters = 4;
 // sorted arrays carry kPad zero-charge entries past the last particle
 constexpr int kPad = 64;
 // home cells of more particles than this use the matrix-core candidate search
@@ -1254,8 +1256,9 @@ __global__ void k_unsort_add(int n, const int *__restrict__ pos, const double *_
 namespace p3m {
 
 void cells_release(CellArrays &a) {
-    void *ptrs[] = {a.key, a.fcount, a.fstart, a.fcursor, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.counter, a.sorted, a.sorted32, a.bbox, a.gsorted, a.fsorted};
+    // fcursor and counter live in the fcount allocation
+    void *ptrs[] = {a.key, a.fcount, a.fstart, a.start, a.items, a.istart, a.perm,
+                    a.scan, a.sorted, a.sorted32, a.bbox, a.gsorted, a.fsorted};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -1267,15 +1270,21 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     a.max_items = ncell + n / kWave + 1;
     int rc = P3MK_OK;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.key, sizeof(int) * nn);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcount, sizeof(int) * nf);
+    // fcount, fcursor and the work-item counters in one allocation, cleared
+    // by one fill: [fcount | fcursor | counters], each part a multiple of
+    // 16 bytes
+    const size_t nf4 = (nf + 3) & ~(size_t)3;
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcount, sizeof(int) * (2 * nf4 + kCounters));
+    if (rc == P3MK_OK) {
+        a.fcursor = a.fcount + nf4;
+        a.counter = a.fcount + 2 * nf4;
+    }
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fstart, sizeof(int) * nf);
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcursor, sizeof(int) * nf);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.start, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.items, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.istart, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.perm, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.scan, sizeof(int) * p3mk_scan_scratch_ints(8 * ncell));
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.counter, sizeof(int) * 4);
     const size_t np = nn + kPad;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * np);
@@ -1286,10 +1295,11 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     return rc;
 }
 
-// zero the octant counters before the keys are counted
+// zero the octant counts and cursors and the work-item counters (one
+// allocation) before the keys are counted
 static int cells_begin(CellArrays &a, long ncell, p3mk_queue *qu) {
-    P3MK_TRY(p3mk_fill_zero(a.fcount, sizeof(int) * (8 * ncell + 1), qu));
-    return p3mk_fill_zero(a.fcursor, sizeof(int) * (8 * ncell + 1), qu);
+    (void)ncell;
+    return p3mk_fill_zero(a.fcount, sizeof(int) * (size_t)(a.counter + kCounters - a.fcount), qu);
 }
 
 // octant starts after the count (before the scatter)
@@ -1306,8 +1316,7 @@ static int cells_end(CellArrays &a, long ncell, p3mk_queue *qu) {
     P3MK_TRY(p3mk_exclusive_scan(a.items, ncell, a.istart, a.scan, qu));
     hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items + 3) / 4)), dim3(4 * kWave), 0, s,
                        a.start, a.istart, ncell, a.max_items, a.sorted, a.bbox);
-    P3MK_TRY(p3mk_launch_check("item boxes"));
-    return p3mk_fill_zero(a.counter, sizeof(int) * 4, qu);
+    return p3mk_launch_check("item boxes");
 }
 
 }  // namespace p3m
