@@ -183,33 +183,38 @@ __device__ __forceinline__ float up(double v) {
 }
 
 // bounding box of every work item's particles (outward-rounded FP32):
-// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0); one
-// wavefront per item, a lane per particle, min / max across the lanes
+// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0).  Eight
+// lanes per item, each over every eighth of its (up to 64) particles, then
+// min / max across the eight: a thread per item serialises 64 loads, a
+// wavefront per item multiplies the cell searches by 64.
+constexpr int kBoxLanes = 8;
+
 __global__ void k_item_bbox(const int *__restrict__ start, const int *__restrict__ istart, long ncell,
                             long max_items, const double4 *__restrict__ sorted,
                             float4 *__restrict__ bbox) {
-    const long t = (long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = g / kBoxLanes;
+    const int sub = (int)(g % kBoxLanes);
     if (t >= max_items || t >= istart[ncell])
-        return; // the whole wavefront
+        return; // all eight lanes of the item
     const long c = prefix_find(istart, ncell, (int)t);
-    const int p = start[c] + (int)(t - istart[c]) * kWave + lane;
+    const int p0 = start[c] + (int)(t - istart[c]) * kWave;
+    const int p1 = min(start[c + 1], p0 + kWave);
     double xl = 1e300, yl = 1e300, zl = 1e300, xh = -1e300, yh = -1e300, zh = -1e300;
-    if (p < start[c + 1]) {
+    for (int p = p0 + sub; p < p1; p += kBoxLanes) {
         const double4 r = sorted[p];
-        xl = xh = r.x;
-        yl = yh = r.y;
-        zl = zh = r.z;
+        xl = fmin(xl, r.x); yl = fmin(yl, r.y); zl = fmin(zl, r.z);
+        xh = fmax(xh, r.x); yh = fmax(yh, r.y); zh = fmax(zh, r.z);
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        xl = fmin(xl, __shfl_xor(xl, off, kWave));
-        yl = fmin(yl, __shfl_xor(yl, off, kWave));
-        zl = fmin(zl, __shfl_xor(zl, off, kWave));
-        xh = fmax(xh, __shfl_xor(xh, off, kWave));
-        yh = fmax(yh, __shfl_xor(yh, off, kWave));
-        zh = fmax(zh, __shfl_xor(zh, off, kWave));
+    for (int off = kBoxLanes / 2; off > 0; off >>= 1) {
+        xl = fmin(xl, __shfl_xor(xl, off, kBoxLanes));
+        yl = fmin(yl, __shfl_xor(yl, off, kBoxLanes));
+        zl = fmin(zl, __shfl_xor(zl, off, kBoxLanes));
+        xh = fmax(xh, __shfl_xor(xh, off, kBoxLanes));
+        yh = fmax(yh, __shfl_xor(yh, off, kBoxLanes));
+        zh = fmax(zh, __shfl_xor(zh, off, kBoxLanes));
     }
-    if (lane == 0) {
+    if (sub == 0) {
         bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), 0.0f);
         bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), 0.0f);
     }
@@ -1314,7 +1319,7 @@ static int cells_end(CellArrays &a, long ncell, p3mk_queue *qu) {
                        a.fstart, ncell, a.start, a.items);
     P3MK_TRY(p3mk_launch_check("cell starts"));
     P3MK_TRY(p3mk_exclusive_scan(a.items, ncell, a.istart, a.scan, qu));
-    hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items + 3) / 4)), dim3(4 * kWave), 0, s,
+    hipLaunchKernelGGL(k_item_bbox, dim3((unsigned)((a.max_items * kBoxLanes + 255) / 256)), dim3(256), 0, s,
                        a.start, a.istart, ncell, a.max_items, a.sorted, a.bbox);
     return p3mk_launch_check("item boxes");
 }
