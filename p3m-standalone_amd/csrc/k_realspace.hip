@@ -183,7 +183,9 @@ __device__ __forceinline__ float up(double v) {
 }
 
 // bounding box of every work item's particles (outward-rounded FP32):
-// bbox[2 t] = (xlo, ylo, zlo, 0), bbox[2 t + 1] = (xhi, yhi, zhi, 0).  Eight
+// bbox[2 t] = (xlo, ylo, zlo, cell), bbox[2 t + 1] = (xhi, yhi, zhi, chunk),
+// the item's cell and chunk stored as int bits (read by the half-shell
+// kernels instead of a search of istart; nothing reads them as floats).  Eight
 // lanes per item, each over every eighth of its (up to 64) particles, then
 // min / max across the eight: a thread per item serialises 64 loads, a
 // wavefront per item multiplies the cell searches by 64.
@@ -215,8 +217,8 @@ __global__ void k_item_bbox(const int *__restrict__ start, const int *__restrict
         zh = fmax(zh, __shfl_xor(zh, off, kBoxLanes));
     }
     if (sub == 0) {
-        bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), 0.0f);
-        bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), 0.0f);
+        bbox[2 * t] = make_float4(down(xl), down(yl), down(zl), __int_as_float((int)c));
+        bbox[2 * t + 1] = make_float4(up(xh), up(yh), up(zh), __int_as_float((int)(t - istart[c])));
     }
 }
 
@@ -1229,9 +1231,10 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total)
             break;
-        const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item_half<FP32, true>(home, item - istart[home], 0, sorted, sorted32, start, istart, bbox, nc,
-                                  reach, L, rcut, alpha, stride, fs, sh, en_tot);
+        // the item's cell and chunk, stored with its box
+        const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
+        realspace_item_half<FP32, true>(home, chunk, 0, sorted, sorted32, start, istart, bbox, nc, reach, L,
+                                        rcut, alpha, stride, fs, sh, en_tot);
     }
     if (energy != nullptr) {
         for (int off = 32; off > 0; off >>= 1)
@@ -1568,13 +1571,13 @@ k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, 
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total)
             break;
-        const int home = (int)prefix_find(istart, ncell, item);
+        const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
         const int s = p3m::fft::prefix_slot(cell0s, nsys, (long)home);
         const BSys &S = sys[s];
         if (!(S.rcut > 0.0) || S.small)
             continue; // no real-space part, or nearest images (k_brealspace)
         double en = 0.0;
-        realspace_item_half<FP32, FP32>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32, start,
+        realspace_item_half<FP32, FP32>(home - S.cell0, chunk, S.cell0, sorted, sorted32, start,
                                   istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fs, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
