@@ -184,7 +184,7 @@ __device__ __forceinline__ float up(double v) {
 
 // bounding box of every work item's particles (outward-rounded FP32):
 // bbox[2 t] = (xlo, ylo, zlo, cell), bbox[2 t + 1] = (xhi, yhi, zhi, chunk),
-// the item's cell and chunk stored as int bits (read by the half-shell
+// the item's cell and chunk stored as int bits (read by the real-space
 // kernels instead of a search of istart; nothing reads them as floats).  Eight
 // lanes per item, each over every eighth of its (up to 64) particles, then
 // min / max across the eight: a thread per item serialises 64 loads, a
@@ -725,8 +725,8 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total)
             break;
-        const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<FP32, SMALL_BOX>(home, item - istart[home], 0, sorted, sorted32, perm,
+        const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
+        realspace_item<FP32, SMALL_BOX>(home, chunk, 0, sorted, sorted32, perm,
                                         start, istart, bbox, nc, reach, L, rcut, alpha, n, fr, sh,
                                         en_tot);
     }
@@ -760,8 +760,8 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total)
             break;
-        const int home = (int)prefix_find(istart, ncell, item);
-        realspace_item<false, SMALL_BOX, true>(home, item - istart[home], 0, sorted, sorted32,
+        const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
+        realspace_item<false, SMALL_BOX, true>(home, chunk, 0, sorted, sorted32,
                                                perm, start, istart, bbox, nc, reach, L, rcut, alpha,
                                                n, dpos, sh, en_unused, gsorted, dq);
     }
@@ -1527,13 +1527,13 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total)
             break;
-        const int home = (int)prefix_find(istart, ncell, item);
+        const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
         const int s = p3m::fft::prefix_slot(cell0s, nsys, (long)home);
         const BSys &S = sys[s];
         if (!(S.rcut > 0.0) || !S.small)
             continue; // no real-space part, or a cell grid (k_brealspace_half)
         double en = 0.0;
-        realspace_item<FP32, true>(home - S.cell0, item - istart[home], S.cell0, sorted, sorted32,
+        realspace_item<FP32, true>(home - S.cell0, chunk, S.cell0, sorted, sorted32,
                                    perm, start, istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot,
                                    fr, sh, en);
         if (energy != nullptr) {
