@@ -313,13 +313,17 @@ int p3m_hip_realteil_vjp_device(data_t *d, int n, double length, const double *x
                                 const double *gF, double *dpos, double *dq);
 
 /* Error estimates with a status (Error / Error_k of the method table):
- * realpart.c:343-346 with p3m-ik.c:234-286, p3m-ik-i.c:280-305,394-399,
- * p3m-ad.c:475-505, p3m-ad-i.c:237-279.  The k-space sums run on the GPU. */
+ * realpart.c:343-346 with p3m-ik.c:249-301, p3m-ik-i.c:295-320,409-414,
+ * p3m-ad.c:436-505, p3m-ad-i.c:205-294.  For methods 0, 1 and 6 at the
+ * reference's 83 tabulated meshes (q.c:1) and alpha*L <= 100 the k-space
+ * estimate comes from the built-in Q_ik / Q_ik_i tables, interpolated as
+ * find_error.c:84-107 does; otherwise the k-space sums run on the GPU. */
 int p3m_hip_error(int method_id, system_t *s, parameters_t *p, double *total, double *kspace);
-/* Supply one row of the reference's tabulated ik / ik-i error kernel
- * (q_ik.c / q_ik_i.c: 101 values at alpha*L = 0..100) for (mesh, cao);
- * kind 0 = Q_ik (methods 0, 6), 1 = Q_ik_i (method 1).  With a row present
- * the estimate interpolates it as find_error.c:84-107 does. */
+/* Replace one row of the tabulated ik / ik-i error kernel (101 values at
+ * alpha*L = 0..100) for (mesh, cao); kind 0 = Q_ik (methods 0, 6), 1 =
+ * Q_ik_i (method 1).  No counterpart in the reference (its tables are
+ * linked in); a registered row takes precedence over the built-in one, and
+ * also serves meshes the built-in tables do not hold. */
 int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101);
 
 /* FP64 Ewald sum on sampled particles: the reference's Ewald method

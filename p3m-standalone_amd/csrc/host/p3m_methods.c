@@ -17,6 +17,7 @@
 
 #include "../p3m_kernels.h"
 #include "host_common.h"
+#include "q_tables.h"
 
 #define PI_D 3.14159265358979323846
 
@@ -831,21 +832,42 @@ int p3m_hip_set_error_table(int kind, int mesh, int cao, const double *table101)
     return P3M_HIP_OK;
 }
 
-/* find_error.c:84-107: -1 when (mesh, cao) is not tabulated or alpha L > 100 */
+static int compare_ints(const void *a, const void *b) {
+    const int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* find_error.c:84-107: -1 when (mesh, cao) is not tabulated or alpha L > 100;
+ * otherwise the linear interpolation of the table row on the grid alpha L =
+ * 0, 1, ..., 100 (find_error.c:38-58).  Rows registered with
+ * p3m_hip_set_error_table take precedence over the built-in tables (the
+ * reference's Q_ik / Q_ik_i for its 83 meshes, q_tables.c).  At alpha L =
+ * 100 the reference reads one entry past the row; the last entry is used
+ * here. */
 static double table_lookup(int kind, int mesh, int cao, double alphaL) {
+    const double *row = NULL;
+    const int *mp;
     int k, l;
     double dd;
-    for (k = 0; k < g_ntables; k++)
-        if (g_tables[k].kind == kind && g_tables[k].mesh == mesh && g_tables[k].cao == cao)
-            break;
-    if (k == g_ntables || alphaL > 100.0)
+    if (alphaL > 100.0 || cao < 1 || cao > 7)
         return -1.0;
+    for (k = 0; k < g_ntables; k++)
+        if (g_tables[k].kind == kind && g_tables[k].mesh == mesh && g_tables[k].cao == cao) {
+            row = g_tables[k].v;
+            break;
+        }
+    if (row == NULL) {
+        mp = bsearch(&mesh, p3m_q_meshes, P3M_Q_NMESHES, sizeof(int), compare_ints);
+        if (mp == NULL)
+            return -1.0;
+        row = (kind == 1 ? p3m_q_ik_i : p3m_q_ik)[mp - p3m_q_meshes][cao - 1];
+    }
     dd = alphaL;
     l = (int)floor(dd);
     dd -= l;
     if (l >= 100)
-        return g_tables[k].v[100];
-    return (1. - dd) * g_tables[k].v[l] + dd * g_tables[k].v[l + 1];
+        return row[100];
+    return (1. - dd) * row[l] + dd * row[l + 1];
 }
 
 static int kspace_error(int id, const system_t *s, const parameters_t *p, double *out) {

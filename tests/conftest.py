@@ -70,3 +70,22 @@ def fold(s):
 def rel_err(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def error_table_row(method, M, cao):
+    """The reference's Q_ik / Q_ik_i row (q_ik.c, q_ik_i.c) that Error_k of
+    methods 0, 6 / 1 reads at mesh M, or None (ad methods, or a mesh outside
+    the committed fixture)."""
+    if method not in (0, 1, 6):
+        return None
+    d = np.load(os.path.join(GOLDEN, "q_tables.npz"))
+    hit = np.flatnonzero(d["meshes"] == M)
+    if hit.size == 0:
+        return None
+    return np.array(d["Q_ik_i" if method == 1 else "Q_ik"][hit[0], cao - 1])
+
+
+def ref_kspace_error(orc, method, N, L, q2, M, cao, alpha):
+    """The oracle's Error_k as the reference computes it: through the
+    tabulated row where the reference has one (find_error.c:84-107)."""
+    return orc.kspace_error(method, N, L, q2, M, cao, alpha, table=error_table_row(method, M, cao))

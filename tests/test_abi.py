@@ -129,3 +129,52 @@ def test_error_table_argument_checks(pkg):
     assert pkg.lib().p3m_hip_set_error_table(2, 16, 5, tp) == pkg.EINVAL
     assert pkg.lib().p3m_hip_set_error_table(0, 16, 8, tp) == pkg.EINVAL
     assert pkg.lib().p3m_hip_set_error_table(0, 16, 5, None) == pkg.EINVAL
+
+
+def _table_lookup(tab, alpha, length):
+    """find_error.c:38-58: linear interpolation on alpha L = 0, 1, ..., 100."""
+    import math
+
+    aL = alpha * length
+    l = int(math.floor(aL))
+    d = aL - l
+    return tab[100] if l >= 100 else (1.0 - d) * tab[l] + d * tab[l + 1]
+
+
+@pytest.mark.parametrize("method", [0, 1, 6])
+def test_builtin_error_tables(pkg, method):
+    """Error_k of ids 0, 1 and 6 at tabulated meshes reads the reference's
+    Q_ik / Q_ik_i tables without any registration (find_error.c:84-107),
+    through p3m_hip_error and through the method table's Error_k; both equal
+    the estimate formed from the rows of q_ik.c / q_ik_i.c in
+    tests/golden/q_tables.npz.  A table hit is host logic: no GPU needed."""
+    import numpy as np
+
+    d = np.load(os.path.join(ROOT, "tests", "golden", "q_tables.npz"))
+    tabs = d["Q_ik_i" if method == 1 else "Q_ik"]
+    s = random_system(1000, 10.0, seed=2)
+    system = pkg.HostSystem.from_dict(s)
+    q2, L = float(np.sum(s["q"] ** 2)), 10.0
+    m = pkg.method(method)
+    t, k = ctypes.c_double(), ctypes.c_double()
+    checked = 0
+    for mi, M in enumerate(d["meshes"]):
+        for cao in range(1, 8):
+            for aL in (0.0, 0.37, 1.0, 7.5, 20.0, 33.3, 41.76, 99.5, 100.0):
+                alpha = aL / L
+                p = pkg.parameters(int(M), cao, alpha, 3.0)
+                he = _table_lookup(tabs[mi, cao - 1], alpha, L)
+                if he < 0.0:
+                    # the tables hold a few rounding-noise negatives at small
+                    # alpha L; there the reference falls back to the direct
+                    # sum (p3m-ik.c:270), a GPU computation
+                    continue
+                # p3m-ik.c:300, p3m-ik-i.c:319
+                want = 2.0 * q2 * np.sqrt(he / 1000) / (L * L)
+                rc = pkg.lib().p3m_hip_error(method, system.ptr, ctypes.pointer(p), ctypes.byref(t),
+                                            ctypes.byref(k))
+                assert rc == pkg.OK
+                assert abs(k.value - want) <= 1e-14 * abs(want), (M, cao, aL, k.value, want)
+                assert m.Error_k(system.ptr, ctypes.pointer(p)) == k.value
+                checked += 1
+    assert checked > 200

@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import fold, load_fixture, random_system, rel_err
+from conftest import error_table_row, fold, load_fixture, random_system, ref_kspace_error, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -166,12 +166,34 @@ def test_error_estimates(pkg, orc, method):
     system = pkg.HostSystem.from_dict(s)
     total, kspace = pkg.error_estimate(method, system, pkg.parameters(M, cao, alpha, rcut))
     q2 = float(np.sum(s["q"] ** 2))
-    kref = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha)
+    kref = ref_kspace_error(orc, method, 1000, 10.0, q2, M, cao, alpha)
     rref = orc.realspace_error(1000, 10.0, q2, rcut, alpha)
     # the summand alias1 - alias2^2/(...) cancels to ~1e-4 of its terms, so
     # summation order shows at ~1e-10 of the sum
     assert abs(kspace - kref) <= 1e-8 * abs(kref)
     assert abs(total - np.hypot(kref, rref)) <= 1e-8 * np.hypot(kref, rref)
+
+
+@pytest.mark.parametrize("method,M,cao,alpha", [(0, 16, 6, 0.15), (0, 32, 7, 0.6), (1, 16, 4, 0.25),
+                                                (1, 16, 6, 0.2), (6, 16, 7, 0.2)])
+def test_error_table_negative_lookup(pkg, orc, method, M, cao, alpha):
+    """Where the interpolated table value is negative (rounding noise of the
+    reference's tables at small alpha L) the reference takes the direct sum:
+    ik starting from the negative lookup, ik-i from 0, then fabs
+    (p3m-ik.c:267-297, p3m-ik-i.c:301-318)."""
+    s = random_system(1000, 10.0, seed=5)
+    row = error_table_row(method, M, cao)
+    aL = alpha * 10.0
+    l = int(np.floor(aL))
+    assert (1.0 - (aL - l)) * row[l] + (aL - l) * row[l + 1] < 0.0
+    system = pkg.HostSystem.from_dict(s)
+    _, k = pkg.error_estimate(method, system, pkg.parameters(M, cao, alpha, 3.0))
+    q2 = float(np.sum(s["q"] ** 2))
+    kref = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha, table=row)
+    kdirect = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha)
+    print("method %d M %d cao %d alpha L %.2f: %.6e (oracle %.6e, without the lookup %.6e)" %
+          (method, M, cao, aL, k, kref, kdirect))
+    assert abs(k - kref) <= 1e-8 * abs(kref)
 
 
 @pytest.mark.parametrize("name,method_id", [("method_p3m_ik", 0), ("method_p3m_ik_i", 1),
@@ -206,7 +228,7 @@ def test_method_table_drives_like_main(pkg, orc, name, method_id):
     else:
         assert np.all(forces.total() == 0.0)
     q2 = float(np.sum(s["q"] ** 2))
-    kref = orc.kspace_error(method_id, 400, 10.0, q2, M, cao, params.alpha)
+    kref = ref_kspace_error(orc, method_id, 400, 10.0, q2, M, cao, params.alpha)
     rref = orc.realspace_error(400, 10.0, q2, rcut, params.alpha)
     assert abs(m.Error_k(system.ptr, pptr) - kref) <= 1e-8 * abs(kref)
     assert abs(m.Error(system.ptr, pptr) - np.hypot(kref, rref)) <= 1e-8 * np.hypot(kref, rref)
