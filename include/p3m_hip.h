@@ -213,7 +213,7 @@ const method_t *p3m_hip_method(int method_id);
 enum {
     P3M_HIP_OK = 0,
     P3M_HIP_EINVAL = 1,      /* bad argument: cao not in 1..7, odd or too small mesh, ... */
-    P3M_HIP_ERANGE = 2,      /* a particle position outside [0, L) or not finite */
+    P3M_HIP_ERANGE = 2,      /* a k-space position outside [-L, 2L) or not finite */
     P3M_HIP_ENOMEM = 3,      /* device or host allocation failed */
     P3M_HIP_EDEVICE = 4,     /* a HIP runtime call failed */
     P3M_HIP_ENODEVICE = 5    /* no usable GPU */
@@ -233,12 +233,12 @@ const char *p3m_hip_last_error(void);
 /* Init of method `method_id` with the given precision; *out receives the
  * data_t.  Replaces method.Init (p3m-common.c:145-232 + Init_*). */
 int p3m_hip_init(int method_id, system_t *s, parameters_t *p, int precision, data_t **out);
-/* Influence_function with a status (p3m-ik.c:120-158, p3m-ik-i.c:111-151,
- * p3m-ad.c:96-143, p3m-ad-i.c:106-135). */
+/* Influence_function with a status (p3m-ik.c:135-173, p3m-ik-i.c:126-166,
+ * p3m-ad.c:111-158, p3m-ad-i.c:121-150). */
 int p3m_hip_influence_function(system_t *s, parameters_t *p, data_t *d);
-/* Kspace_force with a status (p3m-ik.c:164-230, p3m-ik-i.c:154-231,
- * p3m-ad.c:146-195, p3m-ad-i.c:138-188, p3m-ik-real.c:68-132,
- * p3m-ad-real.c:54-103).  Host arrays in, host arrays accumulated. */
+/* Kspace_force with a status (p3m-ik.c:179-245, p3m-ik-i.c:169-246,
+ * p3m-ad.c:161-210, p3m-ad-i.c:153-203, p3m-ik-real.c:83-147,
+ * p3m-ad-real.c:69-118).  Host arrays in, host arrays accumulated. */
 int p3m_hip_kspace_force(system_t *s, parameters_t *p, data_t *d, forces_t *f);
 /* Frees the data_t and its device state.  Replaces Free_data
  * (p3m-common.c:234-287) for data_t objects made by this library. */
@@ -258,7 +258,7 @@ int p3m_hip_kspace_force_device(data_t *d, int n, double length, const double *x
 int p3m_hip_synchronize(data_t *d);
 
 /* Real-space part on the GPU: the pair set and arithmetic of Realteil
- * (realpart.c:109-152): every i != j with minimum-image distance <= rcut
+ * (realpart.c:124-167): every i != j with minimum-image distance <= rcut
  * (rcut <= L/2).  f->f_r += forces, s->energy += energy. */
 int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f);
 /* Device-resident real-space part: fr (3n) += forces; energy (device
@@ -266,7 +266,7 @@ int p3m_hip_realteil(system_t *s, parameters_t *p, forces_t *f);
 int p3m_hip_realteil_device(data_t *d, int n, double length, const double *x, const double *y,
                             const double *z, const double *q, double rcut, double alpha,
                             double *fr, double *energy);
-/* Device-resident Calculate_forces (common.c:253-312): zero f, fk, fr
+/* Device-resident Calculate_forces (common.c:268-327): zero f, fk, fr
  * (3n each, device, FP64, x/y/z blocks); fr += real space if rcut != 0;
  * fk += k space (f -= self force for ad / ad-r); f += fk + fr.  energy
  * (device double, may be NULL) receives the real-space energy.  Enqueued on
@@ -275,7 +275,7 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
                                     const double *y, const double *z, const double *q, double rcut,
                                     double alpha, double *f, double *fk, double *fr,
                                     double *energy);
-/* Calculate_forces (common.c:253-312) on the GPU: zero f, f_k, f_r; real
+/* Calculate_forces (common.c:268-327) on the GPU: zero f, f_k, f_r; real
  * space if p->rcut != 0; Kspace_force; f += f_k + f_r. */
 int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, data_t *d,
                              forces_t *f);
@@ -373,7 +373,7 @@ int p3m_hip_batch_set_alpha(p3m_hip_batch *b, int s, double alpha);
  * ad-r).  Enqueued on the batch's stream. */
 int p3m_hip_batch_kspace_force_device(p3m_hip_batch *b, const double *x, const double *y,
                                       const double *z, const double *q, double *fk, double *f);
-/* Calculate_forces (common.c:253-312) of every system: zero f, fk, fr;
+/* Calculate_forces (common.c:268-327) of every system: zero f, fk, fr;
  * fr += real space (systems with rcut > 0); fk += k space; f += fk + fr;
  * energy[s] (device, nsys doubles, may be NULL) receives system s's
  * real-space energy.  Enqueued on the batch's stream. */

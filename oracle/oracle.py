@@ -198,7 +198,7 @@ def ewald_kspace(sysd, alpha, kmax):
 
 
 def ewald_reference(sysd, precision=1e-8):
-    """Reference forces as common.c:314-348 computes them: rc = 0.49 L,
+    """Reference forces as common.c:329-363 computes them: rc = 0.49 L,
     bisected alpha, kmax grown by 2 until the estimate <= precision.  Returns
     (f_total, f_kspace, alpha, kmax)."""
     L, N = float(sysd["L"]), len(sysd["x"])
@@ -213,7 +213,7 @@ def ewald_reference(sysd, precision=1e-8):
             break
         kmax += 2
     # The reference's influence function fills n^2 <= (kmax-1)^2 only
-    # (ewald.c:74-75 with p->mesh = kmax), then sums over the kmax cube.
+    # (ewald.c:89-90 with p->mesh = kmax), then sums over the kmax cube.
     fk = ewald_kspace(sysd, alpha, kmax - 1)
     fr, _ = realteil(sysd, rcut, alpha, cells=N > 2000)
     return fr + fk, fk, alpha, kmax

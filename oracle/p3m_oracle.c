@@ -266,7 +266,7 @@ void orc_irfft3d(double *c, int M)
 /* Operators and influence functions                                         */
 /* ------------------------------------------------------------------------ */
 
-/* p3m-common.c:95-116 */
+/* p3m-common.c:110-131 */
 void orc_dn(int M, double *Dn)
 {
     int i;
@@ -275,7 +275,7 @@ void orc_dn(int M, double *Dn)
     Dn[M / 2] = 0.0;
 }
 
-/* p3m-common.c:118-128 */
+/* p3m-common.c:133-143 */
 void orc_nshift(int M, double *ns)
 {
     int i;
@@ -283,7 +283,7 @@ void orc_nshift(int M, double *ns)
         ns[i] = i - round(i / (double)M) * (double)M;
 }
 
-/* sinc (p3m-common.c:56-60) */
+/* sinc (p3m-common.c:71-75) */
 static double sinc(double d)
 {
     double pd = M_PI * d;
@@ -305,7 +305,7 @@ static double bspline_k(int cao, double d)
     return (d == 0.0) ? 1.0 : pow(sin(pd) / pd, cao);
 }
 
-/* ik (p3m-ik.c:93-158): uses U_hat^2 */
+/* ik (p3m-ik.c:93-173): uses U_hat^2 */
 static double influence_ik_point(int NX, int NY, int NZ, int M, int cao, double L,
                                  double alpha, int mc, const double *ns, const double *Dn)
 {
@@ -335,7 +335,7 @@ static double influence_ik_point(int NX, int NY, int NZ, int M, int cao, double 
     return 2.0 * zwi / M_PI;
 }
 
-/* ik-i (p3m-ik-i.c:63-151) */
+/* ik-i (p3m-ik-i.c:78-166) */
 static double influence_iki_point(int NX, int NY, int NZ, int M, int cao, double L,
                                   double alpha, int mc, const double *ns, const double *Dn)
 {
@@ -370,7 +370,7 @@ static double influence_iki_point(int NX, int NY, int NZ, int M, int cao, double
     return 2.0 * zwi / M_PI;
 }
 
-/* ad (p3m-ad.c:54-129) and ad-i (p3m-ad-i.c:55-131) */
+/* ad (p3m-ad.c:69-158) and ad-i (p3m-ad-i.c:70-150) */
 static double influence_ad_point(int interlaced, int NX, int NY, int NZ, int M, int cao,
                                  double L, double alpha, int mc, const double *ns)
 {
@@ -413,7 +413,7 @@ int orc_influence(int method, int M, int cao, double L, double alpha, int mc, do
     orc_nshift(M, ns);
     orc_dn(M, Dn);
     if ((method == 2 || method == 7) && alpha == 0.0) {
-        /* p3m-ad.c:105-108 */
+        /* p3m-ad.c:120-123 */
         memset(G, 0, sizeof(double) * M * M * M);
         free(ns);
         free(Dn);
@@ -663,9 +663,9 @@ static void gather_ad(int method, int N, double L, const double *x, const double
     }
 }
 
-/* Kspace_force of methods 0,1,2,3,6,7 (p3m-ik.c:164-230, p3m-ik-i.c:154-231,
- * p3m-ad.c:146-195, p3m-ad-i.c:138-188, p3m-ik-real.c:68-132,
- * p3m-ad-real.c:54-103) */
+/* Kspace_force of methods 0,1,2,3,6,7 (p3m-ik.c:179-245, p3m-ik-i.c:169-246,
+ * p3m-ad.c:161-210, p3m-ad-i.c:153-203, p3m-ik-real.c:83-147,
+ * p3m-ad-real.c:69-118) */
 int orc_kspace_force(int method, int N, double L, const double *x, const double *y,
                      const double *z, const double *q, int M, int cao, const double *G,
                      const double *b, double *fk, double *f)
@@ -701,7 +701,7 @@ int orc_kspace_force(int method, int N, double L, const double *x, const double 
                     double T1 = G[((long)i * M + j) * M + k];
                     double dop[3] = {Dn[i], Dn[j], Dn[k]};
                     if (method == 0) {
-                        /* p3m-ik.c:196-216 */
+                        /* p3m-ik.c:211-231 */
                         double qr = 2.0 * M_PI * Leni * Q[ci] * T1;
                         double qi = -2.0 * M_PI * Leni * Q[ci + 1] * T1;
                         for (d = 0; d < 3; d++) {
@@ -709,7 +709,7 @@ int orc_kspace_force(int method, int N, double L, const double *x, const double 
                             F[d][ci + 1] = dop[d] * qr;
                         }
                     } else if (method == 1) {
-                        /* p3m-ik-i.c:187-212 */
+                        /* p3m-ik-i.c:202-227 */
                         Q[ci] *= T1;
                         Q[ci + 1] *= T1;
                         for (d = 0; d < 3; d++) {
@@ -717,7 +717,7 @@ int orc_kspace_force(int method, int N, double L, const double *x, const double 
                             F[d][ci + 1] = 2.0 * M_PI * Leni * dop[d] * Q[ci];
                         }
                     } else {
-                        /* p3m-ik-real.c:98-121 */
+                        /* p3m-ik-real.c:113-136 */
                         double qr = Q[ci] * T1, qi = Q[ci + 1] * T1;
                         for (d = 0; d < 3; d++) {
                             F[d][ci] = -2.0 * M_PI * Leni * dop[d] * qi;
@@ -761,7 +761,7 @@ int orc_kspace_force(int method, int N, double L, const double *x, const double 
 /* Real space                                                                */
 /* ------------------------------------------------------------------------ */
 
-static const double WUPI = 1.77245385090551602729816748334; /* realpart.c:120 */
+static const double WUPI = 1.77245385090551602729816748334; /* realpart.c:135 */
 
 static void pair_force(int t1, int t2, double L, double Li, const double *x, const double *y,
                        const double *z, const double *q, double rcut, double alpha,
@@ -786,7 +786,7 @@ static void pair_force(int t1, int t2, double L, double Li, const double *x, con
     }
 }
 
-/* realpart.c:109-152 */
+/* realpart.c:124-167 */
 void orc_realteil(int N, double L, const double *x, const double *y, const double *z,
                   const double *q, double rcut, double alpha, double *fr, double *energy)
 {
@@ -876,7 +876,7 @@ void orc_realteil_cells(int N, double L, const double *x, const double *y, const
 /* Ewald reference                                                           */
 /* ------------------------------------------------------------------------ */
 
-/* ewald.c:69-94.  G has (kmax+1)^3 entries; entries with n^2 > kmax^2 are 0. */
+/* ewald.c:84-109.  G has (kmax+1)^3 entries; entries with n^2 > kmax^2 are 0. */
 void orc_ewald_influence(int kmax, double L, double alpha, double *G)
 {
     double fak1 = 2.0 / SQR(L), fak2 = SQR(M_PI / (alpha * L));
@@ -893,7 +893,7 @@ void orc_ewald_influence(int kmax, double L, double alpha, double *G)
             }
 }
 
-/* ewald.c:107-148: fk += sum_n n q_i G(n) (Re S sin(k.r_i) + Im S cos(k.r_i)),
+/* ewald.c:122-163: fk += sum_n n q_i G(n) (Re S sin(k.r_i) + Im S cos(k.r_i)),
  * S = sum_j q_j exp(-i k.r_j), over the cube |n_d| <= kmax, n^2 <= kmax^2. */
 void orc_ewald_kspace(int N, double L, const double *x, const double *y, const double *z,
                       const double *q, double alpha, int kmax, const double *G, double *fk)
@@ -925,7 +925,7 @@ void orc_ewald_kspace(int N, double L, const double *x, const double *y, const d
             }
 }
 
-/* Kolafa-Perram real-space estimate (ewald.c:160-167) */
+/* Kolafa-Perram real-space estimate (ewald.c:175-182) */
 double orc_ewald_error_r(int N, double L, double q2, double rcut, double alpha)
 {
     (void)N;
@@ -933,14 +933,14 @@ double orc_ewald_error_r(int N, double L, double q2, double rcut, double alpha)
            (SQR(alpha) * rcut * rcut);
 }
 
-/* Petersen k-space estimate (ewald.c:169-183) */
+/* Petersen k-space estimate (ewald.c:184-198) */
 double orc_ewald_error_k(int N, double L, double q2, int kmax, double alpha)
 {
     return 2.0 * q2 * alpha * (1.0 / L) * sqrt(1.0 / (M_PI * kmax * N)) *
            exp(-SQR(M_PI * kmax / (alpha * L)));
 }
 
-/* bisection (ewald.c:189-214) */
+/* bisection (ewald.c:204-229) */
 double orc_ewald_optimal_alpha(int N, double L, double q2, double rcut, int kmax)
 {
     double lo = 0.01, hi = 10.0, flo, fhi, g, fg;
@@ -969,7 +969,7 @@ double orc_realspace_error(int N, double L, double q2, double rcut, double alpha
     return (2.0 * q2 * exp(-SQR(rcut * alpha))) / sqrt((double)N * rcut * L * L * L);
 }
 
-/* analytic_cotangent_sum (p3m-common.c:62-92) */
+/* analytic_cotangent_sum (p3m-common.c:77-107) */
 static double cotangent_sum(int n, double mesh_i, int cao)
 {
     double c = SQR(cos(M_PI * mesh_i * (double)n));
@@ -996,8 +996,8 @@ double orc_table_lookup(const double *table, double alphaL)
     return (1. - d) * table[l] + d * table[l + 1];
 }
 
-/* tuning alias sums of p3m-ik.c:289-322, p3m-ik-i.c:234-276,
- * p3m-ad.c:436-458, p3m-ad-i.c:190-235 */
+/* tuning alias sums of p3m-ik.c:304-337, p3m-ik-i.c:249-291,
+ * p3m-ad.c:436-458, p3m-ad-i.c:205-250 */
 static void alias_sums(int nx, int ny, int nz, int M, int cao, double L, double alpha, int mc,
                        double *a)
 {
@@ -1033,7 +1033,7 @@ double orc_kspace_error(int method, int N, double L, double q2, int M, int cao,
     double he_q = 0.0, mesh_i = 1.0 / M, a[8];
     int nx, ny, nz, h = M / 2;
     if (method == 0 || method == 6) {
-        /* p3m-ik.c:248-286: a tabulated value >= 0 is returned as is; otherwise
+        /* p3m-ik.c:263-301: a tabulated value >= 0 is returned as is; otherwise
          * the direct sum starts from the (negative) lookup result, -1 when the
          * mesh is not tabulated */
         he_q = (table != NULL) ? orc_table_lookup(table, alpha * L) : -1.0;
@@ -1055,7 +1055,7 @@ double orc_kspace_error(int method, int N, double L, double q2, int M, int cao,
         }
         he_q = fabs(he_q);
     } else if (method == 1) {
-        /* p3m-ik-i.c:280-305: a negative lookup restarts from 0 */
+        /* p3m-ik-i.c:295-320: a negative lookup restarts from 0 */
         he_q = (table != NULL) ? orc_table_lookup(table, alpha * L) : -1.0;
         if (he_q >= 0.0)
             return 2.0 * q2 * sqrt(he_q / (double)N) / SQR(L);
@@ -1081,7 +1081,7 @@ double orc_kspace_error(int method, int N, double L, double q2, int M, int cao,
                         he_q += a[0] - (SQR(a[2]) / (a[3] * a[4]));
                     }
     } else if (method == 3) {
-        /* p3m-ad-i.c:237-269 */
+        /* p3m-ad-i.c:252-284 */
         for (nx = -h; nx < h; nx++)
             for (ny = -h; ny < h; ny++)
                 for (nz = -h; nz < h; nz++)

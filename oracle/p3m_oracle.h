@@ -36,12 +36,12 @@ void orc_fft3d(double *c, int M, int sign);
 void orc_rfft3d(double *r, int M);   /* in-place r2c, FFTW padded layout */
 void orc_irfft3d(double *c, int M);  /* in-place c2r, FFTW padded layout */
 
-/* Differential operator and shifted wave numbers (p3m-common.c:95-128) */
+/* Differential operator and shifted wave numbers (p3m-common.c:110-143) */
 void orc_dn(int M, double *Dn);
 void orc_nshift(int M, double *ns);
 
-/* Influence functions (p3m-ik.c:93-158, p3m-ik-i.c:78-151, p3m-ad.c:69-143,
- * p3m-ad-i.c:70-135).  `method` is a method id; 6 uses ik, 7 uses ad. */
+/* Influence functions (p3m-ik.c:93-173, p3m-ik-i.c:78-166, p3m-ad.c:69-158,
+ * p3m-ad-i.c:70-150).  `method` is a method id; 6 uses ik, 7 uses ad. */
 int orc_influence(int method, int M, int cao, double L, double alpha, int mc, double *G);
 
 /* Self-force coefficients b_d(m'), 27 x 3 doubles (p3m-ad-self-forces.c:30-112) */
@@ -63,7 +63,7 @@ int orc_kspace_force(int method, int N, double L, const double *x, const double 
 int orc_assign_mesh(int method, int N, double L, const double *x, const double *y,
                     const double *z, const double *q, int M, int cao, double *mesh);
 
-/* Real-space part (realpart.c:109-152): all i != j, minimum image,
+/* Real-space part (realpart.c:124-167): all i != j, minimum image,
  * r <= rcut.  fr += forces; *energy += energy.  O(N^2). */
 void orc_realteil(int N, double L, const double *x, const double *y, const double *z,
                   const double *q, double rcut, double alpha, double *fr, double *energy);
@@ -72,7 +72,7 @@ void orc_realteil_cells(int N, double L, const double *x, const double *y, const
                         const double *q, double rcut, double alpha, double *fr, double *energy,
                         int i_begin, int i_end);
 
-/* Ewald reference (ewald.c:69-214, common.c:314-348) */
+/* Ewald reference (ewald.c:84-229, common.c:329-363) */
 void orc_ewald_influence(int kmax, double L, double alpha, double *G);
 void orc_ewald_kspace(int N, double L, const double *x, const double *y, const double *z,
                       const double *q, double alpha, int kmax, const double *G, double *fk);
@@ -80,8 +80,8 @@ double orc_ewald_error_r(int N, double L, double q2, double rcut, double alpha);
 double orc_ewald_error_k(int N, double L, double q2, int kmax, double alpha);
 double orc_ewald_optimal_alpha(int N, double L, double q2, double rcut, int kmax);
 
-/* Error estimates (realpart.c:343-346, p3m-ik.c:248-322, p3m-ik-i.c:234-305,
- * p3m-ad.c:436-497, p3m-ad-i.c:190-269, find_error.c:84-107).
+/* Error estimates (realpart.c:343-346, p3m-ik.c:249-337, p3m-ik-i.c:249-320,
+ * p3m-ad.c:436-497, p3m-ad-i.c:205-284, find_error.c:84-107).
  * `table` (101 values at alpha*L = 0,1,..,100) replaces the q_*.c lookup for
  * methods 0/1/6 when non-NULL; pass NULL for an untabulated mesh. */
 double orc_realspace_error(int N, double L, double q2, double rcut, double alpha);

@@ -380,7 +380,7 @@ class Solver:
 
 
 def realteil(system, rcut, alpha, forces=None):
-    """Realteil on the GPU (realpart.c:109-152): returns forces with f_r filled."""
+    """Realteil on the GPU (realpart.c:124-167): returns forces with f_r filled."""
     forces = forces or HostForces(system.n)
     p = parameters(2, 1, alpha, rcut)
     check(lib().p3m_hip_realteil(system.ptr, ctypes.pointer(p), forces.ptr))

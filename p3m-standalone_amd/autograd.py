@@ -66,7 +66,7 @@ def kspace_force(pos, q, solver):
 
 
 class TotalForce(torch.autograd.Function):
-    """Calculate_forces (common.c:253-312): f = f_k + f_r (ad / ad-r with the
+    """Calculate_forces (common.c:268-327): f = f_k + f_r (ad / ad-r with the
     self-force term), differentiable in positions and charges; the backward
     pass adds the k-space adjoint and the real-space adjoint
     (p3m_hip_realteil_vjp_device)."""

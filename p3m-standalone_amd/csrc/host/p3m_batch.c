@@ -45,7 +45,7 @@ static int batch_influence(p3m_hip_batch *b, int s) {
     TRYK(p3mk_batch_system_arrays(b->kb, s, &G, &Dn, &selfb), "batch arrays");
     TRYK(p3mk_influence(b->method, &g, S->alpha, 1, G, Dn, b->q), "influence function");
     if (p3mh_has_self_force(b->method)) {
-        /* p3m-ad.c:105-108: no self-force coefficients at alpha == 0 */
+        /* p3m-ad.c:120-123: no self-force coefficients at alpha == 0 */
         if (S->alpha != 0.0)
             TRYK(p3mk_self_force_coeffs(&g, G, selfb, b->scratch, b->q), "self force");
         else
@@ -125,8 +125,8 @@ int p3m_hip_batch_create(int method_id, int precision, int nsys, const p3m_hip_b
         in[s].cao = sys[s].cao;
         in[s].rcut = sys[s].rcut;
         in[s].alpha = sys[s].alpha;
-        /* back-interpolation prefactors: ik 1/(2 L^3) (p3m-ik.c:227),
-         * ad M/L^3 (p3m-ad.c:187) */
+        /* back-interpolation prefactors: ik 1/(2 L^3) (p3m-ik.c:242),
+         * ad M/L^3 (p3m-ad.c:202) */
         in[s].prefac = p3mh_is_ad(method_id) ? (double)sys[s].mesh / (L * L * L) : 1.0 / (2.0 * L * L * L);
     }
     rc = p3mk_queue_create(&b->q);
@@ -189,7 +189,7 @@ static int batch_forces_enqueue(p3m_hip_batch *b, const double *x, const double 
                                 double *energy) {
     const size_t n3 = 3 * (size_t)b->ntot;
     int s, any_rs = 0;
-    /* common.c:259-263 */
+    /* common.c:274-278 */
     TRYK(p3mk_fill_zero(f, sizeof(double) * n3, b->q), "zero");
     TRYK(p3mk_fill_zero(fk, sizeof(double) * n3, b->q), "zero");
     TRYK(p3mk_fill_zero(fr, sizeof(double) * n3, b->q), "zero");
@@ -200,9 +200,8 @@ static int batch_forces_enqueue(p3m_hip_batch *b, const double *x, const double 
     if (any_rs)
         TRYK(p3mk_batch_realspace(b->kb, x, y, z, q, fr, energy, b->q), "batched real space");
     TRYK(p3mk_batch_kspace(b->kb, x, y, z, q, fk, f, b->q), "batched k-space force");
-    /* common.c:300-304 */
-    TRYK(p3mk_axpy(n3, 1.0, fk, f, b->q), "sum");
-    TRYK(p3mk_axpy(n3, 1.0, fr, f, b->q), "sum");
+    /* common.c:315-319 */
+    TRYK(p3mk_sum_parts(n3, fk, fr, f, b->q), "sum");
     return P3M_HIP_OK;
 }
 

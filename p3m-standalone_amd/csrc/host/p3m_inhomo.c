@@ -90,7 +90,8 @@ static int ws_setup(inhomo_ws *w, const system_t *s, const p3mk_geom *g, int nme
     if (rc == P3MK_OK) rc = p3mk_copy_h2d(w->q, s->q, b, w->qu);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&flag, sizeof(int));
     if (rc == P3MK_OK)
-        rc = p3mk_check_positions(s->nparticles, s->length, w->x, w->y, w->z, &bad, flag, w->qu);
+        rc = p3mk_check_positions(s->nparticles, -s->length, 2.0 * s->length, w->x, w->y, w->z,
+                                  &bad, flag, w->qu);
     p3mk_release(flag);
     if (rc != P3MK_OK) {
         ws_release(w);
@@ -98,7 +99,7 @@ static int ws_setup(inhomo_ws *w, const system_t *s, const p3mk_geom *g, int nme
     }
     if (bad) {
         ws_release(w);
-        return p3mh_fail(P3M_HIP_ERANGE, "%d positions outside [0, L)", bad);
+        return p3mh_fail(P3M_HIP_ERANGE, "%d positions outside [-L, 2L) or not finite", bad);
     }
     return P3M_HIP_OK;
 }

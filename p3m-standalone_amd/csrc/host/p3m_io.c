@@ -178,7 +178,7 @@ int p3m_hip_read_system(const char *path, system_t **out) {
     f = fopen(path, "r");
     if (f == NULL)
         return p3mh_fail(P3M_HIP_EINVAL, "read_system: cannot open '%s': %s", path, strerror(errno));
-    /* io.c:125-126 */
+    /* io.c:126-127 */
     if (fscanf(f, "# Teilchenzahl: %d\n", &n) != 1 || fscanf(f, "# Len: %lf\n", &L) != 1 || n <= 0) {
         fclose(f);
         return p3mh_fail(P3M_HIP_EINVAL, "read_system: '%s' lacks the '# Teilchenzahl:' / '# Len:' header",
@@ -215,7 +215,7 @@ int p3m_hip_write_system(const system_t *s, const char *path) {
     f = fopen(path, "w");
     if (f == NULL)
         return p3mh_fail(P3M_HIP_EINVAL, "write_system: cannot open '%s': %s", path, strerror(errno));
-    /* io.c:214-223: six decimals */
+    /* io.c:215-221: six decimals */
     fprintf(f, "# Teilchenzahl: %d\n", s->nparticles);
     fprintf(f, "# Len: %lf\n", s->length);
     for (i = 0; i < s->nparticles; i++)

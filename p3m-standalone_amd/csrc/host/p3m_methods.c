@@ -194,7 +194,7 @@ int p3m_hip_influence_function(system_t *s, parameters_t *p, data_t *d) {
     c->geom.L = s->length;
     TRYK(p3mk_influence(c->method, &c->geom, p->alpha, 1, c->G, c->Dn, c->q), "influence function");
     /* ad / ad-r also recompute the self-force coefficients, except at
-     * alpha == 0 where the reference returns early (p3m-ad.c:105-108,138-142) */
+     * alpha == 0 where the reference returns early (p3m-ad.c:120-123,138-142) */
     if (has_self_force(c->method) && p->alpha != 0.0) {
         TRYK(p3mk_self_force_coeffs(&c->geom, c->G, c->selfb, c->scratch, c->q), "self force");
         if (d->self_force_corrections != NULL)
@@ -299,16 +299,16 @@ static int field_enqueue(p3m_ctx *c, int n, const double *x, const double *y, co
     TRYK(p3mk_record(c->q, 1), "event");
     TRYK(p3mk_fft_forward_zy(fft, c->mesh, spec, c->q), "forward FFT");
     if (!is_ad(c->method)) {
-        /* ik: F_d = i (2 pi/L) Dn_d G Q^ (p3m-ik.c:196-216, p3m-ik-i.c:187-212,
-         * p3m-ik-real.c:98-121), three backward transforms */
+        /* ik: F_d = i (2 pi/L) Dn_d G Q^ (p3m-ik.c:211-231, p3m-ik-i.c:202-227,
+         * p3m-ik-real.c:113-136), three backward transforms */
         for (k = 0; k < 3; k++)
             outs[k] = is_r2c(c->method) ? c->spec[k] : c->F[k];
         TRYK(p3mk_convolve(fft, 3, c->G, c->Dn, spec, outs, c->q), "convolution");
         for (k = 0; k < 3; k++)
             TRYK(p3mk_fft_backward_yz(fft, outs[k], c->F[k], c->q), "backward FFT");
     } else {
-        /* ad: phi = FFT^-1[G Q^] (p3m-ad.c:167-181, p3m-ad-i.c:161-175,
-         * p3m-ad-real.c:75-89) */
+        /* ad: phi = FFT^-1[G Q^] (p3m-ad.c:182-196, p3m-ad-i.c:176-190,
+         * p3m-ad-real.c:90-104) */
         outs[0] = spec;
         outs[1] = outs[2] = NULL;
         TRYK(p3mk_convolve(fft, 1, c->G, c->Dn, spec, outs, c->q), "convolution");
@@ -318,8 +318,8 @@ static int field_enqueue(p3m_ctx *c, int n, const double *x, const double *y, co
     return P3M_HIP_OK;
 }
 
-/* prefactor of the back-interpolation sums: 1/(2 L^3) for ik (p3m-ik.c:227),
- * M/L^3 for ad (p3m-ad.c:187) */
+/* prefactor of the back-interpolation sums: 1/(2 L^3) for ik (p3m-ik.c:242),
+ * M/L^3 for ad (p3m-ad.c:202) */
 static double interp_prefactor(const p3m_ctx *c) {
     const double L = c->geom.L;
     return is_ad(c->method) ? (double)c->geom.M / (L * L * L) : 1.0 / (2.0 * L * L * L);
@@ -335,7 +335,7 @@ static int kspace_enqueue(p3m_ctx *c, int n, const double *x, const double *y, c
              "back-interpolation");
     } else {
         /* assign_forces_ad* and the self-force subtraction of methods 2 and 7
-         * (p3m-ad.c:189-191) */
+         * (p3m-ad.c:204-206) */
         TRYK(p3mk_interp_ad(g, n, c->psort, c->mesh, interp_prefactor(c),
                             has_self_force(c->method) ? c->selfb : NULL, fk, f, c->q),
              "back-interpolation");
@@ -404,11 +404,16 @@ static int upload_positions(p3m_ctx *c, const system_t *s) {
     TRYK(p3mk_copy_h2d(c->pos + n, s->p->y, b, c->q), "upload");
     TRYK(p3mk_copy_h2d(c->pos + 2 * n, s->p->z, b, c->q), "upload");
     TRYK(p3mk_copy_h2d(c->pos + 3 * n, s->q, b, c->q), "upload");
-    TRYK(p3mk_check_positions((int)n, s->length, c->pos, c->pos + n, c->pos + 2 * n, &bad,
-                              c->iscratch, c->q),
+    /* the reference wraps mesh indices by one period (charge-assign.c:37-39),
+     * so it takes positions up to one box length outside [0, L) */
+    TRYK(p3mk_check_positions((int)n, -s->length, 2.0 * s->length, c->pos, c->pos + n,
+                              c->pos + 2 * n, &bad, c->iscratch, c->q),
          "position check");
     if (bad)
-        return p3mh_fail(P3M_HIP_ERANGE, "%d particles outside [0, L) or not finite", bad);
+        return p3mh_fail(P3M_HIP_ERANGE,
+                         "%d particles outside [-L, 2L) (the one-period tolerance of the "
+                         "reference's mesh wrap) or not finite",
+                         bad);
     return P3M_HIP_OK;
 }
 
@@ -539,18 +544,17 @@ int p3m_hip_calculate_forces_device(data_t *d, int n, double length, const doubl
         return p3mh_fail(P3M_HIP_EINVAL, "force arrays must not be NULL");
     if (n != c->n)
         return p3mh_fail(P3M_HIP_EINVAL, "particle count %d differs from Init's %d", n, c->n);
-    /* common.c:259-263 */
+    /* common.c:274-278 */
     TRYK(p3mk_fill_zero(f, sizeof(double) * 3 * (size_t)n, c->q), "zero");
     TRYK(p3mk_fill_zero(fk, sizeof(double) * 3 * (size_t)n, c->q), "zero");
     TRYK(p3mk_fill_zero(fr, sizeof(double) * 3 * (size_t)n, c->q), "zero");
     if (energy != NULL)
         TRYK(p3mk_fill_zero(energy, sizeof(double), c->q), "zero");
-    if (rcut != 0.0) /* common.c:275-279 */
+    if (rcut != 0.0) /* common.c:290-294 */
         TRY(p3m_hip_realteil_device(d, n, length, x, y, z, q, rcut, alpha, fr, energy));
     TRY(p3m_hip_kspace_force_device(d, n, length, x, y, z, q, fk, f));
-    /* common.c:300-304: f += f_k + f_r */
-    TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fk, f, c->q), "sum");
-    TRYK(p3mk_axpy(3 * (size_t)n, 1.0, fr, f, c->q), "sum");
+    /* common.c:315-319: f += f_k + f_r */
+    TRYK(p3mk_sum_parts(3 * (size_t)n, fk, fr, f, c->q), "sum");
     if (c->ring_rs != NULL) {
         int k = p3mk_ring_filled(c->q);
         if (k < c->ring_sets)
@@ -765,7 +769,7 @@ int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, da
     if (f == NULL || f->f == NULL || f->f_k == NULL || f->f_r == NULL)
         return p3mh_fail(P3M_HIP_EINVAL, "forces must not be NULL");
     n = (size_t)s->nparticles;
-    /* common.c:259-263: all three force arrays start from zero */
+    /* common.c:274-278: all three force arrays start from zero */
     for (dim = 0; dim < 3; dim++) {
         memset(field(f->f, dim), 0, sizeof(double) * n);
         memset(field(f->f_k, dim), 0, sizeof(double) * n);
@@ -795,7 +799,7 @@ int p3m_hip_calculate_forces(const method_t *m, system_t *s, parameters_t *p, da
         for (i = 0; i < n; i++) {
             fk[i] = hk[i];
             fr[i] = hr[i];
-            ft[i] = hs[i] + (fk[i] + fr[i]); /* common.c:300-304 */
+            ft[i] = hs[i] + (fk[i] + fr[i]); /* common.c:315-319 */
         }
     }
     s->energy += energy;
@@ -877,13 +881,13 @@ static int kspace_error(int id, const system_t *s, const parameters_t *p, double
     const int M = p->mesh;
     const double L = s->length, N = (double)s->nparticles;
     if (id == 0 || id == 6 || id == 1) {
-        /* p3m-ik.c:252-286, p3m-ik-i.c:286-304 */
+        /* p3m-ik.c:263-301, p3m-ik-i.c:295-320 */
         const double t = table_lookup(id == 1 ? 1 : 0, M, p->cao, p->alpha * L);
         if (t >= 0.0) {
             *out = 2.0 * s->q2 * sqrt(t / N) / (L * L);
             return P3M_HIP_OK;
         }
-        he_q = (id == 1) ? 0.0 : t; /* ik keeps the lookup's -1 (p3m-ik.c:255-274) */
+        he_q = (id == 1) ? 0.0 : t; /* ik keeps the lookup's -1 (p3m-ik.c:267-297) */
     }
     if (p3mk_queue_create(&q) != P3MK_OK)
         return p3mh_kfail(P3M_HIP_ENODEVICE, "stream");
@@ -899,7 +903,7 @@ static int kspace_error(int id, const system_t *s, const parameters_t *p, double
     p3mk_queue_destroy(q);
     if (rc != P3M_HIP_OK)
         return rc;
-    if (id != 2 && id != 7) /* ad has no fabs (p3m-ad.c:497) */
+    if (id != 2 && id != 7) /* ad has no fabs (p3m-ad.c:496) */
         he_q = fabs(he_q);
     *out = 2.0 * s->q2 * sqrt(he_q / N) / (L * L);
     return P3M_HIP_OK;
@@ -992,7 +996,7 @@ int p3m_hip_kspace_force_vjp_device(data_t *d, int n, double length, const doubl
         TRYK(p3mk_fft_backward_yz(fft, in[0], c->adj[0], c->q), "backward FFT");
         F[0] = c->mesh;
         F[1] = F[2] = NULL;
-        /* the ad derivative weights carry 1/L (p3m-ad.c:187 with
+        /* the ad derivative weights carry 1/L (p3m-ad.c:202 with
          * charge-assign.c:1123-1125) */
         TRYK(p3mk_adjoint_gather(g, 1, n, c->psort, q, gF, F, c->adj[0], interp_prefactor(c) / L,
                                  has_self_force(c->method) ? c->selfb : NULL, dpos, dq, c->q),

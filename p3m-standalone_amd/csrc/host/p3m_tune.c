@@ -16,7 +16,7 @@
  *     smaller in mesh and order than the best is not timed;
  *   - the winner has the smallest sum of per-stage minimum times.
  * One deliberate difference: the reference's mesh loop runs one index past
- * its list (tuning.c:300 reads smooth_numbers[smooth_numbers_n]); here the
+ * its list (tuning.c:295-296 reads smooth_numbers[smooth_numbers_n]); here the
  * scan ends with the list, and the "no best yet" mesh is INT_MAX.
  *
  * Timing (time_full, tuning.c:92-147): a data_t made with p->tuning set,

@@ -18,7 +18,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr double kPi = 3.14159265358979323846;
 
-// p3m-common.c:118-128 / :95-116
+// p3m-common.c:133-143 / :95-116
 __device__ __forceinline__ double nshift_of(int i, int M) {
     return (double)i - round((double)i / (double)M) * (double)M;
 }
@@ -26,7 +26,7 @@ __device__ __forceinline__ double dn_of(int i, int M) {
     return (i == M / 2) ? 0.0 : nshift_of(i, M);
 }
 
-// p3m-common.c:56-60
+// p3m-common.c:71-75
 __device__ __forceinline__ double sinc(double d) {
     const double pd = kPi * d;
     return (d == 0.0) ? 1.0 : sin(pd) / pd;
@@ -67,7 +67,7 @@ k_influence(int kind, int M, int cao, double L, double alpha, int mc, double *__
         return;
     }
     if (kind <= 1 && (NX % h == 0) && (NY % h == 0) && (NZ % h == 0)) {
-        // ik corner zeros (p3m-ik.c:141-142, p3m-ik-i.c:132-133)
+        // ik corner zeros (p3m-ik.c:156-157, p3m-ik-i.c:147-148)
         G[ind] = 0.0;
         return;
     }
@@ -92,14 +92,14 @@ k_influence(int kind, int M, int cao, double L, double alpha, int mc, double *__
                 const bool even = ((MX + MY + MZ) % 2) == 0;
                 N1 += S3;
                 if (kind <= 1) {
-                    // p3m-ik.c:104-113, p3m-ik-i.c:89-103
+                    // p3m-ik.c:119-128, p3m-ik-i.c:104-118
                     const double zwi = S3 * TE / NM2;
                     Z0 += NMX * zwi * Leni;
                     Z1 += NMY * zwi * Leni;
                     Z2 += NMZ * zwi * Leni;
                     N2 += even ? S3 : -S3;
                 } else {
-                    // p3m-ad.c:82-90, p3m-ad-i.c:83-99
+                    // p3m-ad.c:97-105, p3m-ad-i.c:98-114
                     N2 += S3 * NM2;
                     N3 += even ? S3 : -S3;
                     N4 += even ? S3 * NM2 : -S3 * NM2;
@@ -115,16 +115,16 @@ k_influence(int kind, int M, int cao, double L, double alpha, int mc, double *__
         const double d2 = (Dnx * Leni) * (Dnx * Leni) + (Dny * Leni) * (Dny * Leni) +
                           (Dnz * Leni) * (Dnz * Leni);
         if (kind == 0) {
-            zwi /= d2 * (N1 * N1); // p3m-ik.c:148-150
+            zwi /= d2 * (N1 * N1); // p3m-ik.c:163-165
         } else {
-            zwi /= d2;               // p3m-ik-i.c:142-146
+            zwi /= d2;               // p3m-ik-i.c:157-161
             zwi /= 0.5 * (N1 * N1 + N2 * N2);
         }
         g = 2.0 * zwi / kPi;
     } else if (kind == 2) {
-        g = Zad / (kPi * N1 * N2); // p3m-ad.c:128
+        g = Zad / (kPi * N1 * N2); // p3m-ad.c:143
     } else {
-        g = Zad / (0.5 * kPi * (N1 * N2 + N3 * N4)); // p3m-ad-i.c:130
+        g = Zad / (0.5 * kPi * (N1 * N2 + N3 * N4)); // p3m-ad-i.c:145
     }
     G[ind] = g;
 }
@@ -222,8 +222,8 @@ __global__ void k_self_force_finish(const double *__restrict__ partials, int nbl
     }
 }
 
-// ---- k-space error sums (p3m-ik.c:248-322, p3m-ik-i.c:234-305,
-//      p3m-ad.c:436-497, p3m-ad-i.c:190-269) ----------------------------------
+// ---- k-space error sums (p3m-ik.c:249-337, p3m-ik-i.c:249-320,
+//      p3m-ad.c:436-497, p3m-ad-i.c:205-284) ----------------------------------
 
 __device__ __forceinline__ double cotangent_sum(int n, double mesh_i, int cao) {
     const double cc = cos(kPi * mesh_i * (double)n);
@@ -322,7 +322,7 @@ extern "C" int p3mk_influence(int method, const p3mk_geom *g, double alpha, int 
         P3MK_TRY(p3mk_launch_check("Dn"));
     }
     const long M3 = (long)M * M * M;
-    if (kind == 2 && alpha == 0.0) // p3m-ad.c:105-108
+    if (kind == 2 && alpha == 0.0) // p3m-ad.c:120-123
         return p3mk_fill_zero(G, sizeof(double) * M3, q);
     hipLaunchKernelGGL(k_influence, dim3((unsigned)((M3 + kThreads - 1) / kThreads)), dim3(kThreads),
                        0, s, kind, M, g->cao, g->L, alpha, mc, G);

@@ -36,8 +36,10 @@
 //
 // Periodic images are a per-neighbour-cell shift of 0 or +-L, which equals
 // the reference's round(dx/L)*L for every pair inside the cutoff
-// (rcut <= L/2), so dx = (x_i - x_j) - shift is the reference's dx bit for
-// bit.
+// (rcut <= L/2), so for positions inside [0, L) dx = (x_i - x_j) - shift is
+// the reference's dx bit for bit.  Positions outside the box are folded into
+// it first; their dx then differs from the reference's raw difference by
+// rounding only.
 
 #include <hip/hip_runtime.h>
 
@@ -230,7 +232,7 @@ __device__ __forceinline__ double bcast(double v, int k) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// 2 alpha / sqrt(pi) with the reference's sqrt(pi) literal (realpart.c:120)
+// 2 alpha / sqrt(pi) with the reference's sqrt(pi) literal (realpart.c:135)
 __device__ __forceinline__ double sqrt_pi() { return 1.77245385090551602729816748334; }
 
 struct PairAcc {
@@ -275,7 +277,7 @@ __device__ __forceinline__ float erfcx_f(float x) {
     return acc;
 }
 
-// The reference's inclusion test r <= rcut (realpart.c:134-148) on its dx,
+// The reference's inclusion test r <= rcut (realpart.c:149-163) on its dx,
 // decided exactly: r2 against rcut^2, with the correctly rounded sqrt only
 // within 1e-12 of the edge.
 __device__ __forceinline__ bool within_cutoff(double dx, double dy, double dz, double rcut) {
@@ -776,7 +778,7 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // home particle).  The owner receives f and the neighbour -f.  The reverse
 // pair's difference is -dx exactly, so its r, its inclusion and its force
 // factor are the same, and -f is the force the reference computes for it bit
-// for bit (realpart.c:134-148).  Forces are summed in sorted order
+// for bit (realpart.c:149-163).  Forces are summed in sorted order
 // (fs[d stride + i]) and added to the caller's array by perm afterwards.
 //
 // Neighbour particles within rcut of the item's box are staged in a window

@@ -274,7 +274,27 @@ extern "C" int p3mk_axpy(size_t n, double a, const double *x, double *y, p3mk_qu
     return p3mk_launch_check("axpy");
 }
 
-__global__ void k_check_positions(int n, double L, const double *__restrict__ x,
+// f[i] += fk[i] + fr[i]: the sum of Calculate_forces in its order
+// (common.c:315-319), so the total force rounds as the reference's does.
+__global__ void k_sum_parts(size_t n, const double *__restrict__ fk, const double *__restrict__ fr,
+                            double *__restrict__ f) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        f[i] += fk[i] + fr[i];
+}
+
+extern "C" int p3mk_sum_parts(size_t n, const double *fk, const double *fr, double *f, p3mk_queue *q) {
+    if (n == 0)
+        return P3MK_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)blocks), dim3(256), 0, q->stream, n, fk, fr, f);
+    return p3mk_launch_check("sum_parts");
+}
+
+__global__ void k_check_positions(int n, double lo, double hi, const double *__restrict__ x,
                                   const double *__restrict__ y, const double *__restrict__ z,
                                   int *bad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -282,7 +302,7 @@ __global__ void k_check_positions(int n, double L, const double *__restrict__ x,
     if (i < n) {
         const double v[3] = {x[i], y[i], z[i]};
         for (int d = 0; d < 3; d++)
-            b |= !(v[d] >= 0.0 && v[d] < L);
+            b |= !(v[d] >= lo && v[d] < hi);
     }
     // one atomic per wavefront
     unsigned long long m = __ballot(b);
@@ -290,7 +310,7 @@ __global__ void k_check_positions(int n, double L, const double *__restrict__ x,
         atomicAdd(bad, (int)__popcll(m));
 }
 
-extern "C" int p3mk_check_positions(int n, double L, const double *x, const double *y,
+extern "C" int p3mk_check_positions(int n, double lo, double hi, const double *x, const double *y,
                                     const double *z, int *bad, int *scratch, p3mk_queue *q) {
     *bad = 0;
     if (n <= 0)
@@ -298,8 +318,8 @@ extern "C" int p3mk_check_positions(int n, double L, const double *x, const doub
     int rc = p3mk_fill_zero(scratch, sizeof(int), q);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, q->stream, n, L,
-                       x, y, z, scratch);
+    hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, q->stream, n, lo,
+                       hi, x, y, z, scratch);
     rc = p3mk_launch_check("check_positions");
     if (rc)
         return rc;

@@ -54,9 +54,11 @@ int p3mk_copy_d2d(void *dst, const void *src, size_t bytes, p3mk_queue *q);
 int p3mk_fill_zero(void *p, size_t bytes, p3mk_queue *q);
 /* y[i] += a * x[i] for i < n (FP64) */
 int p3mk_axpy(size_t n, double a, const double *x, double *y, p3mk_queue *q);
-/* positions inside [0, L) and finite?  *bad receives the count of others */
-int p3mk_check_positions(int n, double L, const double *x, const double *y, const double *z,
-                         int *bad, int *scratch, p3mk_queue *q);
+/* f[i] += fk[i] + fr[i] for i < n (FP64; Calculate_forces' order) */
+int p3mk_sum_parts(size_t n, const double *fk, const double *fr, double *f, p3mk_queue *q);
+/* positions inside [lo, hi) (not NaN)?  *bad receives the count of others */
+int p3mk_check_positions(int n, double lo, double hi, const double *x, const double *y,
+                         const double *z, int *bad, int *scratch, p3mk_queue *q);
 
 /* ---- mesh geometry --------------------------------------------------- */
 typedef struct {

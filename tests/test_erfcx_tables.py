@@ -1,6 +1,6 @@
 """The erfcx tables compiled into the real-space kernel: evaluated on the
 CPU exactly as the kernel evaluates them, erfc(x) = exp(-x^2) erfcx(x)
-matches libm's erfc (the reference's ERFC, realpart.c:139)."""
+matches libm's erfc (the reference's ERFC, realpart.c:154)."""
 import math
 import os
 import re

@@ -142,6 +142,39 @@ def test_calculate_forces_tests_wall(pkg, orc):
     solver.close()
 
 
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
+def test_positions_one_period_outside(pkg, orc, method):
+    """The reference takes positions up to one period outside the box (its
+    mesh-index wrap, charge-assign.c:37-39): particles moved by +-L give the
+    oracle's forces for the same raw positions, and those of the folded
+    system to rounding; positions beyond [-L, 2L) are rejected."""
+    s = random_system(800, 10.0, seed=70 + method)
+    L, M, cao, alpha = 10.0, 16, 5, 1.1
+    moved = dict(s)
+    rng = np.random.default_rng(method)
+    for k in "xyz":
+        v = np.array(s[k], copy=True)
+        pick = rng.random(v.size) < 0.3
+        v[pick] = np.where(v[pick] >= 0.5 * L, v[pick] - L, v[pick] + L)
+        moved[k] = v
+    solver = pkg.Solver(method, pkg.HostSystem.from_dict(moved), pkg.parameters(M, cao, alpha))
+    got = solver.kspace_force()
+    fk_ref, f_ref = orc.kspace_force(method, moved, M, cao, alpha)
+    fk_in, _ = orc.kspace_force(method, s, M, cao, alpha)
+    assert rel_err(got.kspace(), fk_ref) < FP64_TOL
+    assert rel_err(got.total() + got.kspace(), f_ref + fk_ref) < FP64_TOL
+    assert rel_err(got.kspace(), fk_in) < 1e-9
+    far = dict(moved)
+    far["x"] = np.array(moved["x"], copy=True)
+    far["x"][3] = 2.0 * L
+    solver2 = pkg.Solver(method, pkg.HostSystem.from_dict(far), pkg.parameters(M, cao, alpha))
+    with pytest.raises(pkg.P3MError) as e:
+        solver2.kspace_force()
+    assert e.value.code == pkg.ERANGE
+    solver.close()
+    solver2.close()
+
+
 @pytest.mark.parametrize("method", [1, 3, 7])
 def test_mixed_precision(pkg, orc, method):
     """FP32 mesh / FFT / interpolation: close to the FP64 result, far below

@@ -1,6 +1,6 @@
 """Real space on cell grids (the half-shell kernel: each unordered pair once,
 the reaction added to the partner) against the oracle's Realteil
-(realpart.c:109-152): dense cells of several 64-particle chunks next to
+(realpart.c:124-167): dense cells of several 64-particle chunks next to
 sparse ones, clusters that wrap across the periodic boundary, both cell
 sizes (side rcut / 2, and side rcut for sparse systems), the device entry
 point accumulating into the caller's arrays, and the mixed form."""

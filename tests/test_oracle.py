@@ -106,6 +106,36 @@ def test_locate_quantisation(orc):
         assert 0 <= arg <= 131072
 
 
+@pytest.mark.parametrize("method", [0, 2, 3])
+def test_force_is_constant_inside_a_quantisation_bin(orc, method):
+    """The weights are evaluated at positions quantised to 1/131072 of a mesh
+    spacing with floor (charge-assign.c:153-156, interpol.c:53-60): moving a
+    particle anywhere inside its bin, lower edge included, leaves every force
+    bit-identical; moving it onto the next bin's lower edge changes them.
+    With L/M = 1 the bin edges are the exact binary fractions j/131072 (also
+    on the h/2-shifted mesh of the interlaced method and for odd and even
+    cao), independent of the oracle's own formula."""
+    L, M, cao, alpha = 16.0, 16, 5 if method != 2 else 4, 1.0
+    s = random_system(40, L, seed=11 + method)
+    j = 5
+    base = int(s["x"][j] * 131072)
+    forces = []
+    for xj in ((base + 0.5) / 131072, base / 131072, (base + 1) / 131072 - 2.0 ** -40,
+               (base + 1) / 131072):
+        t = dict(s)
+        t["x"] = np.array(s["x"], copy=True)
+        t["x"][j] = xj
+        fk, _ = orc.kspace_force(method, t, M, cao, alpha)
+        # (the ad self-force term is a smooth function of the raw position,
+        # p3m-ad-self-forces.c:122-135, so only the mesh force is compared)
+        forces.append(fk)
+    assert np.array_equal(forces[0], forces[1])
+    assert np.array_equal(forces[0], forces[2])
+    assert not np.array_equal(forces[0], forces[3])
+    # the step across one bin edge is of the size of a 1/131072 move
+    assert 0.0 < np.abs(forces[3] - forces[0]).max() < 1e-3 * np.abs(forces[0]).max()
+
+
 # ---------------------------------------------------------------------------
 # FFT
 # ---------------------------------------------------------------------------
@@ -168,6 +198,86 @@ def random1000(orc):
     return s, fref
 
 
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_influence_function_symmetries(orc, method):
+    """G(n) = G(-n) for every n off the Nyquist planes (n_d = M/2, where the
+    reference's nshift is one-sided); G(0) = 0; for ik and ik-i G vanishes
+    wherever every component is 0 or M/2 (p3m-ik.c:156-157,
+    p3m-ik-i.c:147-148) and nowhere else; ad vanishes only at n = 0."""
+    M, cao, L, alpha = 12, 4, 9.0, 1.1
+    G = orc.influence(method, M, cao, L, alpha).reshape(M, M, M)
+    assert G[0, 0, 0] == 0.0
+    idx = np.arange(M)
+    neg = (-idx) % M
+    inner = idx != M // 2
+    Gi = G[np.ix_(inner, inner, inner)]
+    Gn = G[np.ix_(neg[inner], neg[inner], neg[inner])]
+    assert np.allclose(Gi, Gn, rtol=1e-13, atol=0.0)
+    corner = np.zeros((M, M, M), dtype=bool)
+    c = np.array([0, M // 2])
+    corner[np.ix_(c, c, c)] = True
+    if method in (0, 1):
+        assert np.all(G[corner] == 0.0)
+        assert np.all(G[~corner] != 0.0)
+    else:
+        assert np.count_nonzero(G == 0.0) == 1
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
+def test_influence_function_is_optimal(orc, random1000, method):
+    """The influence function minimises the force error of its method: at
+    an aliasing-dominated setting (M = 16, cao 5, alpha 1.2) the measured
+    rms of the k-space force against the Ewald k-space force rises when G is
+    scaled by 1 +- 5 % or built without the aliasing sums (mc = 0).  With
+    interlacing the odd aliasing images cancel, so dropping the sums there
+    may not lower the error either (held to 0.1 %)."""
+    s, _ = random1000
+    M, cao, alpha, L = 16, 5, 1.2, 10.0
+    fE = orc.ewald_kspace(s, alpha, 18)
+
+    def rms(G):
+        b = orc.self_force_coeffs(M, cao, L, G) if method in (2, 7) else None
+        fk, fs = orc.kspace_force(method, s, M, cao, alpha, G=G, b=b)
+        return orc.rms(fk + fs, fE)
+
+    G = orc.influence(method, M, cao, L, alpha)
+    best = rms(G)
+    assert rms(1.05 * G) > 2.0 * best
+    assert rms(0.95 * G) > 2.0 * best
+    mc0 = rms(orc.influence(method, M, cao, L, alpha, mc=0))
+    print("method %d: rms %.4e, without aliasing sums %.4e" % (method, best, mc0))
+    if method in (1, 3):
+        assert mc0 > 0.999 * best
+    else:
+        assert mc0 > 1.01 * best
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_ad_estimate_band(orc, random1000, method):
+    """The ad and ad-i k-space estimates against the measured k-space rms
+    (Ewald k-space at the same alpha) over M = 16, 24, 32, cao 5 and 7 and
+    alpha 0.8 ... 1.6, where the estimate is above the 1/131072 weight
+    quantisation floor (~2e-6 here): ad within 1.1-1.9 (the estimate leaves
+    out the self-force term the correction removes only in part), ad-i
+    within 1.1-2.6 (no self-force correction; the estimate skips every k with
+    a zero component, p3m-ad-i.c:271)."""
+    s, _ = random1000
+    lo, hi = (1.1, 1.9) if method == 2 else (1.1, 2.6)
+    seen = 0
+    for alpha in (0.8, 1.2, 1.6):
+        fE = orc.ewald_kspace(s, alpha, 22)
+        for M in (16, 24, 32):
+            for cao in (5, 7):
+                est = orc.kspace_error(method, 1000, 10.0, 1000.0, M, cao, alpha)
+                if est < 1e-5:
+                    continue
+                fk, fs = orc.kspace_force(method, s, M, cao, alpha)
+                r = orc.rms(fk + fs, fE) / est
+                assert lo < r < hi, (M, cao, alpha, r)
+                seen += 1
+    assert seen >= 10
+
+
 @pytest.mark.parametrize("method", [0, 1, 2, 3, 6, 7])
 @pytest.mark.parametrize("alpha", [0.6, 1.0, 1.4])
 def test_p3m_error_matches_estimate(orc, random1000, method, alpha):
@@ -182,7 +292,7 @@ def test_p3m_error_matches_estimate(orc, random1000, method, alpha):
     q2 = 1000.0
     est_k = orc.kspace_error(method if method not in (0, 6) else 1, 1000, 10.0, q2, M, cao, alpha)
     if method in (0, 6):
-        # the ik estimate carries the reference's -1 start (p3m-ik.c:252-282);
+        # the ik estimate carries the reference's -1 start (p3m-ik.c:267-297);
         # compare with the interlaced-free direct sum instead
         est_k = orc.kspace_error(2, 1000, 10.0, q2, M, cao, alpha)
     est = math.hypot(orc.realspace_error(1000, 10.0, q2, rc, alpha), est_k)
@@ -228,7 +338,7 @@ def test_self_force_coefficients_antisymmetric(orc):
     for a in range(27):
         m = (a // 9 - 1, (a // 3) % 3 - 1, a % 3 - 1)
         neg = (-m[0] + 1) * 9 + (-m[1] + 1) * 3 + (-m[2] + 1)
-        # only approximately: nshift[M/2] = -M/2 (p3m-common.c:126) makes the
+        # only approximately: nshift[M/2] = -M/2 (p3m-common.c:141) makes the
         # aliasing sums of the Nyquist planes one-sided
         assert np.allclose(b[a], -b[neg], rtol=1e-3, atol=1e-15 * np.abs(b).max())
         for d in range(3):
