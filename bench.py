@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector / matrix, dense
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix, dense
+FP32_UNPACKED_TFLOPS = 78.6  # FP32 vector without packed (two-wide) instructions
 
 WORKLOADS = {
     "gaussian-1M-ad-i-m128": dict(method=3, N=1_000_000, L=100.0, M=128, cao=7, rcut=4.0, alpha=0.75,
@@ -250,26 +251,56 @@ def roofline_object(dom, per, workload):
     kernel = STAGE_KERNEL.get(dom, dom)
     traffic = measured_traffic(workload, kernel)
     if dom == "realspace":
-        return {"bound": "mfma", "kernel": kernel + " (cell lists + half-shell pair kernel, FP32 pairs)",
+        return {"bound": "valu", "kernel": kernel + " (cell lists + half-shell pair kernel, FP32 pairs)",
                 "achieved": st["TFLOP/s"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": st["frac_valu"], "traffic": traffic,
-                "note": "compute bound (vector ALUs and LDS atomics); achieved = %d ordered pairs "
-                        "within rcut / 2 (each unordered pair once) x %d flops / median stage time; "
-                        "peak = the published FP32 figure, which counts packed FP32 (built without "
-                        "packed FP32 here: 78.6 TFLOP/s unpacked); traffic = HBM bytes per launch "
-                        "from profiles/traffic_%s.md" % (st["pairs"], HALF_PAIR_FLOPS, workload)}
+                "peak_unpacked_fp32": FP32_UNPACKED_TFLOPS,
+                "frac_unpacked_fp32": round(st["TFLOP/s"] / FP32_UNPACKED_TFLOPS, 4),
+                "note": "vector-ALU bound (pair arithmetic in FP32, FP64 LDS accumulation; the only "
+                        "matrix-core use is the FP16 candidate distance test); achieved = %d ordered "
+                        "pairs within rcut / 2 (each unordered pair once) x %d flops / median stage "
+                        "time; peak = the published FP32 figure, which counts packed FP32 (the build "
+                        "has packed FP32 off: 78.6 TFLOP/s unpacked, frac_unpacked_fp32); traffic = "
+                        "HBM bytes per launch from profiles/traffic_%s.md"
+                        % (st["pairs"], HALF_PAIR_FLOPS, workload)}
     return {"bound": "hbm", "kernel": kernel, "achieved": st["GB/s"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": st["frac_hbm"], "traffic": traffic}
 
 
+def host_cpu():
+    """(logical CPUs usable by this process, CPU model name)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return ncpu, model
+
+
 def cpu_baseline(w, sysd):
-    """The oracle (plain-C FP64 restatement of the reference) on this host:
-    one full k-space evaluation plus the real-space force of a 2000-particle
-    sample (cell-list restatement of Realteil), scaled to N."""
+    """The oracle (plain-C FP64 restatement of the reference: the reference
+    itself needs FFTW and GSL, absent here) on this host: one full k-space
+    evaluation plus the real-space force of a 2000-particle sample
+    (cell-list restatement of Realteil; the reference's O(N^2) loop is
+    infeasible at this size), scaled to N.  One thread, as the reference's
+    default build runs; beside it the same with the real-space rows split
+    over all usable host CPUs (threads; ctypes releases the interpreter
+    lock).  The reference's assignment, FFT and back-interpolation carry no
+    OpenMP pragmas, so k space stays on one thread in both."""
+    from concurrent.futures import ThreadPoolExecutor
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     N, L, M, cao, alpha = w["N"], w["L"], w["M"], w["cao"], w["alpha"]
+    ncpu, model = host_cpu()
     G = orc.influence(w["method"], M, cao, L, alpha)
     b = orc.self_force_coeffs(M, cao, L, G) if w["method"] in (2, 7) else None
     t0 = time.perf_counter()
@@ -280,9 +311,22 @@ def cpu_baseline(w, sysd):
     orc.realteil(sysd, w["rcut"], alpha, cells=True, i_range=(0, sample))
     tr = (time.perf_counter() - t0) * N / sample
     t_eval = tk + tr
+    # all cores: ncpu times the sample, one slice per thread
+    big = min(N, sample * ncpu)
+    bounds = np.linspace(0, big, ncpu + 1).astype(int)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(ncpu) as ex:
+        list(ex.map(lambda k: orc.realteil(sysd, w["rcut"], alpha, cells=True,
+                                           i_range=(bounds[k], bounds[k + 1])), range(ncpu)))
+    tr_all = (time.perf_counter() - t0) * N / big
     return {"value": N / t_eval, "unit": "particle-force evals/s", "cores": 1, "kind": "port",
+            "host_cpus": ncpu, "cpu_model": model,
             "sample": "oracle (FP64 C, 1 thread): full k-space evaluation (%.1f s) + real space of "
-                      "%d of %d particles on a cell list, scaled (%.1f s)" % (tk, sample, N, tr)}
+                      "%d of %d particles on a cell list, scaled (%.1f s)" % (tk, sample, N, tr),
+            "all_cores": {"value": N / (tk + tr_all), "cores": ncpu,
+                          "sample": "k space as above (1 thread: no OpenMP in the reference's "
+                                    "assignment / FFT / interpolation) + real space of %d particles "
+                                    "over %d threads, scaled (%.1f s)" % (big, ncpu, tr_all)}}
 
 
 def batch_systems(nsys=256, seed=4, rcut=3.0, eps=1e-4, density=0.5):
@@ -483,6 +527,9 @@ def main():
         "fp64": {"value": round(value64, 1), "ms_per_step": round(r64["ms_per_step"], 4),
                  "stages": per64},
         "speedup_mixed_vs_fp64": round(r64["ms_per_step"] / rmx["ms_per_step"], 3),
+        "speedup_mixed_vs_fp64_kspace": round(r64["stage_ms"]["kspace"] / rmx["stage_ms"]["kspace"], 3),
+        "speedup_mixed_vs_fp64_realspace": round(r64["stage_ms"]["realspace"] /
+                                                 rmx["stage_ms"]["realspace"], 3),
         "accuracy": {"rms_mixed_vs_fp64": rms_mixed_vs_fp64, "rms_force": rms_force,
                      "estimate_total": est_total, "estimate_kspace": est_k, "ewald": gates},
     }

@@ -207,13 +207,49 @@ def test_error_estimates(pkg, orc, method):
     assert abs(total - np.hypot(kref, rref)) <= 1e-8 * np.hypot(kref, rref)
 
 
+@pytest.mark.parametrize("method", [0, 1, 6])
+def test_error_table_negative_lookup(pkg, orc, method):
+    """A negative table value means "not tabulated" to the reference, which
+    then takes the direct sum: ik starting from the negative lookup, ik-i from
+    0, then fabs (p3m-ik.c:267-297, p3m-ik-i.c:301-318).  Checked with a
+    registered row of -0.5 at alpha L = 10, where the direct sum is well
+    conditioned; the built-in row is restored afterwards."""
+    import math
+
+    s = random_system(1000, 10.0, seed=5)
+    M, cao, alpha = 16, 5, 1.0
+    kind = 1 if method == 1 else 0
+    row = np.full(101, -0.5)
+    system = pkg.HostSystem.from_dict(s)
+    q2 = float(np.sum(s["q"] ** 2))
+    try:
+        pkg.set_error_table(kind, M, cao, row)
+        _, k = pkg.error_estimate(method, system, pkg.parameters(M, cao, alpha, 3.0))
+    finally:
+        pkg.set_error_table(kind, M, cao, error_table_row(method, M, cao))
+    kref = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha, table=row)
+    direct = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha)
+    print("method %d: %.9e (oracle %.9e, plain direct sum %.9e)" % (method, k, kref, direct))
+    assert abs(k - kref) <= 1e-8 * abs(kref)
+    if method == 1:
+        assert abs(k - direct) <= 1e-8 * direct
+    else:
+        # 2 q2 sqrt(|-0.5 + S| / N) / L^2 against 2 q2 sqrt(|-1 + S| / N) / L^2
+        assert abs(k - direct) > 0.1 * direct
+    # the built-in row is back: the tabulated value again
+    _, k2 = pkg.error_estimate(method, system, pkg.parameters(M, cao, alpha, 3.0))
+    he = error_table_row(method, M, cao)[10]
+    assert abs(k2 - 2.0 * q2 * math.sqrt(he / 1000) / 100.0) <= 1e-14 * k2
+
+
 @pytest.mark.parametrize("method,M,cao,alpha", [(0, 16, 6, 0.15), (0, 32, 7, 0.6), (1, 16, 4, 0.25),
                                                 (1, 16, 6, 0.2), (6, 16, 7, 0.2)])
-def test_error_table_negative_lookup(pkg, orc, method, M, cao, alpha):
-    """Where the interpolated table value is negative (rounding noise of the
-    reference's tables at small alpha L) the reference takes the direct sum:
-    ik starting from the negative lookup, ik-i from 0, then fabs
-    (p3m-ik.c:267-297, p3m-ik-i.c:301-318)."""
+def test_error_builtin_negative_entries(pkg, orc, method, M, cao, alpha):
+    """The reference's tables hold rounding-noise negatives at small alpha L;
+    there the estimate is the direct sum (from the lookup for ik), itself a
+    sum of cancelling terms at the 1e-17 level, so GPU and oracle agree only
+    to a few percent -- and for ik both are far below the estimate the
+    untabulated path's -1 start would give."""
     s = random_system(1000, 10.0, seed=5)
     row = error_table_row(method, M, cao)
     aL = alpha * 10.0
@@ -226,7 +262,9 @@ def test_error_table_negative_lookup(pkg, orc, method, M, cao, alpha):
     kdirect = orc.kspace_error(method, 1000, 10.0, q2, M, cao, alpha)
     print("method %d M %d cao %d alpha L %.2f: %.6e (oracle %.6e, without the lookup %.6e)" %
           (method, M, cao, aL, k, kref, kdirect))
-    assert abs(k - kref) <= 1e-8 * abs(kref)
+    assert abs(k - kref) <= 0.05 * abs(kref)
+    if method != 1:
+        assert k < 1e-3 * kdirect
 
 
 @pytest.mark.parametrize("name,method_id", [("method_p3m_ik", 0), ("method_p3m_ik_i", 1),
