@@ -338,6 +338,22 @@ int p3m_hip_ewald_sample_device(int n, double length, const double *x, const dou
                                 const double *z, const double *q, int nsample, const int *idx,
                                 double alpha, double rcut, int kmax, double alpha_split,
                                 double *f_total, double *f_kspace);
+/* Allocation-free form of p3m_hip_ewald_sample_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  Same arguments and
+ * result, enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * without allocating, synchronising or creating streams; the caller waits
+ * on the stream.  Its working memory is one device array of the library,
+ * ~12 MiB per GPU, held until the process ends; limits: kmax <= 63
+ * (<= 2^18 lattice vectors of the half sphere) and nsample <= 32768,
+ * beyond which it returns P3M_HIP_EINVAL.  The first call on a device
+ * resolves the array's address there (make it outside graph capture).
+ * Calls that overlap on one GPU share the array and race. */
+int p3m_hip_ewald_sample_device_noalloc(int n, double length, const double *x, const double *y,
+                                        const double *z, const double *q, int nsample,
+                                        const int *idx, double alpha, double rcut, int kmax,
+                                        double alpha_split, double *f_total, double *f_kspace,
+                                        void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batches of independent systems                                          */

@@ -151,6 +151,117 @@ __global__ void k_ewald_combine(long m, const double *__restrict__ k, const doub
         f_split[t] += tot - r2[t];
 }
 
+// ---- allocation-free form ---------------------------------------------------
+//
+// The sampler's working memory -- the lattice vectors of the half sphere,
+// their influence function, the structure factors and the per-sample partial
+// forces -- in one file-scope device array instead of per-call allocations,
+// for callers that cannot allocate (graph capture, tight serving loops).
+// Limits: kmax <= 63, at most kNoAllocMaxK lattice vectors and
+// kNoAllocMaxSample sampled particles.
+
+constexpr int kNoAllocMaxKmax = 63;
+constexpr int kNoAllocMaxLines = (kNoAllocMaxKmax + 1) * (2 * kNoAllocMaxKmax + 1);
+constexpr long kNoAllocMaxK = 1L << 18;
+constexpr long kNoAllocMaxSample = 1L << 15;
+constexpr size_t kOffKv = 0;
+constexpr size_t kOffG = kOffKv + sizeof(int4) * kNoAllocMaxK;
+constexpr size_t kOffRho = kOffG + sizeof(double) * kNoAllocMaxK;
+constexpr size_t kOffTmp = kOffRho + sizeof(double2) * kNoAllocMaxK;
+constexpr size_t kScratchBytes = kOffTmp + sizeof(double) * 9 * kNoAllocMaxSample;
+
+// Lattice vectors n of the half sphere 0 < n^2 <= kmax^2 (n_x > 0, or n_x = 0
+// and (n_y, n_z) lexicographically positive) and G(n), in the order nx, ny,
+// nz ascending -- the order of the allocating form's host enumeration.  One
+// workgroup: each (nx, ny) line counts its vectors, an LDS scan gives the
+// line offsets, then each line writes its run.
+__device__ __forceinline__ bool lattice_keep(int nx, int ny, int nz, long kmax2) {
+    const long n2 = (long)nx * nx + (long)ny * ny + (long)nz * nz;
+    if (n2 == 0 || n2 > kmax2)
+        return false;
+    return !(nx == 0 && (ny < 0 || (ny == 0 && nz < 0)));
+}
+
+__global__ void __launch_bounds__(1024)
+k_ewald_lattice(int kmax, double fak1, double fak2, int4 *__restrict__ kv, double *__restrict__ G) {
+    __shared__ int off[kNoAllocMaxLines];
+    __shared__ int part[1024];
+    const int t = threadIdx.x, nyn = 2 * kmax + 1, nlines = (kmax + 1) * nyn;
+    const long kmax2 = (long)kmax * kmax;
+    const int chunk = (nlines + 1023) / 1024, b = min(nlines, t * chunk), e = min(nlines, b + chunk);
+    int sum = 0;
+    for (int l = b; l < e; l++) {
+        const int nx = l / nyn, ny = l % nyn - kmax;
+        int c = 0;
+        for (int nz = -kmax; nz <= kmax; nz++)
+            c += lattice_keep(nx, ny, nz, kmax2);
+        off[l] = c;
+        sum += c;
+    }
+    part[t] = sum;
+    __syncthreads();
+    // inclusive scan of the 1024 chunk sums (Hillis-Steele)
+    for (int w = 1; w < 1024; w <<= 1) {
+        const int v = t >= w ? part[t - w] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int l = b; l < e; l++) {
+        const int c = off[l];
+        off[l] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int l = t; l < nlines; l += 1024) {
+        const int nx = l / nyn, ny = l % nyn - kmax;
+        int k = off[l];
+        for (int nz = -kmax; nz <= kmax; nz++) {
+            if (!lattice_keep(nx, ny, nz, kmax2))
+                continue;
+            const double n2 = (double)((long)nx * nx + (long)ny * ny + (long)nz * nz);
+            kv[k] = make_int4(nx, ny, nz, 0);
+            G[k] = fak1 / n2 * exp(-fak2 * n2);
+            k++;
+        }
+    }
+}
+
+__global__ void k_zero_doubles(long m, double *__restrict__ p) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m)
+        p[t] = 0.0;
+}
+
+// The workspace.  In host code this name is only a stand-in for the symbol:
+// kernels receive the device address that hipGetSymbolAddress returns for
+// the current device (noalloc_scratch_address).
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device, resolved once per
+// device on its first use and kept.
+int noalloc_scratch_address(unsigned char **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return p3mk_fail(P3MK_EDEVICE, "hipGetDevice", (int)e);
+    if (dev < 0 || dev >= kMaxDevices)
+        return p3mk_fail(P3MK_EDEVICE, "device index beyond the workspace table", dev);
+    void *p = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
+    if (p == nullptr) {
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return p3mk_fail(P3MK_EDEVICE, "hipGetSymbolAddress(scratch)", (int)e);
+        __atomic_store_n(&g_scratch_ptr[dev], p, __ATOMIC_RELEASE);
+    }
+    *out = (unsigned char *)p;
+    return P3MK_OK;
+}
+
 }  // namespace
 
 extern "C" int p3mk_ewald_sample(int n, double L, const double *x, const double *y, const double *z,
@@ -221,4 +332,60 @@ extern "C" int p3mk_ewald_sample(int n, double L, const double *x, const double 
     p3mk_release(drho);
     p3mk_release(tmp);
     return rc;
+}
+
+extern "C" int p3mk_ewald_sample_noalloc(int n, double L, const double *x, const double *y,
+                                         const double *z, const double *q, int ns, const int *idx,
+                                         double alpha, double rcut, int kmax, double alpha2,
+                                         double *f_total, double *f_split, void *stream) {
+    if (n <= 0 || ns <= 0)
+        return P3MK_OK;
+    if (!(L > 0.0) || !(alpha > 0.0) || rcut < 0.0 || rcut > 0.5 * L || kmax < 1)
+        return p3mk_fail(P3MK_EINVAL, "Ewald sample needs alpha > 0, 0 <= rcut <= L/2, kmax >= 1", 0);
+    if (kmax > kNoAllocMaxKmax)
+        return p3mk_fail(P3MK_EINVAL, "allocation-free Ewald sample: kmax beyond the maximum 63",
+                         kmax);
+    if ((long)ns > kNoAllocMaxSample)
+        return p3mk_fail(P3MK_EINVAL,
+                         "allocation-free Ewald sample: more sampled particles than the maximum 32768",
+                         ns);
+    // lattice vectors of the half sphere: the same count the kernel finds
+    const long kmax2 = (long)kmax * kmax;
+    long nk = 0;
+    for (int nx = 0; nx <= kmax; nx++)
+        for (int ny = -kmax; ny <= kmax; ny++)
+            for (int nz = -kmax; nz <= kmax; nz++) {
+                const long n2 = (long)nx * nx + (long)ny * ny + (long)nz * nz;
+                if (n2 == 0 || n2 > kmax2 || (nx == 0 && (ny < 0 || (ny == 0 && nz < 0))))
+                    continue;
+                nk++;
+            }
+    if (nk > kNoAllocMaxK)
+        return p3mk_fail(P3MK_EINVAL, "allocation-free Ewald sample: too many lattice vectors", (int)nk);
+    unsigned char *base = nullptr;
+    int rc = noalloc_scratch_address(&base);
+    if (rc != P3MK_OK)
+        return rc;
+    int4 *dkv = (int4 *)(base + kOffKv);
+    double *dG = (double *)(base + kOffG);
+    double2 *drho = (double2 *)(base + kOffRho);
+    double *tmp = (double *)(base + kOffTmp);
+    const long m3 = 3L * ns;
+    hipStream_t st = (hipStream_t)stream;
+    const double Li = 1.0 / L;
+    const double fak1 = 2.0 / (L * L), fak2 = (kPi / (alpha * L)) * (kPi / (alpha * L));
+    double *tk = tmp, *tr1 = tmp + m3, *tr2 = tmp + 2 * m3;
+    hipLaunchKernelGGL(k_ewald_lattice, dim3(1), dim3(1024), 0, st, kmax, fak1, fak2, dkv, dG);
+    hipLaunchKernelGGL(k_zero_doubles, dim3((unsigned)((3 * m3 + 255) / 256)), dim3(256), 0, st,
+                       3 * m3, tmp);
+    hipLaunchKernelGGL(k_ewald_rho, dim3((unsigned)((nk + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, st, n, x, y, z, q, (int)nk, dkv, Li, drho);
+    hipLaunchKernelGGL(k_ewald_sample_k, dim3((ns + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
+                       ns, idx, x, y, z, q, (int)nk, dkv, dG, drho, Li, tk);
+    if (rcut > 0.0)
+        hipLaunchKernelGGL(k_ewald_sample_r, dim3(ns), dim3(kThreads), 0, st, n, ns, idx, x, y, z, q,
+                           L, rcut, alpha, alpha2, tr1, tr2);
+    hipLaunchKernelGGL(k_ewald_combine, dim3((unsigned)((m3 + 255) / 256)), dim3(256), 0, st, m3, tk,
+                       tr1, tr2, f_total, f_split);
+    return p3mk_launch_check("allocation-free Ewald sample");
 }

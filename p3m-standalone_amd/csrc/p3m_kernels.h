@@ -224,6 +224,15 @@ int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double *y, const 
 int p3mk_ewald_sample(int n, double L, const double *x, const double *y, const double *z,
                       const double *q, int ns, const int *idx, double alpha, double rcut, int kmax,
                       double alpha2, double *f_total, double *f_split, p3mk_queue *q_);
+/* The same on `stream` (a hipStream_t, NULL = the null stream) without any
+ * allocation or synchronisation: the working memory is a file-scope device
+ * array of k_ewald.hip (kmax <= 63, <= 2^18 lattice vectors, ns <= 32768;
+ * larger requests return P3MK_EINVAL).  Concurrent calls on one device share
+ * that array and race. */
+int p3mk_ewald_sample_noalloc(int n, double L, const double *x, const double *y, const double *z,
+                              const double *q, int ns, const int *idx, double alpha, double rcut,
+                              int kmax, double alpha2, double *f_total, double *f_split,
+                              void *stream);
 
 /* ---- graphs ------------------------------------------------------------- */
 typedef struct p3mk_graph p3mk_graph;
