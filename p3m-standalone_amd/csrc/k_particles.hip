@@ -158,6 +158,28 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
                                      reinterpret_cast<double *>(smem_raw));
 }
 
+// Row-parallel form of k_assign_halo (charge and charge-times-factor
+// deposits): FP64 LDS accumulation, one mesh of an interlaced pair at a time.
+template <int P, typename T, bool R2C, bool IL, int MODE>
+__global__ void __launch_bounds__(kThreads)
+k_assign_rows(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
+              const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
+              T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
+              int region_doubles) {
+    extern __shared__ unsigned char smem_raw[];
+    const int item = blockIdx.x;
+    if (item >= cstart[ntiles])
+        return;
+    const long tile = item_tile(cstart, ntiles, item);
+    const int chunk = item - cstart[tile];
+    const int pb = tstart[tile] + chunk * kChunk;
+    const int pe = min(tstart[tile + 1], pb + kChunk);
+    double *acc = reinterpret_cast<double *>(smem_raw);
+    T *stage = reinterpret_cast<T *>(smem_raw + sizeof(double) * (size_t)region_doubles);
+    assign_rows_item<P, T, R2C, IL, MODE, double>(sorted, pb, pe, tile, nt, Tt, M, Hi, mesh, perm,
+                                                  gv, acc, stage);
+}
+
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
 k_interp_ik_tile(const double4 *__restrict__ sorted, const int *__restrict__ perm,
@@ -427,15 +449,48 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
     const double Hi = (double)g->M / g->L;
     const int NM = g->interlaced ? 2 : 1;
     const int R = s->T + g->cao - 1 + (g->interlaced ? 1 : 0);
-    const size_t lds = sizeof(double) * NM * (size_t)R * R * R;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
     // upper bound of the number of work items; surplus workgroups exit
     const long items = (long)(n + kChunk - 1) / kChunk + s->ntiles;
     const dim3 grid((unsigned)items);
     T *m = (T *)mesh;
     hipStream_t st = p3mk_stream(qu);
     const bool il = g->interlaced != 0;
+    if constexpr (MODE != kDipole) {
+        const RowRegion rg = row_region(s->T, g->cao, il);
+        size_t stage = 0;
+        switch (g->cao) {
+#define SB(P)                                                                                      \
+    case P:                                                                                        \
+        stage = (size_t)(kThreads / 64) * row_stage_bytes<P, T>();                                \
+        break;
+            SB(1) SB(2) SB(3) SB(4) SB(5) SB(6) SB(7)
+#undef SB
+        default:
+            return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
+        }
+        const size_t lds = sizeof(double) * (size_t)rg.size + stage;
+        if (lds > 64 * 1024)
+            return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+        switch (g->cao) {
+#define CASE(P)                                                                                    \
+    case P:                                                                                        \
+        if (il)                                                                                    \
+            hipLaunchKernelGGL((k_assign_rows<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
+                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               Hi, m, s->perm, gv, rg.size);                                       \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_assign_rows<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
+                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               Hi, m, s->perm, gv, rg.size);                                       \
+        break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+        }
+        return p3mk_launch_check("assign");
+    }
+    const size_t lds = sizeof(double) * NM * (size_t)R * R * R;
+    if (lds > 64 * 1024)
+        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \

@@ -27,17 +27,146 @@ __device__ __forceinline__ long item_tile(const int *__restrict__ cstart, long n
     return lo;
 }
 
+// What a particle deposits (MODE): kCharge q W; kWeighted q g_i W with a
+// per-particle factor g (the adjoint's q g_d); kDipole
+// q sum_d g_{i,d} W^(d), W^(d) = W with the d-axis weight replaced by its
+// derivative (the adjoint of ad back-interpolation).
+enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
+
+// ---- row-parallel halo-tile assignment ----------------------------------------
+//
+// One mesh at a time, a work item (<= kChunk particles of one tile) deposits
+// into an LDS copy of the tile's halo region.  Per round each wavefront
+// stages kRowBatch particles (a lane each: quantised weights of the three
+// axes, q folded into the x weights, local base offsets) in LDS; then the
+// wavefront deposits them one after another with its lanes spread over the
+// stencil: lane l takes stencil row (a0, a1) = ((l mod P^2) / P, l mod P) of
+// particle l / P^2 of the current group (64 / P^2 particles per step) and
+// adds its P points along z.  The lanes of one particle hit distinct
+// addresses, and the region's strides (z 1, y Sy odd, x Sx = P Sy mod 64)
+// spread the P^2 rows of a step over the LDS banks.  The products are the
+// per-particle form's ((q w0) w1) w2.  Charge (MODE kCharge) and
+// charge-times-factor (kWeighted) deposits; the dipole deposit of the ad
+// adjoint keeps assign_item.
+
+constexpr int kRowBatch = 32; // particles staged per wavefront and round
+
+struct RowRegion {
+    int R, Sy, Sx, size; // side, y and x strides (z stride 1), points incl. padding
+};
+
+__host__ __device__ inline RowRegion row_region(int Tt, int P, bool il) {
+    RowRegion g;
+    g.R = Tt + P - 1 + (il ? 1 : 0);
+    g.Sy = g.R + ((g.R & 1) ? 0 : 1);
+    const int want = (P * g.Sy) & 63, base = g.R * g.Sy;
+    g.Sx = base + ((want - (base & 63)) & 63);
+    g.size = g.R * g.Sx;
+    return g;
+}
+
+template <int P> __host__ __device__ constexpr int row_record() {
+    return (3 * P + 1) & ~1; // w2[P], q w0[P], w1[P], padded to an even count
+}
+
+// LDS bytes of one wavefront's staging area: kRowBatch records of T and the
+// particles' region offsets
+template <int P, typename T> __host__ __device__ constexpr int row_stage_bytes() {
+    return kRowBatch * (row_record<P>() * (int)sizeof(T) + (int)sizeof(int));
+}
+
+template <int P, typename T, bool R2C, bool IL, int MODE, typename A>
+__device__ __forceinline__ void
+assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, int nt, int Tt, int M,
+                 double Hi, T *__restrict__ mesh, const int *__restrict__ perm,
+                 const double *__restrict__ gv, A *acc, T *stage) {
+    static_assert(MODE == kCharge || MODE == kWeighted, "row assignment deposits charges");
+    constexpr int NM = IL ? 2 : 1;
+    constexpr int REC = row_record<P>();
+    constexpr int P2 = P * P;
+    constexpr int G = 64 / P2; // particles per deposit step
+    const RowRegion rg = row_region(Tt, P, IL);
+    const int tz = (int)(tl % nt), ty = (int)((tl / nt) % nt), tx = (int)(tl / ((long)nt * nt));
+    const int t0x = tx * Tt, t0y = ty * Tt, t0z = tz * Tt;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    T *wst = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(stage) +
+                                   wave * row_stage_bytes<P, T>());
+    int *woff = reinterpret_cast<int *>(wst + kRowBatch * REC);
+    const int grp = lane / P2, row = lane - grp * P2, ra0 = row / P, ra1 = row - ra0 * P;
+    const bool row_lane = grp < G;
+    const int cnt = pe - pb, per_round = nwaves * kRowBatch;
+    const int rounds = (cnt + per_round - 1) / per_round;
+
+    for (int ii = 0; ii < NM; ii++) {
+        const double half = 0.5 * ii;
+        for (int k = threadIdx.x; k < rg.size; k += blockDim.x)
+            acc[k] = A(0);
+        __syncthreads();
+        for (int rd = 0; rd < rounds; rd++) {
+            const int b0 = pb + rd * per_round + wave * kRowBatch;
+            const int nb = max(0, min(kRowBatch, pe - b0));
+            if (lane < nb) {
+                const int p = b0 + lane;
+                const double4 r = sorted[p];
+                T qv = (T)r.w;
+                if (MODE == kWeighted)
+                    qv = (T)(r.w * gv[perm[p]]);
+                int c0, c1, c2;
+                T w0[P], w1[P], w2[P];
+                axis_weights<P, T>(r.x, Hi, half, M, c0, w0);
+                axis_weights<P, T>(r.y, Hi, half, M, c1, w1);
+                axis_weights<P, T>(r.z, Hi, half, M, c2, w2);
+                int o0 = c0 - t0x, o1 = c1 - t0y, o2 = c2 - t0z;
+                o0 += (o0 < 0) ? M : 0;
+                o1 += (o1 < 0) ? M : 0;
+                o2 += (o2 < 0) ? M : 0;
+                T *rec = wst + lane * REC;
+#pragma unroll
+                for (int a = 0; a < P; a++) {
+                    rec[a] = w2[a];
+                    rec[P + a] = qv * w0[a];
+                    rec[2 * P + a] = w1[a];
+                }
+                woff[lane] = o0 * rg.Sx + o1 * rg.Sy + o2;
+            }
+            __syncthreads();
+            for (int j = 0; j < nb; j += G) {
+                const int jj = j + grp;
+                if (row_lane && jj < nb) {
+                    const T *rec = wst + jj * REC;
+                    const T tr = rec[P + ra0] * rec[2 * P + ra1];
+                    A *a = acc + woff[jj] + ra0 * rg.Sx + ra1 * rg.Sy;
+#pragma unroll
+                    for (int a2 = 0; a2 < P; a2++)
+                        atomicAdd(&a[a2], (A)(tr * rec[a2]));
+                }
+            }
+            __syncthreads();
+        }
+        // add the region to mesh ii (periodic wrap of the halo)
+        const int R = rg.R;
+        for (int k = threadIdx.x; k < R * R * R; k += blockDim.x) {
+            const int lz = k % R, ly = (k / R) % R, lx = k / (R * R);
+            const T v = (T)acc[lx * rg.Sx + ly * rg.Sy + lz];
+            if (v != T(0)) {
+                const int gx = (t0x + lx) % M, gy = (t0y + ly) % M, gz = (t0z + lz) % M;
+                const long g = ((long)gx * M + gy) * M + gz;
+                if (R2C)
+                    atomicAdd(&mesh[g], v);
+                else
+                    atomicAdd(&mesh[2 * g + ii], v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- halo-tile assignment ------------------------------------------------------
 
 // LDS accumulation is FP64 in both forms: fire-and-forget ds_add_f64 (no
 // returned value to wait for), and the FP32 form's products are summed
 // without rounding error growth; the mesh itself is stored in the form's
 // precision.
-// What a particle deposits (MODE): kCharge q W; kWeighted q g_i W with a
-// per-particle factor g (the adjoint's q g_d); kDipole
-// q sum_d g_{i,d} W^(d), W^(d) = W with the d-axis weight replaced by its
-// derivative (the adjoint of ad back-interpolation).
-enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
 
 template <int P, typename T, bool R2C, bool IL, int MODE>
 __device__ __forceinline__ void
