@@ -500,6 +500,15 @@ def main():
     params = pkg.parameters(w["M"], w["cao"], w["alpha"], w["rcut"])
     est_total, est_k = pkg.error_estimate(w["method"], system, params)
     gates = ewald_gates(pkg, torch, w, dev, {"fp64": r64, "mixed": rmx})
+    # the inhomogeneous k-space estimate (p3m-common.c:437-719) at the
+    # method's mesh and order with one shell of aliasing copies, beside the
+    # same estimate for a uniform density: their ratio is what clustering
+    # does to the k-space error, to compare with the measured ratio of the
+    # k-space rms against Ewald to the homogeneous a-priori estimate
+    t0 = time.perf_counter()
+    est_inhomo = pkg.error_inhomo(system, w["alpha"], mesh=w["M"], cao=w["cao"], mc=1)
+    est_uniform = pkg.error_inhomo(system, w["alpha"], mesh=w["M"], cao=w["cao"], mc=1, uniform=True)
+    t_inhomo = time.perf_counter() - t0
 
     pairs = pair_count(w, sysd) if w["rcut"] > 0 else 0.0
     dom, per = roofline(w, rmx, 4, pairs)
@@ -531,7 +540,13 @@ def main():
         "speedup_mixed_vs_fp64_realspace": round(r64["stage_ms"]["realspace"] /
                                                  rmx["stage_ms"]["realspace"], 3),
         "accuracy": {"rms_mixed_vs_fp64": rms_mixed_vs_fp64, "rms_force": rms_force,
-                     "estimate_total": est_total, "estimate_kspace": est_k, "ewald": gates},
+                     "estimate_total": est_total, "estimate_kspace": est_k, "ewald": gates,
+                     "estimate_kspace_inhomo": {
+                         "mesh": w["M"], "cao": w["cao"], "mc": 1, "inhomogeneous": est_inhomo,
+                         "uniform_density": est_uniform,
+                         "ratio": (est_inhomo / est_uniform) if est_uniform > 0 else None,
+                         "measured_ratio": gates["fp64"]["rms_kspace_vs_ewald"] / est_k if est_k > 0 else None,
+                         "seconds": round(t_inhomo, 2)}},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, sysd)

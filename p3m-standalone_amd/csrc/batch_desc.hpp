@@ -60,17 +60,18 @@ size_t bfft_lds_bytes(const BForm &f, int M, int kind);
 enum { kPassZ = 0, kPassY = 1, kPassX = 2 };
 
 // particles: sort by global tile, assign, back-interpolate
-int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, int ntot,
-               const double *x, const double *y, const double *z, const double *q, int *key,
-               int *count, int *start, int *cursor, int *perm, int *chunks, int *cstart,
-               int *scan_scratch, double4 *sorted, p3mk_queue *qu);
-int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
-                 long max_items, size_t lds, const int *start, const int *cstart,
-                 const double4 *sorted, p3mk_queue *qu);
-int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
-                 long max_items, size_t lds, const int *start, const int *cstart,
-                 const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
+// imap: work item -> (global tile, chunk, system); max_items entries
+int bpart_sort(const BForm &f, const BSys *sys, const int *psys, const long *tile0s, int nsys,
+               long ntiles, int ntot, const double *x, const double *y, const double *z,
+               const double *q, int *key, int *count, int *start, int *cursor, int *perm,
+               int *chunks, int *cstart, int4 *imap, int *scan_scratch, double4 *sorted,
+               p3mk_queue *qu);
+int bpart_assign(const BForm &f, const BSys *sys, long ntiles, long max_items, size_t lds,
+                 const int *start, const int *cstart, const int4 *imap, const double4 *sorted,
                  p3mk_queue *qu);
+int bpart_interp(const BForm &f, const BSys *sys, long ntiles, long max_items, size_t lds,
+                 const int *start, const int *cstart, const int4 *imap, const double4 *sorted,
+                 const int *perm, int ntot, double *fk, double *ftot, p3mk_queue *qu);
 
 // real space: cell-list arrays (octant-sorted cells, work items of 64
 // particles with bounding boxes) and one persistent launch over all items

@@ -143,6 +143,91 @@ __device__ __forceinline__ void butterfly(const V *__restrict__ src, V *__restri
         dst[base + r * Ns] = v[r];
 }
 
+// ---- register-resident passes for power-of-two lines --------------------
+//
+// A line of N points is held by N / 8 threads, eight values each: thread t
+// owns elements t + (N / 8) s, s = 0..7, which are exactly the inputs of its
+// butterflies in every Stockham pass (radix R: butterflies t + (N / 8) m,
+// m < 8 / R) and, after the last pass, its outputs in natural order.  A
+// pass computes in registers; between passes the outputs go through one LDS
+// line (positions of the autosort Stockham, as butterfly() writes them) and
+// back.  The radix sequence is make_radices', so the results equal the LDS
+// tile passes' bit for bit.
+template <int N> struct P2Plan;
+template <> struct P2Plan<16> { static constexpr int np = 2, r[3] = {8, 2, 1}; };
+template <> struct P2Plan<32> { static constexpr int np = 2, r[3] = {8, 4, 1}; };
+template <> struct P2Plan<64> { static constexpr int np = 2, r[3] = {8, 8, 1}; };
+template <> struct P2Plan<128> { static constexpr int np = 3, r[3] = {8, 8, 2}; };
+template <> struct P2Plan<256> { static constexpr int np = 3, r[3] = {8, 8, 4}; };
+
+// one pass: v holds the inputs t + (N / 8) s, returns the outputs in the
+// same slots; slot m + r (8 / R) is output r of butterfly m
+template <int N, int R, typename V>
+__device__ __forceinline__ void p2_pass(V v[8], int t, int Ns, const V *__restrict__ tw, int inv) {
+    constexpr int TPL = N / 8, B = 8 / R;
+#pragma unroll
+    for (int m = 0; m < B; m++) {
+        const int j = t + m * TPL;
+        const int k = j & (Ns - 1);
+        V x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            x[r] = v[m + r * B];
+        if (Ns > 1) {
+            const int step = k * (N / (Ns * R));
+#pragma unroll
+            for (int r = 1; r < R; r++)
+                x[r] = cmul(x[r], twid(tw, r * step, inv));
+        }
+        Dft<R, V>::run(x, tw, N, inv);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            v[m + r * B] = x[r];
+    }
+}
+
+// the outputs of a pass to their autosort positions in the LDS line
+template <int N, int R, typename V>
+__device__ __forceinline__ void p2_write(V *line, const V v[8], int t, int Ns) {
+    constexpr int TPL = N / 8, B = 8 / R;
+#pragma unroll
+    for (int m = 0; m < B; m++) {
+        const int j = t + m * TPL;
+        const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            line[base + r * Ns] = v[m + r * B];
+    }
+}
+
+template <int N, typename V>
+__device__ __forceinline__ void p2_read(const V *line, V v[8], int t) {
+    constexpr int TPL = N / 8;
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+        v[s] = line[t + TPL * s];
+}
+
+// all passes of the plan on the registers of thread t; `line` is its line's
+// LDS row (N + 1 values), shared by the N / 8 threads of the line
+template <int N, typename V>
+__device__ __forceinline__ void p2_fft(V v[8], V *line, int t, const V *__restrict__ tw, int inv) {
+    using Pl = P2Plan<N>;
+    p2_pass<N, Pl::r[0]>(v, t, 1, tw, inv);
+    p2_write<N, Pl::r[0]>(line, v, t, 1);
+    __syncthreads();
+    p2_read<N>(line, v, t);
+    __syncthreads();
+    p2_pass<N, Pl::r[1]>(v, t, Pl::r[0], tw, inv);
+    if constexpr (Pl::np == 3) {
+        p2_write<N, Pl::r[1]>(line, v, t, Pl::r[0]);
+        __syncthreads();
+        p2_read<N>(line, v, t);
+        __syncthreads();
+        p2_pass<N, Pl::r[2]>(v, t, Pl::r[0] * Pl::r[1], tw, inv);
+    }
+}
+
 // All radix passes over `nl` lines held in LDS (line l at src + l*pitch).
 // Returns the buffer that holds the result.
 template <typename V>
@@ -426,6 +511,19 @@ template <typename T> __host__ __device__ inline int tile_lines(int n, int nbuf)
         lt = cap;
     if (lt < 1)
         lt = 1;
+    return lt;
+}
+
+// lines per tile of the launches: the largest power of two up to 16 that
+// tile_lines allows.  Sixteen lines keep a strided row of the tile aligned
+// and let several workgroups share a CU (at M = 128 the FP32 FFT + multiply
+// stage takes 0.14 ms with 16 lines against 0.28 ms with the 31 that fill
+// 64 KiB; FP64: 8 lines, 0.19 against 0.27 ms with 15).
+template <typename T> __host__ __device__ inline int pow2_tile_lines(int n, int nbuf) {
+    const int fit = tile_lines<T>(n, nbuf);
+    int lt = 1;
+    while (2 * lt <= fit && 2 * lt <= 16)
+        lt *= 2;
     return lt;
 }
 

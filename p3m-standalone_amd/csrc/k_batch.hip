@@ -58,6 +58,7 @@ struct p3mk_batch {
     // particle sort
     int *key = nullptr, *count = nullptr, *start = nullptr, *cursor = nullptr, *perm = nullptr,
         *chunks = nullptr, *cstart = nullptr, *scan = nullptr;
+    int4 *imap = nullptr; // work item -> (tile, chunk, system)
     double4 *sorted = nullptr;
     // cells
     CellArrays cells;
@@ -71,7 +72,8 @@ void release_all(p3mk_batch *b) {
     // cursor lives in the count allocation
     void *ptrs[] = {b->dsys,  b->psys,   b->p0s,        b->cell0s,    b->tile0s, b->blk_z,
                     b->blk_y, b->blk_x,  b->mesh_arena, b->aux_arena, b->key,    b->count,
-                    b->start, b->perm,   b->chunks,     b->cstart,    b->scan,   b->sorted};
+                    b->start, b->perm,   b->chunks,     b->cstart,    b->scan,   b->sorted,
+                    b->imap};
     for (void *p : ptrs)
         p3mk_release(p);
     cells_release(b->cells);
@@ -138,7 +140,7 @@ int build_descriptors(p3mk_batch *b) {
     P3MK_TRY(upload(&b->blk_x, blk_x));
     // particle sort over all tiles
     void **sort_ptrs[] = {(void **)&b->count, (void **)&b->start, (void **)&b->chunks,
-                          (void **)&b->cstart, (void **)&b->scan};
+                          (void **)&b->cstart, (void **)&b->scan, (void **)&b->imap};
     for (void **p : sort_ptrs) {
         p3mk_release(*p);
         *p = nullptr;
@@ -149,6 +151,7 @@ int build_descriptors(p3mk_batch *b) {
     P3MK_TRY(p3mk_alloc((void **)&b->start, sizeof(int) * nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->chunks, sizeof(int) * nt1));
     P3MK_TRY(p3mk_alloc((void **)&b->cstart, sizeof(int) * nt1));
+    P3MK_TRY(p3mk_alloc((void **)&b->imap, sizeof(int4) * (size_t)(b->max_items > 0 ? b->max_items : 1)));
     P3MK_TRY(p3mk_alloc((void **)&b->scan, sizeof(int) * p3mk_scan_scratch_ints(tiles)));
     return P3MK_OK;
 }
@@ -386,10 +389,11 @@ extern "C" int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y
     const BForm &f = b->form;
     const int ns = b->nsys;
     P3MK_TRY(p3mk_fill_zero(b->mesh_arena, b->mesh_region, qu));
-    P3MK_TRY(bpart_sort(f, b->dsys, b->psys, b->ntiles, b->ntot, x, y, z, q, b->key, b->count,
-                        b->start, b->cursor, b->perm, b->chunks, b->cstart, b->scan, b->sorted, qu));
-    P3MK_TRY(bpart_assign(f, b->dsys, b->tile0s, ns, b->ntiles, b->max_items, b->lds_assign,
-                          b->start, b->cstart, b->sorted, qu));
+    P3MK_TRY(bpart_sort(f, b->dsys, b->psys, b->tile0s, ns, b->ntiles, b->ntot, x, y, z, q, b->key,
+                        b->count, b->start, b->cursor, b->perm, b->chunks, b->cstart, b->imap,
+                        b->scan, b->sorted, qu));
+    P3MK_TRY(bpart_assign(f, b->dsys, b->ntiles, b->max_items, b->lds_assign, b->start, b->cstart,
+                          b->imap, b->sorted, qu));
     // forward transform along z and y
     int spec = kBufMesh;
     if (f.r2c) {
@@ -419,8 +423,8 @@ extern "C" int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y
         else
             P3MK_TRY(bfft_pass(f, b->dsys, b->blk_z, ns, b->nblk_z, b->lds_z, kPassZ, kBufMesh, 1, qu));
     }
-    return bpart_interp(f, b->dsys, b->tile0s, ns, b->ntiles, b->max_items, b->lds_assign, b->start,
-                        b->cstart, b->sorted, b->perm, b->ntot, fk, ftot, qu);
+    return bpart_interp(f, b->dsys, b->ntiles, b->max_items, b->lds_assign, b->start, b->cstart,
+                        b->imap, b->sorted, b->perm, b->ntot, fk, ftot, qu);
 }
 
 // real-space force of every system with rcut > 0: fr += (3 blocks of ntot);

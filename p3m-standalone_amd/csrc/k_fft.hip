@@ -34,6 +34,41 @@ k_fft_pass(c2_t<T> *data, LineMap m, Radices rp, const c2_t<T> *__restrict__ tw,
     fft_pass_tile<T>(data, m, rp, tw, inv, lt, contig, (long)blockIdx.x * lt, smem);
 }
 
+// Power-of-two lines: register-resident passes (p2_fft), 256 / (N / 8)
+// lines per workgroup, one LDS row per line.  Contiguous lines (z axis):
+// a line's threads are consecutive, so each load of the eight covers N / 8
+// neighbouring elements; strided lines (y, x): consecutive threads take
+// neighbouring lines.
+template <int N, typename T, bool CONTIG>
+__global__ void __launch_bounds__(kThreads)
+k_fft_p2(c2_t<T> *data, LineMap m, const c2_t<T> *__restrict__ tw, int inv) {
+    using V = c2_t<T>;
+    constexpr int TPL = N / 8, LPW = kThreads / TPL, PITCH = N + 1;
+    __shared__ V buf[LPW * PITCH];
+    const int tid = threadIdx.x;
+    const int l = CONTIG ? tid / TPL : tid % LPW;
+    const int t = CONTIG ? tid % TPL : tid / LPW;
+    const long L = (long)blockIdx.x * LPW + l;
+    const bool ok = L < m.nlines;
+    const long base = ok ? line_start(m, L) : 0;
+    V v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        if (ok) {
+            v[s] = data[base + (long)(t + TPL * s) * m.es];
+        } else {
+            v[s].x = T(0);
+            v[s].y = T(0);
+        }
+    }
+    p2_fft<N>(v, buf + l * PITCH, t, tw, inv);
+    if (ok) {
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+            data[base + (long)(t + TPL * s) * m.es] = v[s];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_fft_r2c_z(const T *__restrict__ in, c2_t<T> *__restrict__ out, long nlines, int n, Radices rp,
@@ -151,19 +186,40 @@ extern "C" void p3mk_fft_destroy(p3mk_fft *f) {
 
 namespace {
 
-// Lines per workgroup of a launch over one mesh: tile_lines, halved (down to
-// 8) while the launch would have fewer workgroups than the GPU has CUs (256).
-// Small meshes need it: at M = 64 an FP32 z pass of 32 lines per workgroup
-// has 128 workgroups.
+// Lines per workgroup of a launch over one mesh: pow2_tile_lines, halved
+// (down to 8) while the launch would have fewer workgroups than the GPU has
+// CUs (256).
 template <typename T> int launch_lines(int n, int nbuf, long nlines) {
-    int lt = tile_lines<T>(n, nbuf);
+    int lt = pow2_tile_lines<T>(n, nbuf);
     while (lt > 8 && (nlines + lt - 1) / lt < 256)
         lt /= 2;
     return lt;
 }
 
+template <int N, typename T>
+int pass_p2(p3mk_fft *f, c2_t<T> *data, const LineMap &m, int inv, bool contig, p3mk_queue *q) {
+    constexpr int LPW = kThreads / (N / 8);
+    const long blocks = (m.nlines + LPW - 1) / LPW;
+    const c2_t<T> *tw = sizeof(T) == 8 ? (const c2_t<T> *)f->tw64 : (const c2_t<T> *)f->tw32;
+    if (contig)
+        hipLaunchKernelGGL((k_fft_p2<N, T, true>), dim3((unsigned)blocks), dim3(kThreads), 0,
+                           p3mk_stream(q), data, m, tw, inv);
+    else
+        hipLaunchKernelGGL((k_fft_p2<N, T, false>), dim3((unsigned)blocks), dim3(kThreads), 0,
+                           p3mk_stream(q), data, m, tw, inv);
+    return p3mk_launch_check("fft pass");
+}
+
 template <typename T>
 int pass(p3mk_fft *f, c2_t<T> *data, const LineMap &m, int inv, bool contig, p3mk_queue *q) {
+    switch (m.n) {
+    case 16: return pass_p2<16, T>(f, data, m, inv, contig, q);
+    case 32: return pass_p2<32, T>(f, data, m, inv, contig, q);
+    case 64: return pass_p2<64, T>(f, data, m, inv, contig, q);
+    case 128: return pass_p2<128, T>(f, data, m, inv, contig, q);
+    case 256: return pass_p2<256, T>(f, data, m, inv, contig, q);
+    default: break;
+    }
     const int lt = launch_lines<T>(m.n, 2, m.nlines);
     const size_t lds = 2 * (size_t)lt * (m.n + 1) * sizeof(c2_t<T>);
     const long blocks = (m.nlines + lt - 1) / lt;
@@ -278,7 +334,7 @@ k_bfft_pass(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys
         m = {M, 1, 1, M, (long)M * M};
     else
         m = {M, W, W, (long)M * W, (long)M * W};
-    const int lt = tile_lines<T>(M, 2);
+    const int lt = pow2_tile_lines<T>(M, 2);
     fft_pass_tile<T>((V *)S.buf[slot], m, S.rp, (const V *)S.tw, inv, lt, axis == kPassZ ? 1 : 0,
                      (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -291,7 +347,7 @@ k_bfft_r2c_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsy
     using V = c2_t<T>;
     const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
     const BSys &S = sys[s];
-    const int M = S.M, lt = tile_lines<T>(M, 2);
+    const int M = S.M, lt = pow2_tile_lines<T>(M, 2);
     fft_r2c_z_tile<T>((const T *)S.buf[in_slot], (V *)S.buf[out_slot], (long)M * M, M, S.rp,
                       (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -304,7 +360,7 @@ k_bfft_c2r_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsy
     using V = c2_t<T>;
     const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
     const BSys &S = sys[s];
-    const int M = S.M, lt = tile_lines<T>(M, 2);
+    const int M = S.M, lt = pow2_tile_lines<T>(M, 2);
     fft_c2r_z_tile<T>((const V *)S.buf[in_slot], (T *)S.buf[out_slot], (long)M * M, M, S.rp,
                       (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -320,7 +376,7 @@ k_bconvolve_x(const BSys *__restrict__ sys, const long *__restrict__ blk, int ns
     const int M = S.M, W = r2c ? M / 2 + 1 : M;
     const long nl = (long)M * W;
     const LineMap m = {M, nl, nl, 0, nl};
-    const int lt = tile_lines<T>(M, nout == 3 ? 3 : 2);
+    const int lt = pow2_tile_lines<T>(M, nout == 3 ? 3 : 2);
     V *o0 = (V *)S.buf[out_slot];
     V *o1 = nout == 3 ? (V *)S.buf[out_slot + 1] : nullptr;
     V *o2 = nout == 3 ? (V *)S.buf[out_slot + 2] : nullptr;
@@ -335,7 +391,7 @@ namespace p3m {
 
 int bfft_lines_per_tile(const BForm &f, int M, int kind) {
     const int nbuf = (kind == kPassX && !f.ad) ? 3 : 2;
-    return f.fp32 ? tile_lines<float>(M, nbuf) : tile_lines<double>(M, nbuf);
+    return f.fp32 ? pow2_tile_lines<float>(M, nbuf) : pow2_tile_lines<double>(M, nbuf);
 }
 
 size_t bfft_lds_bytes(const BForm &f, int M, int kind) {

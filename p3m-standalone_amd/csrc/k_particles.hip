@@ -139,10 +139,36 @@ __global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *_
         chunks[t] = (count[t] + kChunk - 1) / kChunk;
 }
 
+// work item -> (tile, chunk) for every item of the chunk prefix sums, so
+// that a workgroup finds its tile with one load (a binary search over the
+// tiles is a chain of a dozen dependent loads per workgroup)
+__global__ void k_item_map(const int *__restrict__ cstart, long ntiles, int2 *__restrict__ imap) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    const int b = cstart[t], e = cstart[t + 1];
+    for (int c = b; c < e; c++)
+        imap[c] = make_int2((int)t, c - b);
+}
+
+// the batched form: (global tile, chunk, system)
+__global__ void k_bitem_map(const int *__restrict__ cstart, long ntiles, const long *__restrict__ tile0s,
+                            int nsys, int4 *__restrict__ imap) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    const int b = cstart[t], e = cstart[t + 1];
+    if (b == e)
+        return;
+    const int s = fft::prefix_slot(tile0s, nsys, t);
+    for (int c = b; c < e; c++)
+        imap[c] = make_int4((int)t, c - b, s, 0);
+}
+
 template <int P, typename T, bool R2C, bool IL, int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
-              const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
+              const int *__restrict__ cstart, const int2 *__restrict__ imap, long ntiles, int nt, int Tt, int M, double Hi,
               T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
               int n) {
     extern __shared__ unsigned char smem_raw[];
@@ -150,8 +176,9 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const int chunk = item - cstart[tile];
+    const int2 tc = imap[item];
+    const long tile = tc.x;
+    const int chunk = tc.y;
     const int pb = tstart[tile] + chunk * kChunk;
     const int pe = min(tstart[tile + 1], pb + kChunk);
     assign_item<P, T, R2C, IL, MODE>(sorted, pb, pe, tile, nt, Tt, M, Hi, mesh, perm, gv, n,
@@ -163,15 +190,16 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
 template <int P, typename T, bool R2C, bool IL, int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_assign_rows(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
-              const int *__restrict__ cstart, long ntiles, int nt, int Tt, int M, double Hi,
+              const int *__restrict__ cstart, const int2 *__restrict__ imap, long ntiles, int nt, int Tt, int M, double Hi,
               T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
               int region_doubles) {
     extern __shared__ unsigned char smem_raw[];
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const int chunk = item - cstart[tile];
+    const int2 tc = imap[item];
+    const long tile = tc.x;
+    const int chunk = tc.y;
     const int pb = tstart[tile] + chunk * kChunk;
     const int pe = min(tstart[tile + 1], pb + kChunk);
     double *acc = reinterpret_cast<double *>(smem_raw);
@@ -183,15 +211,17 @@ k_assign_rows(const double4 *__restrict__ sorted, const int *__restrict__ tstart
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
 k_interp_ik_tile(const double4 *__restrict__ sorted, const int *__restrict__ perm,
-                 const int *__restrict__ tstart, const int *__restrict__ cstart, long ntiles, int nt,
+                 const int *__restrict__ tstart, const int *__restrict__ cstart,
+                 const int2 *__restrict__ imap, long ntiles, int nt,
                  int Tt, int M, double Hi, const T *__restrict__ F0, const T *__restrict__ F1,
                  const T *__restrict__ F2, double prefac, double *__restrict__ fk, long stride) {
     extern __shared__ unsigned char smem_raw[];
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const int chunk = item - cstart[tile];
+    const int2 tc = imap[item];
+    const long tile = tc.x;
+    const int chunk = tc.y;
     const int pb = tstart[tile] + chunk * kChunk;
     const int pe = min(tstart[tile + 1], pb + kChunk);
     interp_ik_item<P, T, R2C, IL>(sorted, perm, pb, pe, tile, nt, Tt, M, Hi, F0, F1, F2, prefac, fk,
@@ -201,7 +231,8 @@ k_interp_ik_tile(const double4 *__restrict__ sorted, const int *__restrict__ per
 template <int P, typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
 k_interp_ad_tile(const double4 *__restrict__ sorted, const int *__restrict__ perm,
-                 const int *__restrict__ tstart, const int *__restrict__ cstart, long ntiles, int nt,
+                 const int *__restrict__ tstart, const int *__restrict__ cstart,
+                 const int2 *__restrict__ imap, long ntiles, int nt,
                  int Tt, int M, double Hi, double Leni, const T *__restrict__ phi, double prefac,
                  const double *__restrict__ selfb, double h, double *__restrict__ fk,
                  double *__restrict__ f, long stride) {
@@ -209,8 +240,9 @@ k_interp_ad_tile(const double4 *__restrict__ sorted, const int *__restrict__ per
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const int chunk = item - cstart[tile];
+    const int2 tc = imap[item];
+    const long tile = tc.x;
+    const int chunk = tc.y;
     const int pb = tstart[tile] + chunk * kChunk;
     const int pe = min(tstart[tile + 1], pb + kChunk);
     interp_ad_item<P, T, R2C, IL>(sorted, perm, pb, pe, tile, nt, Tt, M, Hi, Leni, phi, prefac,
@@ -393,6 +425,7 @@ struct p3mk_psort {
     int nmax, M, T, nt;
     long ntiles;
     int *key, *count, *start, *cursor, *perm, *chunks, *cstart, *scan_scratch;
+    int2 *imap; // work item -> (tile, chunk)
     double4 *sorted;
     int n; // particles of the last sort
 };
@@ -406,6 +439,7 @@ extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
     p3mk_release(s->perm);
     p3mk_release(s->chunks);
     p3mk_release(s->cstart);
+    p3mk_release(s->imap);
     p3mk_release(s->scan_scratch);
     p3mk_release(s->sorted);
     delete s;
@@ -429,6 +463,9 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->start, sizeof(int) * nt1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->chunks, sizeof(int) * nt1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->cstart, sizeof(int) * nt1);
+    // work items: at most one partial chunk per tile plus the full chunks
+    if (rc == P3MK_OK)
+        rc = p3mk_alloc((void **)&s->imap, sizeof(int2) * ((size_t)nn / kChunk + nt1));
     if (rc == P3MK_OK)
         rc = p3mk_alloc((void **)&s->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(s->ntiles));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->perm, sizeof(int) * nn);
@@ -476,11 +513,11 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
     case P:                                                                                        \
         if (il)                                                                                    \
             hipLaunchKernelGGL((k_assign_rows<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
-                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, rg.size);                                       \
         else                                                                                       \
             hipLaunchKernelGGL((k_assign_rows<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
-                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, rg.size);                                       \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
@@ -496,11 +533,11 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
     case P:                                                                                        \
         if (il)                                                                                    \
             hipLaunchKernelGGL((k_assign_halo<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
-                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, n);                                             \
         else                                                                                       \
             hipLaunchKernelGGL((k_assign_halo<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
-                               st, s->sorted, s->start, s->cstart, s->ntiles, s->nt, s->T, g->M,   \
+                               st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, n);                                             \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
@@ -530,11 +567,11 @@ int launch_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const
     case P:                                                                                        \
         if (il)                                                                                    \
             hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
-                               s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T,    \
+                               s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T,    \
                                g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
         else                                                                                       \
             hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
-                               st, s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T, \
+                               st, s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, \
                                g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
@@ -565,11 +602,11 @@ int launch_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void 
     case P:                                                                                        \
         if (il)                                                                                    \
             hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
-                               s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T,    \
+                               s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T,    \
                                g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
         else                                                                                       \
             hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
-                               st, s->sorted, s->perm, s->start, s->cstart, s->ntiles, s->nt, s->T, \
+                               st, s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, \
                                g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
@@ -684,6 +721,9 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
                        s->count, s->ntiles, s->chunks);
     P3MK_TRY(p3mk_launch_check("tile chunks"));
     P3MK_TRY(p3mk_exclusive_scan(s->chunks, s->ntiles, s->cstart, s->scan_scratch, qu));
+    hipLaunchKernelGGL(k_item_map, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
+                       s->cstart, s->ntiles, s->imap);
+    P3MK_TRY(p3mk_launch_check("item map"));
     return assign_mode(g, n, s, mesh, kCharge, nullptr, qu);
 }
 
@@ -844,17 +884,17 @@ k_btile_scatter_agg(int ntot, int per_block, const BSys *__restrict__ sys, const
 
 template <typename T, bool R2C, bool IL>
 __global__ void __launch_bounds__(kThreads)
-k_bassign(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsys, long ntiles,
-          const int *__restrict__ start, const int *__restrict__ cstart,
+k_bassign(const BSys *__restrict__ sys, long ntiles, const int *__restrict__ start,
+          const int *__restrict__ cstart, const int4 *__restrict__ imap,
           const double4 *__restrict__ sorted) {
     extern __shared__ unsigned char smem_raw[];
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const int s = fft::prefix_slot(tile0s, nsys, tile);
-    const BSys &S = sys[s];
-    const int chunk = item - cstart[tile];
+    const int4 m = imap[item];
+    const long tile = m.x;
+    const BSys &S = sys[m.z];
+    const int chunk = m.y;
     const int pb = start[tile] + chunk * kChunk;
     const int pe = min(start[tile + 1], pb + kChunk);
     T *mesh = (T *)S.buf[kBufMesh];
@@ -876,17 +916,18 @@ k_bassign(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsy
 // of one tile's particles), the tile's halo region staged in LDS
 template <typename T, bool R2C, bool IL, bool AD>
 __global__ void __launch_bounds__(kThreads)
-k_binterp(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsys, long ntiles,
-          const int *__restrict__ start, const int *__restrict__ cstart,
+k_binterp(const BSys *__restrict__ sys, long ntiles, const int *__restrict__ start,
+          const int *__restrict__ cstart, const int4 *__restrict__ imap,
           const double4 *__restrict__ sorted, const int *__restrict__ perm, int ntot,
           double *__restrict__ fk, double *__restrict__ ftot) {
     extern __shared__ unsigned char smem_raw[];
     const int item = blockIdx.x;
     if (item >= cstart[ntiles])
         return;
-    const long tile = item_tile(cstart, ntiles, item);
-    const BSys &S = sys[fft::prefix_slot(tile0s, nsys, tile)];
-    const int chunk = item - cstart[tile];
+    const int4 m = imap[item];
+    const long tile = m.x;
+    const BSys &S = sys[m.z];
+    const int chunk = m.y;
     const int pb = start[tile] + chunk * kChunk;
     const int pe = min(start[tile + 1], pb + kChunk);
     const long tl = tile - S.tile0;
@@ -920,35 +961,34 @@ k_binterp(const BSys *__restrict__ sys, const long *__restrict__ tile0s, int nsy
 }
 
 template <typename T, bool R2C, bool IL>
-void launch_bassign(const BSys *sys, const long *tile0s, int nsys, long ntiles, long max_items,
-                    size_t lds, const int *start, const int *cstart, const double4 *sorted,
-                    hipStream_t st) {
+void launch_bassign(const BSys *sys, long ntiles, long max_items, size_t lds, const int *start,
+                    const int *cstart, const int4 *imap, const double4 *sorted, hipStream_t st) {
     hipLaunchKernelGGL((k_bassign<T, R2C, IL>), dim3((unsigned)max_items), dim3(kThreads), lds, st,
-                       sys, tile0s, nsys, ntiles, start, cstart, sorted);
+                       sys, ntiles, start, cstart, imap, sorted);
 }
 
 template <typename T, bool R2C, bool IL>
-void launch_binterp(bool ad, const BSys *sys, const long *tile0s, int nsys, long ntiles,
-                    long max_items, size_t lds, const int *start, const int *cstart,
-                    const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
-                    hipStream_t st) {
+void launch_binterp(bool ad, const BSys *sys, long ntiles, long max_items, size_t lds,
+                    const int *start, const int *cstart, const int4 *imap, const double4 *sorted,
+                    const int *perm, int ntot, double *fk, double *ftot, hipStream_t st) {
     const dim3 grid((unsigned)max_items);
     if (ad)
-        hipLaunchKernelGGL((k_binterp<T, R2C, IL, true>), grid, dim3(kThreads), lds, st, sys, tile0s,
-                           nsys, ntiles, start, cstart, sorted, perm, ntot, fk, ftot);
+        hipLaunchKernelGGL((k_binterp<T, R2C, IL, true>), grid, dim3(kThreads), lds, st, sys, ntiles,
+                           start, cstart, imap, sorted, perm, ntot, fk, ftot);
     else
         hipLaunchKernelGGL((k_binterp<T, R2C, IL, false>), grid, dim3(kThreads), lds, st, sys,
-                           tile0s, nsys, ntiles, start, cstart, sorted, perm, ntot, fk, ftot);
+                           ntiles, start, cstart, imap, sorted, perm, ntot, fk, ftot);
 }
 
 }  // namespace
 
 namespace p3m {
 
-int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, int ntot,
-               const double *x, const double *y, const double *z, const double *q, int *key,
-               int *count, int *start, int *cursor, int *perm, int *chunks, int *cstart,
-               int *scan_scratch, double4 *sorted, p3mk_queue *qu) {
+int bpart_sort(const BForm &f, const BSys *sys, const int *psys, const long *tile0s, int nsys,
+               long ntiles, int ntot, const double *x, const double *y, const double *z,
+               const double *q, int *key, int *count, int *start, int *cursor, int *perm,
+               int *chunks, int *cstart, int4 *imap, int *scan_scratch, double4 *sorted,
+               p3mk_queue *qu) {
     (void)f;
     if (ntot <= 0)
         return P3MK_OK;
@@ -970,18 +1010,21 @@ int bpart_sort(const BForm &f, const BSys *sys, const int *psys, long ntiles, in
     hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, st,
                        count, ntiles, chunks);
     P3MK_TRY(p3mk_launch_check("batched tile chunks"));
-    return p3mk_exclusive_scan(chunks, ntiles, cstart, scan_scratch, qu);
+    P3MK_TRY(p3mk_exclusive_scan(chunks, ntiles, cstart, scan_scratch, qu));
+    hipLaunchKernelGGL(k_bitem_map, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, st, cstart,
+                       ntiles, tile0s, nsys, imap);
+    return p3mk_launch_check("batched item map");
 }
 
-int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
-                 long max_items, size_t lds, const int *start, const int *cstart,
-                 const double4 *sorted, p3mk_queue *qu) {
+int bpart_assign(const BForm &f, const BSys *sys, long ntiles, long max_items, size_t lds,
+                 const int *start, const int *cstart, const int4 *imap, const double4 *sorted,
+                 p3mk_queue *qu) {
     if (max_items <= 0)
         return P3MK_OK;
     if (lds > 64 * 1024)
         return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
     hipStream_t st = p3mk_stream(qu);
-#define GO(T, R, I) launch_bassign<T, R, I>(sys, tile0s, nsys, ntiles, max_items, lds, start, cstart, sorted, st)
+#define GO(T, R, I) launch_bassign<T, R, I>(sys, ntiles, max_items, lds, start, cstart, imap, sorted, st)
     if (f.fp32) {
         if (f.r2c) { if (f.interlaced) GO(float, true, true); else GO(float, true, false); }
         else { if (f.interlaced) GO(float, false, true); else GO(float, false, false); }
@@ -993,17 +1036,16 @@ int bpart_assign(const BForm &f, const BSys *sys, const long *tile0s, int nsys, 
     return p3mk_launch_check("batched assignment");
 }
 
-int bpart_interp(const BForm &f, const BSys *sys, const long *tile0s, int nsys, long ntiles,
-                 long max_items, size_t lds, const int *start, const int *cstart,
-                 const double4 *sorted, const int *perm, int ntot, double *fk, double *ftot,
-                 p3mk_queue *qu) {
+int bpart_interp(const BForm &f, const BSys *sys, long ntiles, long max_items, size_t lds,
+                 const int *start, const int *cstart, const int4 *imap, const double4 *sorted,
+                 const int *perm, int ntot, double *fk, double *ftot, p3mk_queue *qu) {
     if (ntot <= 0 || max_items <= 0)
         return P3MK_OK;
     if (lds > 64 * 1024)
         return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
     hipStream_t st = p3mk_stream(qu);
     const bool ad = f.ad != 0;
-#define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, tile0s, nsys, ntiles, max_items, lds, start, cstart, sorted, perm, ntot, fk, ftot, st)
+#define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, ntiles, max_items, lds, start, cstart, imap, sorted, perm, ntot, fk, ftot, st)
     if (f.fp32) {
         if (f.r2c) { if (f.interlaced) GO(float, true, true); else GO(float, true, false); }
         else { if (f.interlaced) GO(float, false, true); else GO(float, false, false); }

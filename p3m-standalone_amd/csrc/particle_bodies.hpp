@@ -97,20 +97,32 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
     const int cnt = pe - pb, per_round = nwaves * kRowBatch;
     const int rounds = (cnt + per_round - 1) / per_round;
 
+    // the particle (and factor) of this lane in round rd, read one round ahead
+    auto fetch = [&](int rd, double4 &r, double &gq) {
+        const int p = pb + rd * per_round + wave * kRowBatch + lane;
+        if (rd < rounds && lane < kRowBatch && p < pe) {
+            r = sorted[p];
+            gq = (MODE == kWeighted) ? gv[perm[p]] : 1.0;
+        }
+    };
     for (int ii = 0; ii < NM; ii++) {
         const double half = 0.5 * ii;
+        double4 rn = make_double4(0.0, 0.0, 0.0, 0.0);
+        double gn = 1.0;
+        fetch(0, rn, gn);
         for (int k = threadIdx.x; k < rg.size; k += blockDim.x)
             acc[k] = A(0);
         __syncthreads();
         for (int rd = 0; rd < rounds; rd++) {
             const int b0 = pb + rd * per_round + wave * kRowBatch;
             const int nb = max(0, min(kRowBatch, pe - b0));
+            const double4 r = rn;
+            const double gq = gn;
+            fetch(rd + 1, rn, gn);
             if (lane < nb) {
-                const int p = b0 + lane;
-                const double4 r = sorted[p];
                 T qv = (T)r.w;
                 if (MODE == kWeighted)
-                    qv = (T)(r.w * gv[perm[p]]);
+                    qv = (T)(r.w * gq);
                 int c0, c1, c2;
                 T w0[P], w1[P], w2[P];
                 axis_weights<P, T>(r.x, Hi, half, M, c0, w0);

@@ -306,9 +306,8 @@ def test_method_table_drives_like_main(pkg, orc, name, method_id):
     pkg.check(pkg.lib().p3m_hip_free(d))
 
 
-@pytest.mark.parametrize("M", [16, 18, 20, 22, 24, 26, 28, 30, 36, 42, 44, 48, 64, 128])
-def test_fft3d_vs_numpy(pkg, M):
-    """The Stockham transform (radices 2..13) against numpy's FFT."""
+def _fft_api(pkg):
+    """ctypes handles of the library's internal FFT launchers."""
     L = pkg.lib()
     fn = {}
     for name, res, args in [
@@ -327,27 +326,117 @@ def test_fft3d_vs_numpy(pkg, M):
         f.restype, f.argtypes = res, args
         fn[name] = f
 
-    class Geom(ctypes.Structure):
-        _fields_ = [("M", ctypes.c_int), ("cao", ctypes.c_int), ("r2c", ctypes.c_int),
-                    ("interlaced", ctypes.c_int), ("fp32", ctypes.c_int), ("L", ctypes.c_double)]
+    for name, res, args in [
+        ("p3mk_fft_forward_zy", ctypes.c_int, [ctypes.c_void_p] * 4),
+        ("p3mk_fft_backward_yz", ctypes.c_int, [ctypes.c_void_p] * 4),
+    ]:
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+        fn[name] = f
+    return fn
 
+
+class _Geom(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int), ("cao", ctypes.c_int), ("r2c", ctypes.c_int),
+                ("interlaced", ctypes.c_int), ("fp32", ctypes.c_int), ("L", ctypes.c_double)]
+
+
+class _FftRig:
+    """A queue, a plan and device buffers for one transform geometry."""
+
+    def __init__(self, fn, M, r2c, fp32, nbytes):
+        self.fn = fn
+        self.q, self.plan = ctypes.c_void_p(), ctypes.c_void_p()
+        self.bufs = []
+        g = _Geom(M, 1, r2c, 0, fp32, 1.0)
+        assert fn["p3mk_queue_create"](ctypes.byref(self.q)) == 0
+        assert fn["p3mk_fft_create"](ctypes.byref(self.plan), ctypes.byref(g)) == 0
+        for nb in nbytes:
+            b = ctypes.c_void_p()
+            assert fn["p3mk_alloc"](ctypes.byref(b), nb) == 0
+            self.bufs.append(b)
+
+    def put(self, k, a):
+        assert self.fn["p3mk_copy_h2d"](self.bufs[k], a.ctypes.data, a.nbytes, self.q) == 0
+
+    def get(self, k, out):
+        assert self.fn["p3mk_copy_d2h"](out.ctypes.data, self.bufs[k], out.nbytes, self.q) == 0
+        assert self.fn["p3mk_queue_sync"](self.q) == 0
+        return out
+
+    def close(self):
+        for b in self.bufs:
+            self.fn["p3mk_release"](b)
+        self.fn["p3mk_fft_destroy"](self.plan)
+        self.fn["p3mk_queue_destroy"](self.q)
+
+
+@pytest.mark.parametrize("M", [16, 18, 20, 22, 24, 26, 28, 30, 36, 42, 44, 48, 64, 128])
+def test_fft3d_vs_numpy(pkg, M):
+    """The Stockham transform (radices 2..13) against numpy's FFT."""
+    fn = _fft_api(pkg)
     rng = np.random.default_rng(M)
     a = (rng.standard_normal((M, M, M)) + 1j * rng.standard_normal((M, M, M))).astype(np.complex128)
-    q, plan, buf = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
-    g = Geom(M, 1, 0, 0, 0, 1.0)
-    assert fn["p3mk_queue_create"](ctypes.byref(q)) == 0
-    assert fn["p3mk_fft_create"](ctypes.byref(plan), ctypes.byref(g)) == 0
-    assert fn["p3mk_alloc"](ctypes.byref(buf), a.nbytes) == 0
+    rig = _FftRig(fn, M, 0, 0, [a.nbytes])
     try:
         for sign in (-1, 1):
-            assert fn["p3mk_copy_h2d"](buf, a.ctypes.data, a.nbytes, q) == 0
-            assert fn["p3mk_fft3d"](plan, buf, sign, q) == 0
-            out = np.empty_like(a)
-            assert fn["p3mk_copy_d2h"](out.ctypes.data, buf, a.nbytes, q) == 0
-            assert fn["p3mk_queue_sync"](q) == 0
+            rig.put(0, a)
+            assert fn["p3mk_fft3d"](rig.plan, rig.bufs[0], sign, rig.q) == 0
+            out = rig.get(0, np.empty_like(a))
             ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a) * M ** 3
             assert rel_err(out, ref) < 1e-14
     finally:
-        fn["p3mk_release"](buf)
-        fn["p3mk_fft_destroy"](plan)
-        fn["p3mk_queue_destroy"](q)
+        rig.close()
+
+
+@pytest.mark.parametrize("M", [16, 18, 22, 26, 28, 30, 36, 42, 44, 64, 128])
+def test_fft3d_fp32_vs_numpy(pkg, M):
+    """The FP32 passes of the mixed form (radices 2..13) against numpy's FP64
+    FFT of the same FP32 input: relative error within a few FP32 roundings
+    per radix stage."""
+    fn = _fft_api(pkg)
+    rng = np.random.default_rng(100 + M)
+    a = (rng.standard_normal((M, M, M)) + 1j * rng.standard_normal((M, M, M))).astype(np.complex64)
+    rig = _FftRig(fn, M, 0, 1, [a.nbytes])
+    try:
+        for sign in (-1, 1):
+            rig.put(0, a)
+            assert fn["p3mk_fft3d"](rig.plan, rig.bufs[0], sign, rig.q) == 0
+            out = rig.get(0, np.empty_like(a))
+            a64 = a.astype(np.complex128)
+            ref = np.fft.fftn(a64) if sign < 0 else np.fft.ifftn(a64) * M ** 3
+            err = rel_err(out.astype(np.complex128), ref)
+            print("M %d sign %d: %.2e" % (M, sign, err))
+            assert err < 2e-6
+    finally:
+        rig.close()
+
+
+@pytest.mark.parametrize("M,fp32", [(16, 0), (22, 0), (30, 0), (64, 0), (128, 0), (16, 1), (26, 1),
+                                    (42, 1), (64, 1), (128, 1)])
+def test_fft_r2c_zy_vs_numpy(pkg, M, fp32):
+    """The r2c z pass with the y pass (methods 6 and 7) and the inverse y pass
+    with the c2r z pass, against numpy: spectrum[x, y, k] = fft(rfft(a, z), y)
+    for k <= M/2, and back M^2 irfft(ifft(., y), z)."""
+    fn = _fft_api(pkg)
+    H = M // 2 + 1
+    rdt, cdt = (np.float32, np.complex64) if fp32 else (np.float64, np.complex128)
+    rng = np.random.default_rng(200 + M + fp32)
+    a = rng.standard_normal((M, M, M)).astype(rdt)
+    spec = np.empty((M, M, H), dtype=cdt)
+    rig = _FftRig(fn, M, 1, fp32, [a.nbytes, spec.nbytes])
+    tol = 2e-6 if fp32 else 1e-14
+    try:
+        rig.put(0, a)
+        assert fn["p3mk_fft_forward_zy"](rig.plan, rig.bufs[0], rig.bufs[1], rig.q) == 0
+        rig.get(1, spec)
+        ref = np.fft.fft(np.fft.rfft(a.astype(np.float64), axis=2), axis=1)
+        ef = rel_err(spec.astype(np.complex128), ref)
+        assert fn["p3mk_fft_backward_yz"](rig.plan, rig.bufs[1], rig.bufs[0], rig.q) == 0
+        back = rig.get(0, np.empty_like(a))
+        eb = rel_err(back.astype(np.float64), M * M * a.astype(np.float64))
+        print("M %d fp32 %d: forward %.2e, round trip %.2e" % (M, fp32, ef, eb))
+        assert ef < tol
+        assert eb < tol
+    finally:
+        rig.close()
