@@ -69,6 +69,71 @@ k_fft_p2(c2_t<T> *data, LineMap m, const c2_t<T> *__restrict__ tw, int inv) {
     }
 }
 
+// The fused x pass (convolve_x_tile) for power-of-two meshes on registers:
+// forward along x, X <- G X (FP64 product rounded to T), then for nout == 1
+// backward into out0, for nout == 3 backward of i (2 pi / L) Dn[n_d] X into
+// out_d.  Lines c = j W + k (strided by nl = M W), element e = n_x.
+template <int N, typename T>
+__global__ void __launch_bounds__(kThreads)
+k_convolve_p2(const c2_t<T> *in, c2_t<T> *out0, c2_t<T> *out1, c2_t<T> *out2, long nl, int W,
+              const c2_t<T> *__restrict__ tw, const double *__restrict__ G,
+              const double *__restrict__ Dn, double twopi_L, int nout) {
+    using V = c2_t<T>;
+    constexpr int TPL = N / 8, LPW = kThreads / TPL, PITCH = N + 1;
+    __shared__ V buf[LPW * PITCH];
+    const int tid = threadIdx.x;
+    const int l = tid % LPW, t = tid / LPW;
+    const long L = (long)blockIdx.x * LPW + l;
+    const bool ok = L < nl;
+    const int j = ok ? (int)(L / W) : 0, k = ok ? (int)(L - (long)j * W) : 0;
+    V v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        if (ok) {
+            v[s] = in[L + (long)(t + TPL * s) * nl];
+        } else {
+            v[s].x = T(0);
+            v[s].y = T(0);
+        }
+    }
+    V *line = buf + l * PITCH;
+    p2_fft<N>(v, line, t, tw, 0);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int e = t + TPL * s;
+        const double g = ok ? G[((long)e * N + j) * N + k] : 0.0;
+        v[s].x = (T)((double)v[s].x * g);
+        v[s].y = (T)((double)v[s].y * g);
+    }
+    if (nout == 1) {
+        p2_fft<N>(v, line, t, tw, 1);
+        if (ok) {
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                out0[L + (long)(t + TPL * s) * nl] = v[s];
+        }
+        return;
+    }
+    V *outs[3] = {out0, out1, out2};
+    for (int d = 0; d < 3; d++) {
+        V w[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int e = t + TPL * s;
+            const int nd = (d == 0) ? e : ((d == 1) ? j : k);
+            const double f = twopi_L * Dn[nd];
+            w[s].x = (T)(-f * (double)v[s].y);
+            w[s].y = (T)(f * (double)v[s].x);
+        }
+        p2_fft<N>(w, line, t, tw, 1);
+        if (ok) {
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                outs[d][L + (long)(t + TPL * s) * nl] = w[s];
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_fft_r2c_z(const T *__restrict__ in, c2_t<T> *__restrict__ out, long nlines, int n, Radices rp,
@@ -271,6 +336,21 @@ template <typename T> int backward_yz(p3mk_fft *f, void *spec, void *mesh, p3mk_
     return p3mk_launch_check("fft c2r");
 }
 
+template <int N, typename T>
+int convolve_p2(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spec,
+                void *const out[3], long nl, int W, p3mk_queue *q) {
+    using V = c2_t<T>;
+    constexpr int LPW = kThreads / (N / 8);
+    const c2_t<T> *tw = sizeof(T) == 8 ? (const V *)f->tw64 : (const V *)f->tw32;
+    V *o0 = (V *)out[0];
+    V *o1 = nout == 3 ? (V *)out[1] : nullptr;
+    V *o2 = nout == 3 ? (V *)out[2] : nullptr;
+    hipLaunchKernelGGL((k_convolve_p2<N, T>), dim3((unsigned)((nl + LPW - 1) / LPW)), dim3(kThreads), 0,
+                       p3mk_stream(q), (const V *)spec, o0, o1, o2, nl, W, tw, G, Dn,
+                       2.0 * 3.14159265358979323846 / f->g.L, nout);
+    return p3mk_launch_check("convolve");
+}
+
 template <typename T>
 int convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spec,
              void *const out[3], p3mk_queue *q) {
@@ -278,6 +358,14 @@ int convolve(p3mk_fft *f, int nout, const double *G, const double *Dn, void *spe
     const int M = f->M;
     const int W = f->g.r2c ? f->H : M;
     const long nl = (long)M * W;
+    switch (M) {
+    case 16: return convolve_p2<16, T>(f, nout, G, Dn, spec, out, nl, W, q);
+    case 32: return convolve_p2<32, T>(f, nout, G, Dn, spec, out, nl, W, q);
+    case 64: return convolve_p2<64, T>(f, nout, G, Dn, spec, out, nl, W, q);
+    case 128: return convolve_p2<128, T>(f, nout, G, Dn, spec, out, nl, W, q);
+    case 256: return convolve_p2<256, T>(f, nout, G, Dn, spec, out, nl, W, q);
+    default: break;
+    }
     LineMap x = {M, nl, nl, 0, nl};
     const int nbuf = (nout == 3) ? 3 : 2;
     const int lt = launch_lines<T>(M, nbuf, nl);

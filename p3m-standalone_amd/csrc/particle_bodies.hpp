@@ -61,18 +61,45 @@ __host__ __device__ inline RowRegion row_region(int Tt, int P, bool il) {
     g.Sy = g.R + ((g.R & 1) ? 0 : 1);
     const int want = (P * g.Sy) & 63, base = g.R * g.Sy;
     g.Sx = base + ((want - (base & 63)) & 63);
-    g.size = g.R * g.Sx;
+    g.size = (g.R * g.Sx + 1) & ~1; // even: the staging area after it stays 16-byte aligned
     return g;
 }
 
+// staged record of one particle: z weights w2 in a block of row_w2<P>()
+// values (read by 16-byte loads), then q w0[P] and w1[P]; records are
+// 16-byte aligned
+template <int P> __host__ __device__ constexpr int row_w2() { return (P + 3) & ~3; }
 template <int P> __host__ __device__ constexpr int row_record() {
-    return (3 * P + 1) & ~1; // w2[P], q w0[P], w1[P], padded to an even count
+    return (row_w2<P>() + 2 * P + 3) & ~3;
 }
 
 // LDS bytes of one wavefront's staging area: kRowBatch records of T and the
 // particles' region offsets
 template <int P, typename T> __host__ __device__ constexpr int row_stage_bytes() {
     return kRowBatch * (row_record<P>() * (int)sizeof(T) + (int)sizeof(int));
+}
+
+// n values of a 16-byte-aligned LDS row by 16-byte loads
+template <typename T, int n> __device__ __forceinline__ void load_row(const T *src, T *dst) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 *s = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+        for (int k = 0; k < n / 4; k++) {
+            const float4 v = s[k];
+            dst[4 * k] = v.x;
+            dst[4 * k + 1] = v.y;
+            dst[4 * k + 2] = v.z;
+            dst[4 * k + 3] = v.w;
+        }
+    } else {
+        const double2 *s = reinterpret_cast<const double2 *>(src);
+#pragma unroll
+        for (int k = 0; k < n / 2; k++) {
+            const double2 v = s[k];
+            dst[2 * k] = v.x;
+            dst[2 * k + 1] = v.y;
+        }
+    }
 }
 
 template <int P, typename T, bool R2C, bool IL, int MODE, typename A>
@@ -83,6 +110,7 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
     static_assert(MODE == kCharge || MODE == kWeighted, "row assignment deposits charges");
     constexpr int NM = IL ? 2 : 1;
     constexpr int REC = row_record<P>();
+    constexpr int W2 = row_w2<P>();
     constexpr int P2 = P * P;
     constexpr int G = 64 / P2; // particles per deposit step
     const RowRegion rg = row_region(Tt, P, IL);
@@ -136,8 +164,8 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
 #pragma unroll
                 for (int a = 0; a < P; a++) {
                     rec[a] = w2[a];
-                    rec[P + a] = qv * w0[a];
-                    rec[2 * P + a] = w1[a];
+                    rec[W2 + a] = qv * w0[a];
+                    rec[W2 + P + a] = w1[a];
                 }
                 woff[lane] = o0 * rg.Sx + o1 * rg.Sy + o2;
             }
@@ -146,11 +174,13 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
                 const int jj = j + grp;
                 if (row_lane && jj < nb) {
                     const T *rec = wst + jj * REC;
-                    const T tr = rec[P + ra0] * rec[2 * P + ra1];
+                    T wz[W2];
+                    load_row<T, W2>(rec, wz);
+                    const T tr = rec[W2 + ra0] * rec[W2 + P + ra1];
                     A *a = acc + woff[jj] + ra0 * rg.Sx + ra1 * rg.Sy;
 #pragma unroll
                     for (int a2 = 0; a2 < P; a2++)
-                        atomicAdd(&a[a2], (A)(tr * rec[a2]));
+                        atomicAdd(&a[a2], (A)(tr * wz[a2]));
                 }
             }
             __syncthreads();
