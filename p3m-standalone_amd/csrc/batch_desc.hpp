@@ -140,6 +140,9 @@ struct CellArrays {
     // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, q); the sorted
     // arrays carry 64 padding entries past the last particle
     float4 *sorted32 = nullptr;
+    // relative to the cell centre in fixed point (kFixBits fractional bits
+    // per side): (x, y, z, q as FP32 bits)
+    int4 *sortedq = nullptr;
     double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
     double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n
     float4 *bbox = nullptr;     // 2 per work item

@@ -91,24 +91,38 @@ __device__ __forceinline__ int fine_key(double xf, double yf, double zf, double 
     return ((cx * nc + cy) * nc + cz) * 8 + oct;
 }
 
-// Sorted copies of one particle: FP64 (x, y, z, q), and relative to the
-// centre of its cell (cx, cy, cz) in FP32 the row (-2 x, -2 y, -2 z, q) the
-// candidate search builds its matrix rows from.
+// Fixed-point coordinates relative to a cell centre: kFixBits fractional
+// bits per cell side, so differences of two such coordinates in one frame
+// are exact integers.  A frame spans the (2 reach + 1)^3 cells around its
+// centre, |v| <= 2.5 sides < 2^31 / 2^kFixBits.
+constexpr int kFixBits = 29;
+
+__device__ __forceinline__ int to_fixed(double v, double inv_quantum) {
+    return (int)rint(v * inv_quantum);
+}
+
+// Sorted copies of one particle: FP64 (x, y, z, q); relative to the centre
+// of its cell (cx, cy, cz) in FP32 the row (-2 x, -2 y, -2 z, q) the
+// candidate search builds its matrix rows from; and in fixed point
+// (x, y, z, q as FP32 bits) for the FP32 pair difference.
 __device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int nc, double side,
                                              double4 *__restrict__ sorted,
-                                             float4 *__restrict__ sorted32) {
+                                             float4 *__restrict__ sorted32,
+                                             int4 *__restrict__ sortedq) {
     const int cz = cell % nc, cy = (cell / nc) % nc, cx = cell / (nc * nc);
-    const float lx = (float)(r.x - (cx + 0.5) * side), ly = (float)(r.y - (cy + 0.5) * side),
-                lz = (float)(r.z - (cz + 0.5) * side);
+    const double rx = r.x - (cx + 0.5) * side, ry = r.y - (cy + 0.5) * side, rz = r.z - (cz + 0.5) * side;
     sorted[pos] = r;
-    sorted32[pos] = make_float4(-2.0f * lx, -2.0f * ly, -2.0f * lz, (float)r.w);
+    sorted32[pos] = make_float4(-2.0f * (float)rx, -2.0f * (float)ry, -2.0f * (float)rz, (float)r.w);
+    const double iq = ldexp(1.0, kFixBits) / side;
+    sortedq[pos] = make_int4(to_fixed(rx, iq), to_fixed(ry, iq), to_fixed(rz, iq), __float_as_int((float)r.w));
 }
 
 // the kPad entries past the last particle: finite rows of zero charge
 __device__ __forceinline__ void store_pad(int pos, double4 *__restrict__ sorted,
-                                          float4 *__restrict__ sorted32) {
+                                          float4 *__restrict__ sorted32, int4 *__restrict__ sortedq) {
     sorted[pos] = make_double4(0.0, 0.0, 0.0, 0.0);
     sorted32[pos] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    sortedq[pos] = make_int4(0, 0, 0, 0);
 }
 
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
@@ -130,13 +144,13 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
                                int nc, int *__restrict__ key, const int *__restrict__ start,
                                int *__restrict__ cursor, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
-                               const double *__restrict__ g,
+                               int4 *__restrict__ sortedq, const double *__restrict__ g,
                                double4 *__restrict__ gsorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n + kPad)
         return;
     if (i >= n) {
-        store_pad(i, sorted, sorted32);
+        store_pad(i, sorted, sorted32, sortedq);
         if (g != nullptr)
             gsorted[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
@@ -146,7 +160,7 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32);
+    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32, sortedq);
     if (g != nullptr)
         gsorted[pos] = make_double4(g[i], g[n + i], g[2 * (long)n + i], 0.0);
 }
@@ -799,7 +813,8 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // several times slower in the reaction sums.
 
 constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
-constexpr int kHalfQueue = 3 * kWave;              // 127 pending (paired rounds) + one append
+// 127 pending (paired rounds) + the eight appends of one search step
+constexpr int kHalfQueue = 10 * kWave;
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
 
 template <bool FP32> struct RsHalfShared {
@@ -810,15 +825,19 @@ template <bool FP32> struct RsHalfShared {
     halfx4 win_a[kWave];         // (-2 n, 1) in FP16, n = n' / side, n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
+    // FP32 form: (n', q as FP32 bits), n' the fixed-point position of the
+    // window particle's image relative to the home cell centre
+    int4 win_q[FP32 ? kWave : 1];
     double tab[FP32 ? 1 : kErfcxTab];
 };
 
 template <bool FP32, bool PAIRED_ROUNDS>
 __device__ __forceinline__ void
 realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ sorted,
-                    const float4 *__restrict__ sorted32, const int *__restrict__ start,
-                    const int *__restrict__ istart, const float4 *__restrict__ bbox, int nc,
-                    int reach, double L, double rcut, double alpha, long stride,
+                    const float4 *__restrict__ sorted32, const int4 *__restrict__ sortedq,
+                    const int *__restrict__ start, const int *__restrict__ istart,
+                    const float4 *__restrict__ bbox, int nc, int reach, double L, double rcut,
+                    double alpha, long stride,
                     double *__restrict__ fs, RsHalfShared<FP32> &sh, double &en_tot) {
     const int lane = threadIdx.x;
     const double two_a_spi = 2.0 * alpha / sqrt_pi();
@@ -878,34 +897,84 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     float en32 = 0.0f;
     __syncthreads();
 
-    // one queued pair: what a round reads before its arithmetic (both
-    // positions from global memory, through L1)
+    // FP32 form: the pair difference d = h - n' is the exact integer
+    // difference of the home particle h and the window particle's image n',
+    // both in fixed point relative to the home cell centre (quantum
+    // s = side 2^-kFixBits, each rounded once from FP64), scaled by s in
+    // FP32.  Per component |d~ - d| <= s + 2 u |d|, u = 2^-24, and r2f, the
+    // FP32 sum of squares, is within (2 sqrt(3) s / rcut + 8 u) rcut^2 of r^2
+    // near the cutoff.  The inclusion test is decided on r2f outside a band
+    // of four times that width around rcut^2; inside it the reference's FP64
+    // test runs on the FP64 positions.  The relative error of d is at most
+    // 2 u + sqrt(3) s / r: 3.2e-7 for a pair side / 100 apart.
+    const double quantum = ldexp(side, -kFixBits);
+    const float quantum_f = (float)quantum;
+    const double band = 4.0 * (3.4642 * quantum / rcut + 8.0 * 0x1p-24);
+    const float r2_in = (float)(rcut * rcut * (1.0 - band)), r2_out = (float)(rcut * rcut * (1.0 + band));
+    // one queued pair: what a round reads before its arithmetic (FP32: the
+    // window particle from LDS, the home particle from sortedq through L1;
+    // FP64: both positions from global memory, through L1)
     struct Fetched {
         int o, slot;
         unsigned int js;
         double4 pj, ph;
+        int4 nq, hq;
     };
     auto fetch = [&](int base, Fetched &f) {
         const unsigned int e = sh.qs[base + lane];
         f.o = (int)(e & 63u);
         f.slot = (int)(e >> 6);
-        f.js = sh.win_js[f.slot];
-        f.pj = sorted[f.js & kIdxMask];
-        f.ph = sorted[i0 + f.o];
+        if constexpr (FP32) {
+            f.nq = sh.win_q[f.slot];
+            f.hq = sortedq[i0 + f.o];
+        } else {
+            f.js = sh.win_js[f.slot];
+            f.pj = sorted[f.js & kIdxMask];
+            f.ph = sorted[i0 + f.o];
+        }
     };
-    auto evaluate = [&](const Fetched &f) {
-        const int o = f.o, slot = f.slot;
-        // the image shift, read only by the lanes whose pair wraps
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        const unsigned int sidx = f.js >> 27;
+    // the image shift of window entry js (0 unless the pair wraps)
+    auto image_shift = [&](unsigned int js, double &sx, double &sy, double &sz) {
+        sx = 0.0;
+        sy = 0.0;
+        sz = 0.0;
+        const unsigned int sidx = js >> 27;
         if (sidx != kNoShift) {
             const double4 s3 = sh.shift[sidx];
             sx = s3.x;
             sy = s3.y;
             sz = s3.z;
         }
+    };
+    auto evaluate = [&](const Fetched &f) {
+        const int o = f.o, slot = f.slot;
         PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
-        pair<FP32>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+        if constexpr (FP32) {
+            const float dx = quantum_f * (float)(f.hq.x - f.nq.x), dy = quantum_f * (float)(f.hq.y - f.nq.y),
+                        dz = quantum_f * (float)(f.hq.z - f.nq.z);
+            const float r2f = dx * dx + dy * dy + dz * dz;
+            bool inside = r2f <= r2_in;
+            if (__builtin_expect(!inside && r2f <= r2_out, 0)) {
+                const unsigned int js = sh.win_js[slot];
+                const double4 pj = sorted[js & kIdxMask], ph = sorted[i0 + o];
+                double sx, sy, sz;
+                image_shift(js, sx, sy, sz);
+                double ex, ey, ez;
+                {
+#pragma clang fp contract(off)
+                    ex = (ph.x - pj.x) - sx;
+                    ey = (ph.y - pj.y) - sy;
+                    ez = (ph.z - pj.z) - sz;
+                }
+                inside = within_cutoff(ex, ey, ez, rcut);
+            }
+            pair_force_f32(dx, dy, dz, r2f, inside, __int_as_float(f.hq.w) * __int_as_float(f.nq.w), alpha,
+                           two_a_spi, pa);
+        } else {
+            double sx, sy, sz;
+            image_shift(f.js, sx, sy, sz);
+            pair<false>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
+        }
         atomicAdd(&sh.facc[0][o + oofs], pa.f0);
         atomicAdd(&sh.facc[1][o + oofs], pa.f1);
         atomicAdd(&sh.facc[2][o + oofs], pa.f2);
@@ -1008,7 +1077,6 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                         if (SELFW)
                             ok = ok && keep(sh.win_js[base + rq], i_own);
                         append(ok, e0 + (rq << 6));
-                        flush();
                     }
                 };
                 for (int s = 0; s < G; s += 2) {
@@ -1022,6 +1090,8 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     d1 = __builtin_amdgcn_mfma_f32_4x4x4f16(a1, bop, d1, 0, 0, 0);
                     emit(d0, base0);
                     emit(d1, base1);
+                    // one drain site per step keeps the kernel's code small
+                    flush();
                 }
             };
             if (selfwin)
@@ -1045,11 +1115,11 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 if (selfwin)
                     ok = ok && keep(js, (unsigned)(i0 + o));
                 append(ok, (unsigned)o | (unsigned)lane << 6);
-                flush();
                 o = (o + 1 == nh) ? 0 : o + 1;
             }
         }
         __syncthreads();
+        flush();
         if (count >= kWave)
             drain(kWave);
         if (count > 0)
@@ -1133,6 +1203,10 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                           wz = (int)((info >> 13) & 3u) - 1;
                 const unsigned int sidx = shift_index(wx, wy, wz) << 27;
                 const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
+                // FP32 form: the image relative to the home cell centre
+                const double dox = wx * L - (hx + 0.5) * side, doy = wy * L - (hy + 0.5) * side,
+                             doz = wz * L - (hz + 0.5) * side;
+                const double inv_quantum = ldexp(1.0, kFixBits) / side;
                 // window rows: A' = (-2 (n + o), 1), o the cell centre relative to the home's
                 const float ox2 = (float)(2.0 * a * side), oy2 = (float)(2.0 * b * side),
                             oz2 = (float)(2.0 * c * side);
@@ -1152,8 +1226,14 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     bool keep_l = false;
                     halfx4 arow = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)1.0f};
                     float n2 = 0.0f;
+                    int4 nq = make_int4(0, 0, 0, 0);
                     if (lane < cnt) {
                         const float4 v = sorted32[cb + lane];
+                        if constexpr (FP32) {
+                            const double4 p = sorted[cb + lane];
+                            nq = make_int4(to_fixed(p.x + dox, inv_quantum), to_fixed(p.y + doy, inv_quantum),
+                                           to_fixed(p.z + doz, inv_quantum), __float_as_int(v.w));
+                        }
                         const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
                         const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
                         const float gxx = gap(hlo_r.x, hhi_r.x, nx, nx), gyy = gap(hlo_r.y, hhi_r.y, ny, ny),
@@ -1176,6 +1256,8 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                             sh.win_a[w + rank] = arow;
                             sh.win_n2[w + rank] = n2;
                             sh.win_js[w + rank] = (unsigned)(cb + lane) | sidx;
+                            if constexpr (FP32)
+                                sh.win_q[w + rank] = nq;
                         }
                         w += take;
                         keep_l = keep_l && rank >= take;
@@ -1214,7 +1296,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
 template <bool FP32>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(FP32 ? 4 : 3)))
 k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-                 const int *__restrict__ start, const int *__restrict__ istart,
+                 const int4 *__restrict__ sortedq, const int *__restrict__ start, const int *__restrict__ istart,
                  const float4 *__restrict__ bbox, long ncell, int nc, int reach, double L,
                  double rcut, double alpha, int *__restrict__ counter, long stride,
                  double *__restrict__ fs, double *__restrict__ energy) {
@@ -1235,7 +1317,7 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
             break;
         // the item's cell and chunk, stored with its box
         const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
-        realspace_item_half<FP32, true>(home, chunk, 0, sorted, sorted32, start, istart, bbox, nc, reach, L,
+        realspace_item_half<FP32, true>(home, chunk, 0, sorted, sorted32, sortedq, start, istart, bbox, nc, reach, L,
                                         rcut, alpha, stride, fs, sh, en_tot);
     }
     if (energy != nullptr) {
@@ -1268,7 +1350,7 @@ namespace p3m {
 void cells_release(CellArrays &a) {
     // fcursor and counter live in the fcount allocation
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.sorted, a.sorted32, a.bbox, a.gsorted, a.fsorted};
+                    a.scan, a.sorted, a.sorted32, a.sortedq, a.bbox, a.gsorted, a.fsorted};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -1298,6 +1380,7 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     const size_t np = nn + kPad;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted32, sizeof(float4) * np);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sortedq, sizeof(int4) * np);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.bbox, sizeof(float4) * 2 * (size_t)a.max_items);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fsorted, sizeof(double) * 3 * nn);
     if (rc != P3MK_OK)
@@ -1396,7 +1479,7 @@ static int build_cells(p3mk_cells *c, int n, double L, const double *x, const do
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, g, a.gsorted);
+                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.sortedq, g, a.gsorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     return p3m::cells_end(a, c->ncell, qu);
 }
@@ -1439,11 +1522,11 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
         P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)n, qu));
         if (fp32)
             hipLaunchKernelGGL(k_realspace_half<true>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
-                               a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
                                a.counter, (long)n, a.fsorted, energy);
         else
             hipLaunchKernelGGL(k_realspace_half<false>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
-                               a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
+                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
                                a.counter, (long)n, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("realspace (half shell)"));
         hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a.key,
@@ -1493,12 +1576,13 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
                                 const double *__restrict__ z, const double *__restrict__ q,
                                 int *__restrict__ key, const int *__restrict__ start,
                                 int *__restrict__ cursor, int *__restrict__ perm,
-                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32) {
+                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
+                                int4 *__restrict__ sortedq) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot + kPad)
         return;
     if (i >= ntot) {
-        store_pad(i, sorted, sorted32);
+        store_pad(i, sorted, sorted32, sortedq);
         return;
     }
     const BSys &S = sys[psys[i]];
@@ -1508,7 +1592,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32);
+    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32, sortedq);
 }
 
 template <bool FP32>
@@ -1556,8 +1640,9 @@ template <bool FP32>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int nsys, int ntot,
                   const double4 *__restrict__ sorted, const float4 *__restrict__ sorted32,
-                  const int *__restrict__ start, const int *__restrict__ istart,
-                  const float4 *__restrict__ bbox, long ncell, int *__restrict__ counter,
+                  const int4 *__restrict__ sortedq, const int *__restrict__ start,
+                  const int *__restrict__ istart, const float4 *__restrict__ bbox, long ncell,
+                  int *__restrict__ counter,
                   double *__restrict__ fs, double *__restrict__ energy) {
     __shared__ RsHalfShared<FP32> sh;
     const int total = istart[ncell];
@@ -1579,7 +1664,7 @@ k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, 
         if (!(S.rcut > 0.0) || S.small)
             continue; // no real-space part, or nearest images (k_brealspace)
         double en = 0.0;
-        realspace_item_half<FP32, FP32>(home - S.cell0, chunk, S.cell0, sorted, sorted32, start,
+        realspace_item_half<FP32, FP32>(home - S.cell0, chunk, S.cell0, sorted, sorted32, sortedq, start,
                                   istart, bbox, S.nc, S.reach, S.L, S.rcut, S.alpha, ntot, fs, sh, en);
         if (energy != nullptr) {
             for (int off = 32; off > 0; off >>= 1)
@@ -1608,7 +1693,8 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
     P3MK_TRY(p3mk_launch_check("batched cell count"));
     P3MK_TRY(cells_scan(a, ncell, qu));
     hipLaunchKernelGGL(k_bcell_scatter, dim3((ntot + kPad + 255) / 256), dim3(256), 0, s, ntot, sys,
-                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32);
+                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32,
+                       a.sortedq);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
     if (any_small) {
@@ -1628,11 +1714,11 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
         P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)ntot, qu));
         if (f.fp32)
             hipLaunchKernelGGL(k_brealspace_half<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
-                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.start, a.istart, a.bbox, ncell,
+                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.sortedq, a.start, a.istart, a.bbox, ncell,
                                a.counter + 1, a.fsorted, energy);
         else
             hipLaunchKernelGGL(k_brealspace_half<false>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
-                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.start, a.istart, a.bbox, ncell,
+                               cell0s, nsys, ntot, a.sorted, a.sorted32, a.sortedq, a.start, a.istart, a.bbox, ncell,
                                a.counter + 1, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("batched real space (half shell)"));
         hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, ntot,
