@@ -509,6 +509,11 @@ int p3m_hip_stage_times(data_t *d, double out[5]);
  * reference's runtime_t timers (p3m-common.h:48-57) for a sequence of steps. */
 int p3m_hip_stage_ring_start(data_t *d, int nsteps);
 int p3m_hip_stage_ring_read(data_t *d, double *out, int max_steps, int *nsteps);
+/* For the same recorded steps: out[s] = time (ms) of the real-space pair
+ * kernel alone (the cell-list build excluded), 0 for a step without real
+ * space.  No counterpart in the reference (its runtime_t has no finer
+ * timers). */
+int p3m_hip_stage_ring_read_pairs(data_t *d, double *out, int max_steps, int *nsteps);
 /* Copy the influence function (M^3 doubles, r_ind order) to the host. */
 int p3m_hip_get_influence_function(data_t *d, double *G);
 /* Copy the self-force coefficients (27 x 3 doubles) to the host. */

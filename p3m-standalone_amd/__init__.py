@@ -173,6 +173,7 @@ def lib():
             "p3m_hip_ewald_sample_device": (i, [i, d, vp, vp, vp, vp, i, vp, d, d, i, d, vp, vp]),
             "p3m_hip_stage_ring_start": (i, [_DPT, i]),
             "p3m_hip_stage_ring_read": (i, [_DPT, _dp, i, ip]),
+            "p3m_hip_stage_ring_read_pairs": (i, [_DPT, _dp, i, ip]),
             "p3m_hip_get_influence_function": (i, [_DPT, _dp]),
             "p3m_hip_get_self_force_coefficients": (i, [_DPT, _dp]),
         }
@@ -362,6 +363,15 @@ class Solver:
         k = ctypes.c_int()
         check(lib().p3m_hip_stage_ring_read(self.d, out.ctypes.data_as(_dp), int(max_steps),
                                             ctypes.byref(k)))
+        return out[:k.value]
+
+    def stage_ring_read_pairs(self, max_steps):
+        """(steps,) times in ms of the real-space pair kernel alone in the
+        recorded calls."""
+        out = np.zeros(max_steps)
+        k = ctypes.c_int()
+        check(lib().p3m_hip_stage_ring_read_pairs(self.d, out.ctypes.data_as(_dp), int(max_steps),
+                                                  ctypes.byref(k)))
         return out[:k.value]
 
     # device-resident entry points (pointers to device memory, e.g. torch tensors)

@@ -613,6 +613,27 @@ int p3m_hip_stage_ring_read(data_t *d, double *out, int max_steps, int *nsteps) 
     return P3M_HIP_OK;
 }
 
+int p3m_hip_stage_ring_read_pairs(data_t *d, double *out, int max_steps, int *nsteps) {
+    p3m_ctx *c = ctx_of(d);
+    int s, filled;
+    float t;
+    if (c == NULL || out == NULL || nsteps == NULL)
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
+    TRYK(p3mk_queue_sync(c->q), "sync");
+    filled = p3mk_ring_filled(c->q);
+    if (filled > max_steps)
+        filled = max_steps;
+    for (s = 0; s < filled; s++) {
+        out[s] = 0.0;
+        if (c->ring_rs[s]) {
+            TRYK(p3mk_ring_elapsed_ms(c->q, s, P3MK_EV_PAIRS, P3MK_EV_PAIRS + 1, &t), "event timing");
+            out[s] = t;
+        }
+    }
+    *nsteps = filled;
+    return P3M_HIP_OK;
+}
+
 static void add_host(double *dst, const double *src, size_t n) {
     size_t i;
     for (i = 0; i < n; i++)

@@ -1520,15 +1520,18 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
         // half shell: forces summed in sorted order, then added by perm
         P3MK_TRY(build_cells(c, n, L, x, y, z, q, rcut, nullptr, qu));
         P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)n, qu));
+        // the pair kernel alone between two events
+        P3MK_TRY(p3mk_record(qu, P3MK_EV_PAIRS));
         if (fp32)
             hipLaunchKernelGGL(k_realspace_half<true>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
-                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
-                               a.counter, (long)n, a.fsorted, energy);
+                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                               rcut, alpha, a.counter, (long)n, a.fsorted, energy);
         else
             hipLaunchKernelGGL(k_realspace_half<false>, grid, dim3(kWave), 0, s, a.sorted, a.sorted32,
-                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,
-                               a.counter, (long)n, a.fsorted, energy);
+                               a.sortedq, a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L,
+                               rcut, alpha, a.counter, (long)n, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("realspace (half shell)"));
+        P3MK_TRY(p3mk_record(qu, P3MK_EV_PAIRS + 1));
         hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a.key,
                            a.fsorted, (long)n, fr);
         return p3mk_launch_check("realspace forces to caller order");
@@ -1539,12 +1542,14 @@ extern "C" int p3mk_realspace(p3mk_cells *c, int n, double L, const double *x, c
                        a.perm,                                                                     \
                        a.start, a.istart, a.bbox, c->ncell, c->nc, c->reach, L, rcut, alpha,        \
                        a.counter, fr, energy)
+    P3MK_TRY(p3mk_record(qu, P3MK_EV_PAIRS));
     if (fp32)
         LAUNCH(true, true);
     else
         LAUNCH(false, true);
 #undef LAUNCH
-    return p3mk_launch_check("realspace");
+    P3MK_TRY(p3mk_launch_check("realspace"));
+    return p3mk_record(qu, P3MK_EV_PAIRS + 1);
 }
 
 // ---- batched real space over many systems -----------------------------------

@@ -24,6 +24,8 @@ enum {
 };
 
 #define P3MK_NUM_EVENTS 8
+/* slots P3MK_EV_PAIRS and P3MK_EV_PAIRS + 1: around the real-space pair kernel */
+#define P3MK_EV_PAIRS 6
 
 const char *p3mk_error(void);
 
