@@ -132,6 +132,43 @@ k_tile_scatter_agg(int n, int per_block, const double *__restrict__ x, const dou
     }
 }
 
+// Sort by base mesh point: the key of a particle is its tile (tile-major)
+// and then the base's position inside the tile (the lowest stencil point on
+// mesh 0; T^3 <= 512 per tile), so a tile's particles are contiguous and
+// ordered by base.  Particles sharing a base become consecutive: the row
+// deposit sums their rows in registers before its LDS atomics, and
+// neighbouring lanes of the gathers read the same LDS words (a broadcast) or
+// neighbouring ones.  One device atomic per particle, spread over M^3 keys
+// (at most a few dozen particles share one).  Meshes of more than
+// kFineMaxKeys keys keep the tile sort above.
+constexpr long kFineMaxKeys = 1L << 24;
+
+__global__ void k_fine_count(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, double Hi, int P, int M, int T, int nt,
+                             int *__restrict__ key, int *__restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int bx = axis_base(x[i], Hi, P, M), by = axis_base(y[i], Hi, P, M), bz = axis_base(z[i], Hi, P, M);
+    const int tile = ((bx / T) * nt + by / T) * nt + bz / T;
+    const int k = tile * T * T * T + ((bx % T) * T + by % T) * T + bz % T;
+    key[i] = k;
+    atomicAdd(&count[k], 1);
+}
+
+// tile starts and counts from the fine keys' prefix sums (start[ntiles] =
+// the total)
+__global__ void k_tile_ranges(const int *__restrict__ fstart, long ntiles, int T3, int *__restrict__ start,
+                              int *__restrict__ count) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles)
+        return;
+    const int s = fstart[t * T3];
+    start[t] = s;
+    if (t < ntiles)
+        count[t] = fstart[(t + 1) * T3] - s;
+}
+
 // chunks per tile: ceil(count / kChunk)
 __global__ void k_tile_chunks(const int *__restrict__ count, long ntiles, int *__restrict__ chunks) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,6 +464,10 @@ struct p3mk_psort {
     int *key, *count, *start, *cursor, *perm, *chunks, *cstart, *scan_scratch;
     int2 *imap; // work item -> (tile, chunk)
     double4 *sorted;
+    // sort by base: ntiles T^3 keys (0: the tile sort), their counts,
+    // cursors and prefix sums
+    long nf;
+    int *fcount, *fcursor, *fstart, *fscan_scratch;
     int n; // particles of the last sort
 };
 
@@ -442,6 +483,9 @@ extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
     p3mk_release(s->imap);
     p3mk_release(s->scan_scratch);
     p3mk_release(s->sorted);
+    p3mk_release(s->fcount); // fcursor lives in this allocation
+    p3mk_release(s->fstart);
+    p3mk_release(s->fscan_scratch);
     delete s;
 }
 
@@ -470,6 +514,15 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
         rc = p3mk_alloc((void **)&s->scan_scratch, sizeof(int) * p3mk_scan_scratch_ints(s->ntiles));
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->perm, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->sorted, sizeof(double4) * nn);
+    const long T3 = (long)s->T * s->T * s->T;
+    s->nf = s->ntiles * T3 <= kFineMaxKeys ? s->ntiles * T3 : 0;
+    if (s->nf > 0) {
+        const size_t nf1 = (size_t)s->nf + 1;
+        if (rc == P3MK_OK) rc = alloc_count_cursor(&s->fcount, &s->fcursor, nf1);
+        if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->fstart, sizeof(int) * nf1);
+        if (rc == P3MK_OK)
+            rc = p3mk_alloc((void **)&s->fscan_scratch, sizeof(int) * p3mk_scan_scratch_ints(s->nf));
+    }
     if (rc != P3MK_OK) {
         p3mk_psort_destroy(s);
         return rc;
@@ -693,8 +746,21 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
     const double Hi = (double)g->M / g->L;
     const int nb = (n + 255) / 256;
     const size_t nt1 = (size_t)s->ntiles + 1;
-    P3MK_TRY(zero_count_cursor(s->count, s->cursor, nt1, qu));
-    if (s->ntiles <= kSortMaxTiles) {
+    if (s->nf > 0) {
+        const int T3 = s->T * s->T * s->T;
+        P3MK_TRY(zero_count_cursor(s->fcount, s->fcursor, (size_t)s->nf + 1, qu));
+        hipLaunchKernelGGL(k_fine_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
+                           s->nt, s->key, s->fcount);
+        P3MK_TRY(p3mk_launch_check("base count"));
+        P3MK_TRY(p3mk_exclusive_scan(s->fcount, s->nf, s->fstart, s->fscan_scratch, qu));
+        hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key,
+                           s->fstart, s->fcursor, s->perm, s->sorted);
+        P3MK_TRY(p3mk_launch_check("base scatter"));
+        hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((s->ntiles + 256) / 256)), dim3(256), 0, st,
+                           s->fstart, s->ntiles, T3, s->start, s->count);
+        P3MK_TRY(p3mk_launch_check("tile ranges"));
+    } else if (s->ntiles <= kSortMaxTiles) {
+        P3MK_TRY(zero_count_cursor(s->count, s->cursor, nt1, qu));
         // about 64 blocks: one global atomic per (block, tile) instead of per particle
         const int per_block = std::max(4096, (n + 63) / 64);
         const int blocks = (n + per_block - 1) / per_block;
@@ -709,6 +775,7 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
                            s->key, s->start, s->cursor, s->perm, s->sorted);
         P3MK_TRY(p3mk_launch_check("tile scatter"));
     } else {
+        P3MK_TRY(zero_count_cursor(s->count, s->cursor, nt1, qu));
         hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M,
                            s->T, s->nt, s->key, s->count);
         P3MK_TRY(p3mk_launch_check("tile count"));

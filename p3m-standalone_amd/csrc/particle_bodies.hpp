@@ -44,8 +44,11 @@ enum { kCharge = 0, kWeighted = 1, kDipole = 2 };
 // particle l / P^2 of the current group (64 / P^2 particles per step) and
 // adds its P points along z.  The lanes of one particle hit distinct
 // addresses, and the region's strides (z 1, y Sy odd, x Sx = P Sy mod 64)
-// spread the P^2 rows of a step over the LDS banks.  The products are the
-// per-particle form's ((q w0) w1) w2.  Charge (MODE kCharge) and
+// spread the P^2 rows of a step over the LDS banks.  Consecutive particles
+// with the same base (a tile's particles are ordered by base) are summed in
+// the lanes' registers, in the accumulator's precision, and added to LDS
+// once per run.  The products are the per-particle form's ((q w0) w1) w2.
+// Charge (MODE kCharge) and
 // charge-times-factor (kWeighted) deposits; the dipole deposit of the ad
 // adjoint keeps assign_item.
 
@@ -170,6 +173,13 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
                 woff[lane] = o0 * rg.Sx + o1 * rg.Sy + o2;
             }
             __syncthreads();
+            // a lane's row sums over consecutive particles of one base (the
+            // tile order puts them together) stay in registers until the base
+            // changes
+            A run[P];
+#pragma unroll
+            for (int a2 = 0; a2 < P; a2++)
+                run[a2] = A(0);
             for (int j = 0; j < nb; j += G) {
                 const int jj = j + grp;
                 if (row_lane && jj < nb) {
@@ -177,10 +187,18 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
                     T wz[W2];
                     load_row<T, W2>(rec, wz);
                     const T tr = rec[W2 + ra0] * rec[W2 + P + ra1];
-                    A *a = acc + woff[jj] + ra0 * rg.Sx + ra1 * rg.Sy;
 #pragma unroll
                     for (int a2 = 0; a2 < P; a2++)
-                        atomicAdd(&a[a2], (A)(tr * wz[a2]));
+                        run[a2] += (A)(tr * wz[a2]);
+                    const int off = woff[jj];
+                    if (jj + G >= nb || woff[jj + G] != off) {
+                        A *a = acc + off + ra0 * rg.Sx + ra1 * rg.Sy;
+#pragma unroll
+                        for (int a2 = 0; a2 < P; a2++) {
+                            atomicAdd(&a[a2], run[a2]);
+                            run[a2] = A(0);
+                        }
+                    }
                 }
             }
             __syncthreads();
