@@ -45,7 +45,9 @@ struct p3mk_batch {
     int *psys = nullptr, *p0s = nullptr, *cell0s = nullptr;
     long *tile0s = nullptr, *blk_z = nullptr, *blk_y = nullptr, *blk_x = nullptr;
     long nblk_z = 0, nblk_y = 0, nblk_x = 0;
-    size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_assign = 0;
+    // LDS of the FFT passes, of the interpolation (a tile's halo) and of
+    // the row-parallel assignment (region + staging)
+    size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_assign = 0, lds_rows = 0;
     long ntiles = 0, max_items = 0, ncell = 0;
     int tile_side = 8; // requested assignment tile side (autotuned)
     int rs_grid = 0;   // persistent real-space grid (autotuned)
@@ -85,7 +87,7 @@ int build_descriptors(p3mk_batch *b) {
     const int ns = b->nsys;
     std::vector<long> tile0s(ns + 1), blk_z(ns + 1), blk_y(ns + 1), blk_x(ns + 1);
     long tiles = 0, items = 0, bz = 0, by = 0, bx = 0;
-    size_t lds_a = 0, lz = 0, ly = 0, lx = 0;
+    size_t lds_a = 0, lds_r = 0, lz = 0, ly = 0, lx = 0;
     const int NM = b->form.interlaced ? 2 : 1;
     for (int s = 0; s < ns; s++) {
         BSys &S = b->hs[s];
@@ -97,6 +99,10 @@ int build_descriptors(p3mk_batch *b) {
         items += (S.n + part::kChunk - 1) / part::kChunk + (long)S.nt * S.nt * S.nt;
         const int R = S.T + S.cao - 1 + (b->form.interlaced ? 1 : 0);
         lds_a = std::max(lds_a, sizeof(double) * NM * (size_t)R * R * R);
+        const part::RowRegion rg = part::row_region(S.T, S.cao, b->form.interlaced != 0);
+        lds_r = std::max(lds_r, sizeof(double) * (size_t)rg.size +
+                                    (size_t)(part::kThreads / 64) *
+                                        part::row_stage_bytes_n(S.cao, b->form.fp32 ? 4 : 8));
         const int M = S.M, W = b->form.r2c ? M / 2 + 1 : M;
         const int l2 = bfft_lines_per_tile(b->form, M, kPassZ);
         const int lxx = bfft_lines_per_tile(b->form, M, kPassX);
@@ -123,7 +129,8 @@ int build_descriptors(p3mk_batch *b) {
     b->lds_y = ly;
     b->lds_x = lx;
     b->lds_assign = lds_a;
-    if (lds_a > 64 * 1024)
+    b->lds_rows = lds_r;
+    if (lds_a > 64 * 1024 || lds_r > 64 * 1024)
         return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
     // (re)upload the descriptor arrays
     p3mk_release(b->dsys);
@@ -392,7 +399,7 @@ extern "C" int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y
     P3MK_TRY(bpart_sort(f, b->dsys, b->psys, b->tile0s, ns, b->ntiles, b->ntot, x, y, z, q, b->key,
                         b->count, b->start, b->cursor, b->perm, b->chunks, b->cstart, b->imap,
                         b->scan, b->sorted, qu));
-    P3MK_TRY(bpart_assign(f, b->dsys, b->ntiles, b->max_items, b->lds_assign, b->start, b->cstart,
+    P3MK_TRY(bpart_assign(f, b->dsys, b->ntiles, b->max_items, b->lds_rows, b->start, b->cstart,
                           b->imap, b->sorted, qu));
     // forward transform along z and y
     int spec = kBufMesh;

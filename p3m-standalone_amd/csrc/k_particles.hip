@@ -967,12 +967,17 @@ k_bassign(const BSys *__restrict__ sys, long ntiles, const int *__restrict__ sta
     T *mesh = (T *)S.buf[kBufMesh];
     double *acc = reinterpret_cast<double *>(smem_raw);
     const long tl = tile - S.tile0;
+    // the row-parallel deposit of the single-system form: the system's
+    // region, then the wavefronts' staging areas
     switch (S.cao) {
 #define CASE(P)                                                                                    \
-    case P:                                                                                        \
-        assign_item<P, T, R2C, IL, kCharge>(sorted, pb, pe, tl, S.nt, S.T, S.M, S.Hi, mesh,         \
-                                            nullptr, nullptr, 0, acc);                            \
-        break;
+    case P: {                                                                                      \
+        const RowRegion rg = row_region(S.T, P, IL);                                               \
+        T *stage = reinterpret_cast<T *>(smem_raw + sizeof(double) * (size_t)rg.size);             \
+        assign_rows_item<P, T, R2C, IL, kCharge, double>(sorted, pb, pe, tl, S.nt, S.T, S.M, S.Hi, \
+                                                         mesh, nullptr, nullptr, acc, stage);      \
+        break;                                                                                     \
+    }
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
     default: break;

@@ -74,15 +74,18 @@ __host__ __device__ inline RowRegion row_region(int Tt, int P, bool il) {
 // staged record of one particle: z weights w2 in a block of row_w2<P>()
 // values (read by 16-byte loads), then q w0[P] and w1[P]; records are
 // 16-byte aligned
-template <int P> __host__ __device__ constexpr int row_w2() { return (P + 3) & ~3; }
-template <int P> __host__ __device__ constexpr int row_record() {
-    return (row_w2<P>() + 2 * P + 3) & ~3;
-}
+__host__ __device__ constexpr int row_w2_n(int P) { return (P + 3) & ~3; }
+__host__ __device__ constexpr int row_record_n(int P) { return (row_w2_n(P) + 2 * P + 3) & ~3; }
+template <int P> __host__ __device__ constexpr int row_w2() { return row_w2_n(P); }
+template <int P> __host__ __device__ constexpr int row_record() { return row_record_n(P); }
 
 // LDS bytes of one wavefront's staging area: kRowBatch records of T and the
 // particles' region offsets
+__host__ __device__ constexpr int row_stage_bytes_n(int P, int tsize) {
+    return kRowBatch * (row_record_n(P) * tsize + (int)sizeof(int));
+}
 template <int P, typename T> __host__ __device__ constexpr int row_stage_bytes() {
-    return kRowBatch * (row_record<P>() * (int)sizeof(T) + (int)sizeof(int));
+    return row_stage_bytes_n(P, (int)sizeof(T));
 }
 
 // n values of a 16-byte-aligned LDS row by 16-byte loads
