@@ -355,6 +355,19 @@ def batch_systems(nsys=256, seed=4, rcut=3.0, eps=1e-4, density=0.5):
     return systems, data
 
 
+def batch_pair_count(systems, data):
+    """Ordered pairs within rcut (minimum image) summed over the systems:
+    exact, by a periodic k-d tree per system."""
+    from scipy.spatial import cKDTree
+
+    total = 0
+    for s, d in zip(systems, data):
+        P = np.stack([d["x"], d["y"], d["z"]], 1) % s["L"]
+        tree = cKDTree(P, boxsize=s["L"])
+        total += int(tree.count_neighbors(tree, s["rcut"])) - len(P)
+    return total
+
+
 def bench_batch(pkg, torch, args):
     """One step = Calculate_forces of all 256 systems (real + k space), as
     one replay of the captured graph of batched launches."""
@@ -382,13 +395,32 @@ def bench_batch(pkg, torch, args):
         batch.synchronize()
         torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        forces = f.cpu().numpy()
+        # the batched half-shell kernel alone (HIP events around its launch,
+        # eager evaluations after the timed replays)
+        pair_ms = batch.time_pairs(*ptr, f.data_ptr(), fk.data_ptr(), fr.data_ptr(), reps=5)
         res[name] = {"ms_per_step": round(ms, 4), "value": round(ntot / (ms * 1e-3), 1),
-                     "tile_side": tile, "rs_grid": grid, "forces": f.cpu().numpy()}
+                     "tile_side": tile, "rs_grid": grid, "pair_kernel_ms": round(pair_ms, 4),
+                     "forces": forces}
         batch.close()
     rel = float(np.linalg.norm(res["mixed"]["forces"] - res["fp64"]["forces"]) /
                 np.linalg.norm(res["fp64"]["forces"]))
     for r in res.values():
         r.pop("forces")
+    pairs = batch_pair_count(systems, data)
+    fl = 0.5 * pairs * HALF_PAIR_FLOPS
+    tf = fl / (res["mixed"]["pair_kernel_ms"] * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_brealspace_half<true> (all systems' half-shell pairs, FP32)",
+            "achieved": round(tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+            "traffic": measured_traffic(BATCH_WORKLOAD, "k_brealspace_half<true>"),
+            "peak_unpacked_fp32": FP32_UNPACKED_TFLOPS,
+            "frac_unpacked_fp32": round(tf / FP32_UNPACKED_TFLOPS, 4),
+            "note": "the largest kernel of the batch step; achieved = %d ordered pairs within rcut "
+                    "over all systems (exact, periodic k-d trees) / 2 x %d flops / mean time of the "
+                    "kernel (HIP events around its launch, %s ms, eager calls); traffic = HBM bytes "
+                    "per launch from profiles/traffic_%s.md"
+                    % (pairs, HALF_PAIR_FLOPS, res["mixed"]["pair_kernel_ms"], BATCH_WORKLOAD)}
     return {
         "metric": "particle-force evals/sec at rms err <=1e-4 (mixed vs FP64 GPU), HBM GB/s",
         "value": res["mixed"]["value"], "unit": "particle-force evals/s", "n_gpus": 1,
@@ -400,7 +432,7 @@ def bench_batch(pkg, torch, args):
                    "N_total": ntot, "N_range": [500, 20000], "mesh": "even 11-smooth 16..48",
                    "cao": "3..7", "rcut": 3.0, "alpha": "tuning.c:293 at 1e-4",
                    "parallelism": "single GPU, one launch per stage over all systems, HIP graph"},
-        "roofline": None,
+        "roofline": roof,
         "fp64": res["fp64"], "mixed": res["mixed"],
         "accuracy": {"rel_mixed_vs_fp64": rel},
     }

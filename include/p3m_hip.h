@@ -408,6 +408,13 @@ int p3m_hip_batch_synchronize(p3m_hip_batch *b);
 int p3m_hip_batch_autotune(p3m_hip_batch *b, const double *x, const double *y, const double *z,
                            const double *q, double *f, double *fk, double *fr, int *tile_side,
                            int *rs_grid);
+/* Mean time (ms, HIP events on the batch stream) of the batched half-shell
+ * real-space kernel alone over `reps` eager force evaluations of the given
+ * input (the forces are computed as by the captured step).  No counterpart
+ * in the reference. */
+int p3m_hip_batch_time_pairs(p3m_hip_batch *b, const double *x, const double *y, const double *z,
+                             const double *q, double *f, double *fk, double *fr, int reps,
+                             double *ms);
 /* alpha for a real-space rms force error `precision` at cutoff rcut, the
  * formula of the reference's tuner (tuning.c:293). */
 double p3m_hip_tune_alpha(int n, double length, double q2, double rcut, double precision);

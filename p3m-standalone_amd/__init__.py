@@ -158,6 +158,7 @@ def lib():
             "p3m_hip_batch_replay": (i, [vp]),
             "p3m_hip_batch_synchronize": (i, [vp]),
             "p3m_hip_batch_autotune": (i, [vp] + [vp] * 7 + [ip, ip]),
+            "p3m_hip_batch_time_pairs": (i, [vp] + [vp] * 7 + [i, _dp]),
             "p3m_hip_tune_alpha": (d, [i, d, d, d, d]),
             "p3m_hip_tune_scan": (i, [i, _SP, _PP, d, i, TuneTimerFn, vp, ctypes.POINTER(RuntimeStat)]),
             "p3m_hip_tune": (RuntimeStat, [ctypes.POINTER(Method), _SP, _PP, d]),
@@ -502,6 +503,14 @@ class Batch:
         check(lib().p3m_hip_batch_autotune(self.b, x, y, z, q, f, fk, fr, ctypes.byref(t),
                                            ctypes.byref(g)))
         return t.value, g.value
+
+    def time_pairs(self, x, y, z, q, f, fk, fr, reps=5):
+        """Mean ms of the batched half-shell real-space kernel alone over
+        `reps` eager force evaluations."""
+        ms = ctypes.c_double()
+        check(lib().p3m_hip_batch_time_pairs(self.b, x, y, z, q, f, fk, fr, int(reps),
+                                             ctypes.byref(ms)))
+        return ms.value
 
 
 def tune_alpha(n, length, q2, rcut, precision):

@@ -153,6 +153,6 @@ void cells_release(CellArrays &a);
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
                const double *q, CellArrays &a, int grid, bool any_small, bool any_cells,
-               double *fr, double *energy, p3mk_queue *qu);
+               bool time_pairs, double *fr, double *energy, p3mk_queue *qu);
 
 }  // namespace p3m

@@ -269,6 +269,33 @@ static int time_forces(p3m_hip_batch *b, const double *x, const double *y, const
     return P3M_HIP_OK;
 }
 
+int p3m_hip_batch_time_pairs(p3m_hip_batch *b, const double *x, const double *y, const double *z,
+                             const double *q, double *f, double *fk, double *fr, int reps,
+                             double *ms) {
+    float t = 0.0f;
+    double sum = 0.0;
+    int k, rc = P3M_HIP_OK;
+    if (b == NULL || x == NULL || y == NULL || z == NULL || q == NULL || f == NULL || fk == NULL ||
+        fr == NULL || ms == NULL || reps < 1)
+        return p3mh_fail(P3M_HIP_EINVAL, "bad arguments");
+    TRYK(p3mk_queue_sync(b->q), "sync");
+    TRYK(p3mk_batch_time_pairs(b->kb, 1), "events");
+    for (k = 0; k < reps && rc == P3M_HIP_OK; k++) {
+        int kr;
+        rc = batch_forces_enqueue(b, x, y, z, q, f, fk, fr, NULL);
+        if (rc == P3M_HIP_OK && (kr = p3mk_queue_sync(b->q)) != P3MK_OK)
+            rc = p3mh_kfail(kr, "sync");
+        if (rc == P3M_HIP_OK &&
+            (kr = p3mk_elapsed_ms(b->q, P3MK_EV_PAIRS, P3MK_EV_PAIRS + 1, &t)) != P3MK_OK)
+            rc = p3mh_kfail(kr, "event timing");
+        sum += t;
+    }
+    (void)p3mk_batch_time_pairs(b->kb, 0);
+    if (rc == P3M_HIP_OK)
+        *ms = sum / reps;
+    return rc;
+}
+
 /* Autotuner of the launch shapes: the assignment tile side (4 or 8 mesh
  * points: smaller tiles give more, lighter work items for small meshes and
  * dense clusters) and the size of the persistent real-space grid (waves per

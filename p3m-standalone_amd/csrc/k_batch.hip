@@ -51,6 +51,7 @@ struct p3mk_batch {
     long ntiles = 0, max_items = 0, ncell = 0;
     int tile_side = 8; // requested assignment tile side (autotuned)
     int rs_grid = 0;   // persistent real-space grid (autotuned)
+    bool time_pairs = false; // events around the half-shell kernel (eager calls only)
     bool rs_small = false, rs_cells = false; // systems with nearest-image / cell-grid real space
     int cus = 256;
     // arenas
@@ -440,5 +441,10 @@ extern "C" int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double
                                     const double *z, const double *q, double *fr, double *energy,
                                     p3mk_queue *qu) {
     return brs_forces(b->form, b->dsys, b->psys, b->cell0s, b->nsys, b->ncell, b->ntot, x, y, z, q,
-                      b->cells, b->rs_grid, b->rs_small, b->rs_cells, fr, energy, qu);
+                      b->cells, b->rs_grid, b->rs_small, b->rs_cells, b->time_pairs, fr, energy, qu);
+}
+
+extern "C" int p3mk_batch_time_pairs(p3mk_batch *b, int on) {
+    b->time_pairs = on != 0;
+    return P3MK_OK;
 }

@@ -440,6 +440,29 @@ __device__ __forceinline__ void load_erfcx_table(RsShared<FP32, VJP> &sh) {
     __syncthreads();
 }
 
+// Work-item claims of the persistent grids: a claim takes `per` consecutive
+// items with one device atomic on the shared counter, which is a serial point
+// (190k single-item claims of sparse cells took most of a 2.2 ms batched
+// launch).  per = total / (8 grid), at least 1, leaves a wavefront about
+// eight claims, for the balance of clustered systems.
+__device__ __forceinline__ int claim_size(int total) { return max(1, total / ((int)gridDim.x * 8)); }
+
+// this wavefront's next item (next, end: its current claim), total when the
+// items are exhausted
+__device__ __forceinline__ int next_item(int *counter, int total, int per, int &next, int &end) {
+    if (next >= end) {
+        int first = 0;
+        if (threadIdx.x == 0)
+            first = atomicAdd(counter, per);
+        first = __builtin_amdgcn_readfirstlane(first);
+        if (first >= total)
+            return total;
+        next = first;
+        end = min(total, first + per);
+    }
+    return next++;
+}
+
 // index of the periodic image shift (wx, wy, wz) in {-1, 0, 1}^3
 __device__ __forceinline__ unsigned int shift_index(int wx, int wy, int wz) {
     return (unsigned)((wx + 1) * 9 + (wy + 1) * 3 + (wz + 1));
@@ -734,11 +757,10 @@ k_realspace(int n, const double4 *__restrict__ sorted, const float4 *__restrict_
     const int total = istart[ncell];
     double en_tot = 0.0;
     load_erfcx_table(sh);
+    const int per = claim_size(total);
+    int next = 0, end = 0;
     for (;;) {
-        int item = 0;
-        if (threadIdx.x == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(counter, total, per, next, end);
         if (item >= total)
             break;
         const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
@@ -769,11 +791,10 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
     const int total = istart[ncell];
     double en_unused = 0.0;
     load_erfcx_table(sh);
+    const int per = claim_size(total);
+    int next = 0, end = 0;
     for (;;) {
-        int item = 0;
-        if (threadIdx.x == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(counter, total, per, next, end);
         if (item >= total)
             break;
         const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
@@ -1308,11 +1329,10 @@ k_realspace_half(const double4 *__restrict__ sorted, const float4 *__restrict__ 
             sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
     }
     __syncthreads();
+    const int per = claim_size(total);
+    int next = 0, end = 0;
     for (;;) {
-        int item = 0;
-        if (threadIdx.x == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(counter, total, per, next, end);
         if (item >= total)
             break;
         // the item's cell and chunk, stored with its box
@@ -1611,11 +1631,10 @@ k_brealspace(const BSys *__restrict__ sys, const int *__restrict__ cell0s, int n
     __shared__ RsShared<FP32> sh;
     const int total = istart[ncell];
     load_erfcx_table(sh);
+    const int per = claim_size(total);
+    int next = 0, end = 0;
     for (;;) {
-        int item = 0;
-        if (threadIdx.x == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(counter, total, per, next, end);
         if (item >= total)
             break;
         const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
@@ -1656,11 +1675,10 @@ k_brealspace_half(const BSys *__restrict__ sys, const int *__restrict__ cell0s, 
             sh.tab[k] = (&p3m::kErfcxDoubleCoef[0][0])[k];
     }
     __syncthreads();
+    const int per = claim_size(total);
+    int next = 0, end = 0;
     for (;;) {
-        int item = 0;
-        if (threadIdx.x == 0)
-            item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(counter, total, per, next, end);
         if (item >= total)
             break;
         const int home = __float_as_int(bbox[2 * item].w), chunk = __float_as_int(bbox[2 * item + 1].w);
@@ -1687,7 +1705,7 @@ namespace p3m {
 int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell0s, int nsys,
                long ncell, int ntot, const double *x, const double *y, const double *z,
                const double *q, CellArrays &a, int grid, bool any_small, bool any_cells,
-               double *fr, double *energy, p3mk_queue *qu) {
+               bool time_pairs, double *fr, double *energy, p3mk_queue *qu) {
     if (ntot <= 0)
         return P3MK_OK;
     hipStream_t s = p3mk_stream(qu);
@@ -1717,6 +1735,10 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
     if (any_cells) {
         // cell-grid systems: half shell in sorted order, then added by perm
         P3MK_TRY(p3mk_fill_zero(a.fsorted, sizeof(double) * 3 * (size_t)ntot, qu));
+        // the half-shell kernel alone between two events, on request (not
+        // in captured graphs)
+        if (time_pairs)
+            P3MK_TRY(p3mk_record(qu, P3MK_EV_PAIRS));
         if (f.fp32)
             hipLaunchKernelGGL(k_brealspace_half<true>, dim3((unsigned)grid), dim3(kWave), 0, s, sys,
                                cell0s, nsys, ntot, a.sorted, a.sorted32, a.sortedq, a.start, a.istart, a.bbox, ncell,
@@ -1726,6 +1748,8 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
                                cell0s, nsys, ntot, a.sorted, a.sorted32, a.sortedq, a.start, a.istart, a.bbox, ncell,
                                a.counter + 1, a.fsorted, energy);
         P3MK_TRY(p3mk_launch_check("batched real space (half shell)"));
+        if (time_pairs)
+            P3MK_TRY(p3mk_record(qu, P3MK_EV_PAIRS + 1));
         hipLaunchKernelGGL(k_unsort_add, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, ntot,
                            a.key, a.fsorted, (long)ntot, fr);
         P3MK_TRY(p3mk_launch_check("batched real-space forces to caller order"));

@@ -217,6 +217,9 @@ int p3mk_batch_kspace(p3mk_batch *b, const double *x, const double *y, const dou
 /* fr (3 blocks of ntot) += real-space force; energy[s] += system energy */
 int p3mk_batch_realspace(p3mk_batch *b, const double *x, const double *y, const double *z,
                          const double *q, double *fr, double *energy, p3mk_queue *q_);
+/* record events P3MK_EV_PAIRS, +1 around the batched half-shell kernel in
+ * the following calls (on != 0); not for calls captured into a graph */
+int p3mk_batch_time_pairs(p3mk_batch *b, int on);
 
 /* ---- Ewald sum at sampled particles --------------------------------------- */
 /* f_total (3 ns) += Ewald force (k space: lattice vectors n^2 <= kmax^2 at
