@@ -846,6 +846,7 @@ template <bool FP32> struct RsHalfShared {
     halfx4 win_a[kWave];         // (-2 n, 1) in FP16, n = n' / side, n' relative to the home cell centre
     float win_n2[kWave];         // |n'|^2; 1e30 in empty slots
     unsigned int win_js[kWave];  // sorted index | shift index << 27
+    int4 cells[kWave];           // the half shell's neighbour cells: range, first item, offsets
     // FP32 form: (n', q as FP32 bits), n' the fixed-point position of the
     // window particle's image relative to the home cell centre
     int4 win_q[FP32 ? kWave : 1];
@@ -1208,16 +1209,19 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
         cinfo = (unsigned)((a + 2) | (b + 2) << 3 | (c + 2) << 6 | (wx + 1) << 9 | (wy + 1) << 11 |
                            (wz + 1) << 13);
     }
+    // kept in LDS for the walk (read back uniformly), not in registers
+    sh.cells[lane] = make_int4(cjb, cje, citem, (int)cinfo);
     unsigned long long cells = __ballot(live);
     while (cells) {
         const int k = __builtin_ctzll(cells);
         cells &= cells - 1;
         {
             {
-                const int jb = __builtin_amdgcn_readlane(cjb, k);
-                const int je = __builtin_amdgcn_readlane(cje, k);
-                const int item0 = __builtin_amdgcn_readlane(citem, k);
-                const unsigned int info = (unsigned)__builtin_amdgcn_readlane((int)cinfo, k);
+                const int4 ck = sh.cells[k];
+                const int jb = __builtin_amdgcn_readfirstlane(ck.x);
+                const int je = __builtin_amdgcn_readfirstlane(ck.y);
+                const int item0 = __builtin_amdgcn_readfirstlane(ck.z);
+                const unsigned int info = (unsigned)__builtin_amdgcn_readfirstlane(ck.w);
                 const int a = (int)(info & 7u) - 2, b = (int)((info >> 3) & 7u) - 2,
                           c = (int)((info >> 6) & 7u) - 2;
                 const int wx = (int)((info >> 9) & 3u) - 1, wy = (int)((info >> 11) & 3u) - 1,

@@ -440,3 +440,31 @@ def test_fft_r2c_zy_vs_numpy(pkg, M, fp32):
         assert eb < tol
     finally:
         rig.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,fp32", [(64, 1), (128, 1), (128, 0)])
+def test_fft_c2c_zy_vs_numpy(pkg, M, fp32):
+    """The z and y passes of the c2c forms in place (at M = 128 in FP32 both
+    in one launch, a y-z plane per workgroup in LDS) and their inverse,
+    against numpy: fft(fft(a, z), y), and back M^2 a."""
+    fn = _fft_api(pkg)
+    cdt = np.complex64 if fp32 else np.complex128
+    rng = np.random.default_rng(300 + M + fp32)
+    a = (rng.standard_normal((M, M, M)) + 1j * rng.standard_normal((M, M, M))).astype(cdt)
+    rig = _FftRig(fn, M, 0, fp32, [a.nbytes])
+    tol = 2e-6 if fp32 else 1e-14
+    try:
+        rig.put(0, a)
+        assert fn["p3mk_fft_forward_zy"](rig.plan, rig.bufs[0], rig.bufs[0], rig.q) == 0
+        spec = rig.get(0, np.empty_like(a))
+        a64 = a.astype(np.complex128)
+        ef = rel_err(spec.astype(np.complex128), np.fft.fft(np.fft.fft(a64, axis=2), axis=1))
+        assert fn["p3mk_fft_backward_yz"](rig.plan, rig.bufs[0], rig.bufs[0], rig.q) == 0
+        back = rig.get(0, np.empty_like(a))
+        eb = rel_err(back.astype(np.complex128), M * M * a64)
+        print("M %d fp32 %d: forward %.2e, round trip %.2e" % (M, fp32, ef, eb))
+        assert ef < tol
+        assert eb < tol
+    finally:
+        rig.close()
