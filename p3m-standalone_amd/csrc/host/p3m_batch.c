@@ -298,14 +298,17 @@ int p3m_hip_batch_time_pairs(p3m_hip_batch *b, const double *x, const double *y,
 
 /* Autotuner of the launch shapes: the assignment tile side (4 or 8 mesh
  * points: smaller tiles give more, lighter work items for small meshes and
- * dense clusters) and the size of the persistent real-space grid (waves per
- * CU), each chosen by timing whole force evaluations of the given input. */
+ * dense clusters) and the size of the persistent real-space grid (8 to 64
+ * waves per CU), every pair timed over whole force evaluations of the given
+ * input (12 shapes). */
 int p3m_hip_batch_autotune(p3m_hip_batch *b, const double *x, const double *y, const double *z,
                            const double *q, double *f, double *fk, double *fr, int *tile_side,
                            int *rs_grid) {
     static const int tiles[] = {8, 4};
-    static const int waves_per_cu[] = {32, 16, 64};
-    int best_tile = 8, best_grid = 0, cus_grid = 0, k, fp32, sf;
+    static const int waves_per_cu[] = {8, 16, 24, 32, 48, 64};
+    const int ntiles = (int)(sizeof(tiles) / sizeof(tiles[0]));
+    const int ngrids = (int)(sizeof(waves_per_cu) / sizeof(waves_per_cu[0]));
+    int best_tile = 8, best_grid = 0, cus_grid = 0, k, j, fp32, sf;
     double best = 1e300, ms;
     if (b == NULL || x == NULL || y == NULL || z == NULL || q == NULL || f == NULL || fk == NULL ||
         fr == NULL)
@@ -318,24 +321,19 @@ int p3m_hip_batch_autotune(p3m_hip_batch *b, const double *x, const double *y, c
     TRYK(p3mk_batch_info(b->kb, NULL, NULL, NULL, NULL, &cus_grid), "batch info");
     cus_grid /= 32; /* the default grid is 32 waves per CU */
     best_grid = 32 * cus_grid;
-    for (k = 0; k < 2; k++) {
-        TRYK(p3mk_queue_sync(b->q), "sync");
-        TRYK(p3mk_batch_configure(b->kb, fp32, sf, tiles[k], best_grid), "configure");
-        TRY(time_forces(b, x, y, z, q, f, fk, fr, 3, &ms));
-        if (ms < best) {
-            best = ms;
-            best_tile = tiles[k];
-        }
-    }
-    best = 1e300;
-    for (k = 0; k < 3; k++) {
-        const int grid = waves_per_cu[k] * cus_grid;
-        TRYK(p3mk_queue_sync(b->q), "sync");
-        TRYK(p3mk_batch_configure(b->kb, fp32, sf, best_tile, grid), "configure");
-        TRY(time_forces(b, x, y, z, q, f, fk, fr, 3, &ms));
-        if (ms < best) {
-            best = ms;
-            best_grid = grid;
+    /* every (tile side, grid) pair: the two interact through the claims,
+     * whose size follows the grid */
+    for (k = 0; k < ntiles; k++) {
+        for (j = 0; j < ngrids; j++) {
+            const int grid = waves_per_cu[j] * cus_grid;
+            TRYK(p3mk_queue_sync(b->q), "sync");
+            TRYK(p3mk_batch_configure(b->kb, fp32, sf, tiles[k], grid), "configure");
+            TRY(time_forces(b, x, y, z, q, f, fk, fr, 3, &ms));
+            if (ms < best) {
+                best = ms;
+                best_tile = tiles[k];
+                best_grid = grid;
+            }
         }
     }
     TRYK(p3mk_queue_sync(b->q), "sync");

@@ -143,9 +143,11 @@ k_tile_scatter_agg(int n, int per_block, const double *__restrict__ x, const dou
 // kFineMaxKeys keys keep the tile sort above.
 constexpr long kFineMaxKeys = 1L << 24;
 
+// the particle's key and its rank among the particles of that key (the
+// value the count's atomic returns), so that the scatter needs no atomics
 __global__ void k_fine_count(int n, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, double Hi, int P, int M, int T, int nt,
-                             int *__restrict__ key, int *__restrict__ count) {
+                             int *__restrict__ key, int *__restrict__ rank, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -153,20 +155,36 @@ __global__ void k_fine_count(int n, const double *__restrict__ x, const double *
     const int tile = ((bx / T) * nt + by / T) * nt + bz / T;
     const int k = tile * T * T * T + ((bx % T) * T + by % T) * T + bz % T;
     key[i] = k;
-    atomicAdd(&count[k], 1);
+    rank[i] = atomicAdd(&count[k], 1);
 }
 
-// tile starts and counts from the fine keys' prefix sums (start[ntiles] =
-// the total)
+__global__ void k_fine_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ z, const double *__restrict__ q,
+                               const int *__restrict__ key, const int *__restrict__ rank,
+                               const int *__restrict__ start, int *__restrict__ perm,
+                               double4 *__restrict__ sorted) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int p = start[key[i]] + rank[i];
+    perm[p] = i;
+    sorted[p] = make_double4(x[i], y[i], z[i], q[i]);
+}
+
+// tile starts, counts and chunk counts from the fine keys' prefix sums
+// (start[ntiles] = the total)
 __global__ void k_tile_ranges(const int *__restrict__ fstart, long ntiles, int T3, int *__restrict__ start,
-                              int *__restrict__ count) {
+                              int *__restrict__ count, int *__restrict__ chunks) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntiles)
         return;
     const int s = fstart[t * T3];
     start[t] = s;
-    if (t < ntiles)
-        count[t] = fstart[(t + 1) * T3] - s;
+    if (t < ntiles) {
+        const int c = fstart[(t + 1) * T3] - s;
+        count[t] = c;
+        chunks[t] = (c + kChunk - 1) / kChunk;
+    }
 }
 
 // chunks per tile: ceil(count / kChunk)
@@ -464,10 +482,11 @@ struct p3mk_psort {
     int *key, *count, *start, *cursor, *perm, *chunks, *cstart, *scan_scratch;
     int2 *imap; // work item -> (tile, chunk)
     double4 *sorted;
-    // sort by base: ntiles T^3 keys (0: the tile sort), their counts,
-    // cursors and prefix sums
+    // sort by base: ntiles T^3 keys (0: the tile sort), their counts and
+    // prefix sums
     long nf;
-    int *fcount, *fcursor, *fstart, *fscan_scratch;
+    int *fcount, *fstart, *fscan_scratch;
+    int *rank; // per particle: its rank within its key
     int n; // particles of the last sort
 };
 
@@ -483,9 +502,10 @@ extern "C" void p3mk_psort_destroy(p3mk_psort *s) {
     p3mk_release(s->imap);
     p3mk_release(s->scan_scratch);
     p3mk_release(s->sorted);
-    p3mk_release(s->fcount); // fcursor lives in this allocation
+    p3mk_release(s->fcount);
     p3mk_release(s->fstart);
     p3mk_release(s->fscan_scratch);
+    p3mk_release(s->rank);
     delete s;
 }
 
@@ -518,10 +538,11 @@ extern "C" int p3mk_psort_create(p3mk_psort **out, const p3mk_geom *g, int n) {
     s->nf = s->ntiles * T3 <= kFineMaxKeys ? s->ntiles * T3 : 0;
     if (s->nf > 0) {
         const size_t nf1 = (size_t)s->nf + 1;
-        if (rc == P3MK_OK) rc = alloc_count_cursor(&s->fcount, &s->fcursor, nf1);
+        if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->fcount, sizeof(int) * nf1);
         if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->fstart, sizeof(int) * nf1);
         if (rc == P3MK_OK)
             rc = p3mk_alloc((void **)&s->fscan_scratch, sizeof(int) * p3mk_scan_scratch_ints(s->nf));
+        if (rc == P3MK_OK) rc = p3mk_alloc((void **)&s->rank, sizeof(int) * nn);
     }
     if (rc != P3MK_OK) {
         p3mk_psort_destroy(s);
@@ -748,16 +769,16 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
     const size_t nt1 = (size_t)s->ntiles + 1;
     if (s->nf > 0) {
         const int T3 = s->T * s->T * s->T;
-        P3MK_TRY(zero_count_cursor(s->fcount, s->fcursor, (size_t)s->nf + 1, qu));
+        P3MK_TRY(p3mk_fill_zero(s->fcount, sizeof(int) * ((size_t)s->nf + 1), qu));
         hipLaunchKernelGGL(k_fine_count, dim3(nb), dim3(256), 0, st, n, x, y, z, Hi, g->cao, g->M, s->T,
-                           s->nt, s->key, s->fcount);
+                           s->nt, s->key, s->rank, s->fcount);
         P3MK_TRY(p3mk_launch_check("base count"));
         P3MK_TRY(p3mk_exclusive_scan(s->fcount, s->nf, s->fstart, s->fscan_scratch, qu));
-        hipLaunchKernelGGL(k_tile_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key,
-                           s->fstart, s->fcursor, s->perm, s->sorted);
+        hipLaunchKernelGGL(k_fine_scatter, dim3(nb), dim3(256), 0, st, n, x, y, z, q, s->key, s->rank,
+                           s->fstart, s->perm, s->sorted);
         P3MK_TRY(p3mk_launch_check("base scatter"));
         hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)((s->ntiles + 256) / 256)), dim3(256), 0, st,
-                           s->fstart, s->ntiles, T3, s->start, s->count);
+                           s->fstart, s->ntiles, T3, s->start, s->count, s->chunks);
         P3MK_TRY(p3mk_launch_check("tile ranges"));
     } else if (s->ntiles <= kSortMaxTiles) {
         P3MK_TRY(zero_count_cursor(s->count, s->cursor, nt1, qu));
@@ -784,9 +805,11 @@ extern "C" int p3mk_assign(const p3mk_geom *g, int n, const double *x, const dou
                            s->start, s->cursor, s->perm, s->sorted);
         P3MK_TRY(p3mk_launch_check("tile scatter"));
     }
-    hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
-                       s->count, s->ntiles, s->chunks);
-    P3MK_TRY(p3mk_launch_check("tile chunks"));
+    if (s->nf == 0) {
+        hipLaunchKernelGGL(k_tile_chunks, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
+                           s->count, s->ntiles, s->chunks);
+        P3MK_TRY(p3mk_launch_check("tile chunks"));
+    }
     P3MK_TRY(p3mk_exclusive_scan(s->chunks, s->ntiles, s->cstart, s->scan_scratch, qu));
     hipLaunchKernelGGL(k_item_map, dim3((unsigned)((s->ntiles + 255) / 256)), dim3(256), 0, st,
                        s->cstart, s->ntiles, s->imap);
