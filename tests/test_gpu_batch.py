@@ -185,3 +185,24 @@ def test_batch_without_real_space_and_alpha_change(pkg, orc):
         fk_ref, _ = orc.kspace_force(7, d, S["M"], S["cao"], S["alpha"])
         assert rel_err(per_system(fk, batch.offsets, s), fk_ref) < 1e-12
     batch.close()
+
+
+def test_batch_time_pairs(pkg):
+    """The pair-kernel timer of a batch returns a positive time, leaves the
+    forces as an ordinary evaluation computes them, and a graph captured
+    afterwards (no events recorded in it) replays to the same forces."""
+    import torch
+
+    systems, data = make_batch_systems(8, seed=93, nmin=500, nmax=2500)
+    batch = pkg.Batch(3, systems, pkg.MIXED)
+    dev = device_arrays(torch, data)
+    before = run_batch(torch, batch, dev)[0]
+    n = batch.ntot
+    out = [torch.zeros(3 * n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    ms = batch.time_pairs(*[dev[k].data_ptr() for k in "xyzq"], *[t.data_ptr() for t in out], reps=3)
+    print("batched pair kernel: %.4f ms" % ms)
+    assert 0.0 < ms < 1e3
+    assert rel_err(out[0].cpu().numpy().reshape(3, n), before) < 1e-12
+    after = run_batch(torch, batch, dev, graph=True)[0]
+    assert rel_err(after, before) < 1e-12
+    batch.close()

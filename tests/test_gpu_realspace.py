@@ -161,3 +161,31 @@ def test_pairs_at_the_cutoff(pkg, orc, m, reach):
     fr_ref, e_ref = orc.realteil(s, rcut, alpha)
     assert rel_err(forces.real(), fr_ref) < 1e-13
     assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+def test_stage_ring_pair_kernel_times(pkg):
+    """The per-step pair-kernel times of the stage ring: positive, within the
+    step's real-space stage (which also holds the cell list), and 0 for a
+    step without real space."""
+    import torch
+
+    L, rcut, alpha = 14.0, 3.0, 1.0
+    s = corner_cluster(2500, 700, L, 0.9, seed=14)
+    n = len(s["q"])
+    solver = pkg.Solver(3, pkg.HostSystem.from_dict(s), pkg.parameters(16, 5, alpha, rcut), pkg.MIXED)
+    dev = {k: torch.tensor(np.asarray(s[k]), dtype=torch.float64, device="cuda") for k in "xyzq"}
+    f, fk, fr = (torch.zeros(3 * n, dtype=torch.float64, device="cuda") for _ in range(3))
+    ptr = [dev[k].data_ptr() for k in "xyzq"]
+    torch.cuda.synchronize()
+    solver.stage_ring_start(3)
+    for rc in (rcut, rcut, 0.0):
+        solver.calculate_forces_device(n, L, *ptr, rc, alpha, f.data_ptr(), fk.data_ptr(), fr.data_ptr(),
+                                       None)
+    stages = solver.stage_ring_read(3)
+    pairs = solver.stage_ring_read_pairs(3)
+    solver.stage_ring_start(0)
+    solver.close()
+    print(stages[:, 4], pairs)
+    assert len(pairs) == 3
+    assert all(0.0 < pairs[k] <= stages[k, 4] for k in (0, 1))
+    assert pairs[2] == 0.0
