@@ -474,6 +474,28 @@ k_adjoint_ad(int n, const double4 *__restrict__ sorted, const int *__restrict__ 
 // tile side: 8 mesh points (LDS region (8 + P)^3 per mesh)
 int tile_side(int M) { return M < 8 ? M : 8; }
 
+// Dynamic LDS beyond 64 KB per workgroup (a gfx950 workgroup may use the
+// CU's whole 160 KB) is allowed per kernel before its launch.
+constexpr size_t kMaxGroupLds = 160 * 1024;
+
+int allow_lds(const void *kernel, size_t lds) {
+    if (lds <= 64 * 1024)
+        return P3MK_OK;
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return e == hipSuccess ? P3MK_OK : p3mk_fail(P3MK_EDEVICE, "hipFuncSetAttribute", (int)e);
+}
+
+// launch of a kernel of kThreads threads per workgroup and `lds` bytes of
+// dynamic LDS
+template <typename... Params, typename... Args>
+int launch_lds(void (*kernel)(Params...), dim3 grid, size_t lds, hipStream_t st, Args... args) {
+    P3MK_TRY(allow_lds((const void *)kernel, lds));
+    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, st, args...);
+    return P3MK_OK;
+}
+
 }  // namespace
 
 struct p3mk_psort {
@@ -580,19 +602,19 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
             return p3mk_fail(P3MK_EINVAL, "charge assignment order must be 1..7", 0);
         }
         const size_t lds = sizeof(double) * (size_t)rg.size + stage;
-        if (lds > 64 * 1024)
-            return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+        if (lds > kMaxGroupLds)
+            return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
         switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_assign_rows<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
+            P3MK_TRY(launch_lds(k_assign_rows<P, T, R2C, true, MODE>, grid, lds,  \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
-                               Hi, m, s->perm, gv, rg.size);                                       \
+                               Hi, m, s->perm, gv, rg.size));                                       \
         else                                                                                       \
-            hipLaunchKernelGGL((k_assign_rows<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
+            P3MK_TRY(launch_lds(k_assign_rows<P, T, R2C, false, MODE>, grid, lds, \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
-                               Hi, m, s->perm, gv, rg.size);                                       \
+                               Hi, m, s->perm, gv, rg.size));                                       \
         break;
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -600,19 +622,19 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
         return p3mk_launch_check("assign");
     }
     const size_t lds = sizeof(double) * NM * (size_t)R * R * R;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, true, MODE>), grid, dim3(kThreads), lds,  \
+            P3MK_TRY(launch_lds(k_assign_halo<P, T, R2C, true, MODE>, grid, lds,  \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
-                               Hi, m, s->perm, gv, n);                                             \
+                               Hi, m, s->perm, gv, n));                                             \
         else                                                                                       \
-            hipLaunchKernelGGL((k_assign_halo<P, T, R2C, false, MODE>), grid, dim3(kThreads), lds, \
+            P3MK_TRY(launch_lds(k_assign_halo<P, T, R2C, false, MODE>, grid, lds, \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
-                               Hi, m, s->perm, gv, n);                                             \
+                               Hi, m, s->perm, gv, n));                                             \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -633,20 +655,20 @@ int launch_interp_ik(const p3mk_geom *g, int n, const p3mk_psort *s, void *const
     const int NM = il ? 2 : 1;
     const int R = s->T + g->cao - 1 + (il ? 1 : 0);
     const size_t lds = sizeof(T) * NM * (size_t)R * R * R;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
     const dim3 grid((unsigned)((long)(n + kChunk - 1) / kChunk + s->ntiles));
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
+            P3MK_TRY(launch_lds(k_interp_ik_tile<P, T, R2C, true>, grid, lds, st, \
                                s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T,    \
-                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
+                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n));                         \
         else                                                                                       \
-            hipLaunchKernelGGL((k_interp_ik_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
+            P3MK_TRY(launch_lds(k_interp_ik_tile<P, T, R2C, false>, grid, lds,    \
                                st, s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, \
-                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n);                         \
+                               g->M, Hi, F0, F1, F2, prefac, fk, (long)n));                         \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -668,20 +690,20 @@ int launch_interp_ad(const p3mk_geom *g, int n, const p3mk_psort *s, const void 
     const int NM = il ? 2 : 1;
     const int R = s->T + g->cao - 1 + (il ? 1 : 0);
     const size_t lds = sizeof(T) * NM * (size_t)R * R * R;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
     const dim3 grid((unsigned)((long)(n + kChunk - 1) / kChunk + s->ntiles));
     switch (g->cao) {
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, true>), grid, dim3(kThreads), lds, st, \
+            P3MK_TRY(launch_lds(k_interp_ad_tile<P, T, R2C, true>, grid, lds, st, \
                                s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T,    \
-                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
+                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n));        \
         else                                                                                       \
-            hipLaunchKernelGGL((k_interp_ad_tile<P, T, R2C, false>), grid, dim3(kThreads), lds,    \
+            P3MK_TRY(launch_lds(k_interp_ad_tile<P, T, R2C, false>, grid, lds,    \
                                st, s->sorted, s->perm, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, \
-                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n);        \
+                               g->M, Hi, 1.0 / g->L, ph, prefac, selfb, h, fk, f, (long)n));        \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -1116,8 +1138,8 @@ int bpart_assign(const BForm &f, const BSys *sys, long ntiles, long max_items, s
                  p3mk_queue *qu) {
     if (max_items <= 0)
         return P3MK_OK;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "assignment tile does not fit in LDS", 0);
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
     hipStream_t st = p3mk_stream(qu);
 #define GO(T, R, I) launch_bassign<T, R, I>(sys, ntiles, max_items, lds, start, cstart, imap, sorted, st)
     if (f.fp32) {
@@ -1136,8 +1158,8 @@ int bpart_interp(const BForm &f, const BSys *sys, long ntiles, long max_items, s
                  const int *perm, int ntot, double *fk, double *ftot, p3mk_queue *qu) {
     if (ntot <= 0 || max_items <= 0)
         return P3MK_OK;
-    if (lds > 64 * 1024)
-        return p3mk_fail(P3MK_EINVAL, "interpolation tile does not fit in LDS", 0);
+    if (lds > kMaxGroupLds)
+        return p3mk_fail(P3MK_EINVAL, "tile does not fit in LDS", 0);
     hipStream_t st = p3mk_stream(qu);
     const bool ad = f.ad != 0;
 #define GO(T, R, I) launch_binterp<T, R, I>(ad, sys, ntiles, max_items, lds, start, cstart, imap, sorted, perm, ntot, fk, ftot, st)

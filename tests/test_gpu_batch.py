@@ -190,7 +190,9 @@ def test_batch_without_real_space_and_alpha_change(pkg, orc):
 def test_batch_time_pairs(pkg):
     """The pair-kernel timer of a batch returns a positive time, leaves the
     forces as an ordinary evaluation computes them, and a graph captured
-    afterwards (no events recorded in it) replays to the same forces."""
+    afterwards (no events recorded in it) replays to the same forces (the
+    mixed form to the rounding of its summation order, which varies from
+    run to run)."""
     import torch
 
     systems, data = make_batch_systems(8, seed=93, nmin=500, nmax=2500)
@@ -202,7 +204,7 @@ def test_batch_time_pairs(pkg):
     ms = batch.time_pairs(*[dev[k].data_ptr() for k in "xyzq"], *[t.data_ptr() for t in out], reps=3)
     print("batched pair kernel: %.4f ms" % ms)
     assert 0.0 < ms < 1e3
-    assert rel_err(out[0].cpu().numpy().reshape(3, n), before) < 1e-12
+    assert rel_err(out[0].cpu().numpy().reshape(3, n), before) < 1e-6
     after = run_batch(torch, batch, dev, graph=True)[0]
-    assert rel_err(after, before) < 1e-12
+    assert rel_err(after, before) < 1e-6
     batch.close()
