@@ -133,9 +133,10 @@ inline void cell_geometry(long n, double L, double rcut, int &nc, int &reach) {
 
 struct CellArrays {
     int *key = nullptr;                               // fine key per particle, then its sorted position
-    int *fcount = nullptr, *fstart = nullptr, *fcursor = nullptr; // 8 ncell + 1
+    int *fcount = nullptr, *fstart = nullptr; // 8 ncell + 1
     int *start = nullptr, *items = nullptr, *istart = nullptr;    // ncell + 1
     int *perm = nullptr, *scan = nullptr, *counter = nullptr;
+    int *rank = nullptr; // per particle: its rank within its fine key (from the count's atomic)
     double4 *sorted = nullptr;
     // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, q); the sorted
     // arrays carry 64 padding entries past the last particle

@@ -127,13 +127,13 @@ __device__ __forceinline__ void store_pad(int pos, double4 *__restrict__ sorted,
 
 __global__ void k_cell_count(int n, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, double L, double inv_side, int nc,
-                             int *__restrict__ key, int *__restrict__ count) {
+                             int *__restrict__ key, int *__restrict__ rank, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const int k = fine_key(fold(x[i], L), fold(y[i], L), fold(z[i], L), inv_side, nc);
     key[i] = k;
-    atomicAdd(&count[k], 1);
+    rank[i] = atomicAdd(&count[k], 1); // the scatter then needs no atomics
 }
 
 // n + kPad threads; key[i] is replaced by the particle's sorted position (the
@@ -142,7 +142,7 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
                                int nc, int *__restrict__ key, const int *__restrict__ start,
-                               int *__restrict__ cursor, int *__restrict__ perm,
+                               const int *__restrict__ rank, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
                                int4 *__restrict__ sortedq, const double *__restrict__ g,
                                double4 *__restrict__ gsorted) {
@@ -156,7 +156,7 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
         return;
     }
     const int k = key[i];
-    const int pos = start[k] + atomicAdd(&cursor[k], 1);
+    const int pos = start[k] + rank[i];
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
@@ -1372,9 +1372,9 @@ __global__ void k_unsort_add(int n, const int *__restrict__ pos, const double *_
 namespace p3m {
 
 void cells_release(CellArrays &a) {
-    // fcursor and counter live in the fcount allocation
+    // counter lives in the fcount allocation
     void *ptrs[] = {a.key, a.fcount, a.fstart, a.start, a.items, a.istart, a.perm,
-                    a.scan, a.sorted, a.sorted32, a.sortedq, a.bbox, a.gsorted, a.fsorted};
+                    a.scan, a.sorted, a.sorted32, a.sortedq, a.bbox, a.gsorted, a.fsorted, a.rank};
     for (void *p : ptrs)
         p3mk_release(p);
     a = CellArrays{};
@@ -1386,20 +1386,18 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     a.max_items = ncell + n / kWave + 1;
     int rc = P3MK_OK;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.key, sizeof(int) * nn);
-    // fcount, fcursor and the work-item counters in one allocation, cleared
-    // by one fill: [fcount | fcursor | counters], each part a multiple of
-    // 16 bytes
+    // fcount and the work-item counters in one allocation, cleared by one
+    // fill: [fcount | counters], fcount a multiple of 16 bytes
     const size_t nf4 = (nf + 3) & ~(size_t)3;
-    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcount, sizeof(int) * (2 * nf4 + kCounters));
-    if (rc == P3MK_OK) {
-        a.fcursor = a.fcount + nf4;
-        a.counter = a.fcount + 2 * nf4;
-    }
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fcount, sizeof(int) * (nf4 + kCounters));
+    if (rc == P3MK_OK)
+        a.counter = a.fcount + nf4;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.fstart, sizeof(int) * nf);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.start, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.items, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.istart, sizeof(int) * nc1);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.perm, sizeof(int) * nn);
+    if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.rank, sizeof(int) * nn);
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.scan, sizeof(int) * p3mk_scan_scratch_ints(8 * ncell));
     const size_t np = nn + kPad;
     if (rc == P3MK_OK) rc = p3mk_alloc((void **)&a.sorted, sizeof(double4) * np);
@@ -1412,8 +1410,8 @@ int cells_alloc(CellArrays &a, long n, long ncell) {
     return rc;
 }
 
-// zero the octant counts and cursors and the work-item counters (one
-// allocation) before the keys are counted
+// zero the octant counts and the work-item counters (one allocation) before
+// the keys are counted
 static int cells_begin(CellArrays &a, long ncell, p3mk_queue *qu) {
     (void)ncell;
     return p3mk_fill_zero(a.fcount, sizeof(int) * (size_t)(a.counter + kCounters - a.fcount), qu);
@@ -1499,11 +1497,11 @@ static int build_cells(p3mk_cells *c, int n, double L, const double *x, const do
     const int nb = (n + 255) / 256, nbp = (n + kPad + 255) / 256;
     P3MK_TRY(p3m::cells_begin(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, s, n, x, y, z, L, c->inv_side, c->nc,
-                       a.key, a.fcount);
+                       a.key, a.rank, a.fcount);
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
     hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
-                       a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32, a.sortedq, g, a.gsorted);
+                       a.fstart, a.rank, a.perm, a.sorted, a.sorted32, a.sortedq, g, a.gsorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     return p3m::cells_end(a, c->ncell, qu);
 }
@@ -1588,7 +1586,7 @@ using p3m::BSys;
 __global__ void k_bcell_count(int ntot, const BSys *__restrict__ sys, const int *__restrict__ psys,
                               const double *__restrict__ x, const double *__restrict__ y,
                               const double *__restrict__ z, int *__restrict__ key,
-                              int *__restrict__ count) {
+                              int *__restrict__ rank, int *__restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot)
         return;
@@ -1596,7 +1594,7 @@ __global__ void k_bcell_count(int ntot, const BSys *__restrict__ sys, const int 
     const double L = S.L;
     const int k = 8 * S.cell0 + fine_key(fold(x[i], L), fold(y[i], L), fold(z[i], L), (double)S.nc / L, S.nc);
     key[i] = k;
-    atomicAdd(&count[k], 1);
+    rank[i] = atomicAdd(&count[k], 1);
 }
 
 // as k_cell_scatter: key[i] becomes the particle's sorted position
@@ -1604,7 +1602,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
                                 const double *__restrict__ x, const double *__restrict__ y,
                                 const double *__restrict__ z, const double *__restrict__ q,
                                 int *__restrict__ key, const int *__restrict__ start,
-                                int *__restrict__ cursor, int *__restrict__ perm,
+                                const int *__restrict__ rank, int *__restrict__ perm,
                                 double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
                                 int4 *__restrict__ sortedq) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1617,7 +1615,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     const BSys &S = sys[psys[i]];
     const double L = S.L;
     const int k = key[i];
-    const int pos = start[k] + atomicAdd(&cursor[k], 1);
+    const int pos = start[k] + rank[i];
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
@@ -1716,11 +1714,11 @@ int brs_forces(const BForm &f, const BSys *sys, const int *psys, const int *cell
     const int nb = (ntot + 255) / 256;
     P3MK_TRY(cells_begin(a, ncell, qu));
     hipLaunchKernelGGL(k_bcell_count, dim3(nb), dim3(256), 0, s, ntot, sys, psys, x, y, z, a.key,
-                       a.fcount);
+                       a.rank, a.fcount);
     P3MK_TRY(p3mk_launch_check("batched cell count"));
     P3MK_TRY(cells_scan(a, ncell, qu));
     hipLaunchKernelGGL(k_bcell_scatter, dim3((ntot + kPad + 255) / 256), dim3(256), 0, s, ntot, sys,
-                       psys, x, y, z, q, a.key, a.fstart, a.fcursor, a.perm, a.sorted, a.sorted32,
+                       psys, x, y, z, q, a.key, a.fstart, a.rank, a.perm, a.sorted, a.sorted32,
                        a.sortedq);
     P3MK_TRY(p3mk_launch_check("batched cell scatter"));
     P3MK_TRY(cells_end(a, ncell, qu));
