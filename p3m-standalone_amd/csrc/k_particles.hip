@@ -243,7 +243,7 @@ k_assign_halo(const double4 *__restrict__ sorted, const int *__restrict__ tstart
 // Row-parallel form of k_assign_halo (charge and charge-times-factor
 // deposits): FP64 LDS accumulation, one mesh of an interlaced pair at a time.
 template <int P, typename T, bool R2C, bool IL, int MODE>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kAssignThreads)
 k_assign_rows(const double4 *__restrict__ sorted, const int *__restrict__ tstart,
               const int *__restrict__ cstart, const int2 *__restrict__ imap, long ntiles, int nt, int Tt, int M, double Hi,
               T *__restrict__ mesh, const int *__restrict__ perm, const double *__restrict__ gv,
@@ -487,13 +487,19 @@ int allow_lds(const void *kernel, size_t lds) {
     return e == hipSuccess ? P3MK_OK : p3mk_fail(P3MK_EDEVICE, "hipFuncSetAttribute", (int)e);
 }
 
-// launch of a kernel of kThreads threads per workgroup and `lds` bytes of
+// launch of a kernel of `threads` threads per workgroup and `lds` bytes of
 // dynamic LDS
 template <typename... Params, typename... Args>
-int launch_lds(void (*kernel)(Params...), dim3 grid, size_t lds, hipStream_t st, Args... args) {
+int launch_lds_t(void (*kernel)(Params...), dim3 grid, int threads, size_t lds, hipStream_t st,
+                 Args... args) {
     P3MK_TRY(allow_lds((const void *)kernel, lds));
-    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, st, args...);
+    hipLaunchKernelGGL(kernel, grid, dim3(threads), lds, st, args...);
     return P3MK_OK;
+}
+
+template <typename... Params, typename... Args>
+int launch_lds(void (*kernel)(Params...), dim3 grid, size_t lds, hipStream_t st, Args... args) {
+    return launch_lds_t(kernel, grid, kThreads, lds, st, args...);
 }
 
 }  // namespace
@@ -594,7 +600,7 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
         switch (g->cao) {
 #define SB(P)                                                                                      \
     case P:                                                                                        \
-        stage = (size_t)(kThreads / 64) * row_stage_bytes<P, T>();                                \
+        stage = (size_t)(kAssignThreads / 64) * row_stage_bytes<P, T>();                          \
         break;
             SB(1) SB(2) SB(3) SB(4) SB(5) SB(6) SB(7)
 #undef SB
@@ -608,11 +614,11 @@ int launch_assign(const p3mk_geom *g, int n, p3mk_psort *s, void *mesh, const do
 #define CASE(P)                                                                                    \
     case P:                                                                                        \
         if (il)                                                                                    \
-            P3MK_TRY(launch_lds(k_assign_rows<P, T, R2C, true, MODE>, grid, lds,  \
+            P3MK_TRY(launch_lds_t(k_assign_rows<P, T, R2C, true, MODE>, grid, kAssignThreads, lds, \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, rg.size));                                       \
         else                                                                                       \
-            P3MK_TRY(launch_lds(k_assign_rows<P, T, R2C, false, MODE>, grid, lds, \
+            P3MK_TRY(launch_lds_t(k_assign_rows<P, T, R2C, false, MODE>, grid, kAssignThreads, lds, \
                                st, s->sorted, s->start, s->cstart, s->imap, s->ntiles, s->nt, s->T, g->M,   \
                                Hi, m, s->perm, gv, rg.size));                                       \
         break;

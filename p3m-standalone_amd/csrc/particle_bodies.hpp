@@ -16,6 +16,11 @@ constexpr int kThreads = 256;
 // tiles of a clustered system over the GPU (Gaussian 1M, assignment +
 // interpolation mixed: 1024 0.76 ms, 512 0.67, 256 0.67, 128 0.74)
 constexpr int kChunk = 512;
+// threads of a row-deposit workgroup: its waves share one FP64 region in LDS,
+// so larger workgroups keep more waves per CU within the LDS (assignment,
+// mixed: Gaussian 1M 0.496 ms at 256, 0.487 at 512, 0.615 at 1024;
+// uniform 100k 0.078, 0.068, 0.068)
+constexpr int kAssignThreads = 512;
 
 // work item -> tile: the last tile whose chunk prefix sum is <= item
 __device__ __forceinline__ long item_tile(const int *__restrict__ cstart, long ntiles, int item) {
