@@ -527,6 +527,18 @@ template <typename T> __host__ __device__ inline int pow2_tile_lines(int n, int 
     return lt;
 }
 
+// lines per tile of the batched passes, whose meshes (16-48 points per line)
+// give short lines: the largest power of two that tile_lines allows and
+// that keeps a workgroup at most kBatchTilePoints points
+constexpr int kBatchTilePoints = 2048;
+template <typename T> __host__ __device__ inline int batch_tile_lines(int n, int nbuf) {
+    const int fit = tile_lines<T>(n, nbuf);
+    int lt = 1;
+    while (2 * lt <= fit && 2 * lt * n <= kBatchTilePoints)
+        lt *= 2;
+    return lt;
+}
+
 // last index s in [0, count) with prefix[s] <= v (prefix[0] == 0)
 template <typename I>
 __host__ __device__ inline int prefix_slot(const I *prefix, int count, long v) {

@@ -422,7 +422,7 @@ k_bfft_pass(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsys
         m = {M, 1, 1, M, (long)M * M};
     else
         m = {M, W, W, (long)M * W, (long)M * W};
-    const int lt = pow2_tile_lines<T>(M, 2);
+    const int lt = batch_tile_lines<T>(M, 2);
     fft_pass_tile<T>((V *)S.buf[slot], m, S.rp, (const V *)S.tw, inv, lt, axis == kPassZ ? 1 : 0,
                      (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -435,7 +435,7 @@ k_bfft_r2c_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsy
     using V = c2_t<T>;
     const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
     const BSys &S = sys[s];
-    const int M = S.M, lt = pow2_tile_lines<T>(M, 2);
+    const int M = S.M, lt = batch_tile_lines<T>(M, 2);
     fft_r2c_z_tile<T>((const T *)S.buf[in_slot], (V *)S.buf[out_slot], (long)M * M, M, S.rp,
                       (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -448,7 +448,7 @@ k_bfft_c2r_z(const BSys *__restrict__ sys, const long *__restrict__ blk, int nsy
     using V = c2_t<T>;
     const int s = prefix_slot(blk, nsys, (long)blockIdx.x);
     const BSys &S = sys[s];
-    const int M = S.M, lt = pow2_tile_lines<T>(M, 2);
+    const int M = S.M, lt = batch_tile_lines<T>(M, 2);
     fft_c2r_z_tile<T>((const V *)S.buf[in_slot], (T *)S.buf[out_slot], (long)M * M, M, S.rp,
                       (const V *)S.tw, lt, (long)(blockIdx.x - blk[s]) * lt, smem);
 }
@@ -464,7 +464,7 @@ k_bconvolve_x(const BSys *__restrict__ sys, const long *__restrict__ blk, int ns
     const int M = S.M, W = r2c ? M / 2 + 1 : M;
     const long nl = (long)M * W;
     const LineMap m = {M, nl, nl, 0, nl};
-    const int lt = pow2_tile_lines<T>(M, nout == 3 ? 3 : 2);
+    const int lt = batch_tile_lines<T>(M, nout == 3 ? 3 : 2);
     V *o0 = (V *)S.buf[out_slot];
     V *o1 = nout == 3 ? (V *)S.buf[out_slot + 1] : nullptr;
     V *o2 = nout == 3 ? (V *)S.buf[out_slot + 2] : nullptr;
@@ -479,7 +479,7 @@ namespace p3m {
 
 int bfft_lines_per_tile(const BForm &f, int M, int kind) {
     const int nbuf = (kind == kPassX && !f.ad) ? 3 : 2;
-    return f.fp32 ? pow2_tile_lines<float>(M, nbuf) : pow2_tile_lines<double>(M, nbuf);
+    return f.fp32 ? batch_tile_lines<float>(M, nbuf) : batch_tile_lines<double>(M, nbuf);
 }
 
 size_t bfft_lds_bytes(const BForm &f, int M, int kind) {
