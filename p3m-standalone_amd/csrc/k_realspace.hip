@@ -322,15 +322,16 @@ __device__ __forceinline__ void pair_force_f32(float fx, float fy, float fz, flo
                                                PairAcc &acc) {
     const float rinv = __builtin_amdgcn_rsqf(r2f);
     const float x = (float)alpha * (r2f * rinv);
-    const float e = __expf(-x * x);
-    const float ec = e * erfcx_f(x);
-    const float qqm = inside ? qq : 0.0f;
-    const float ecr = ec * rinv;
-    const float fak = qqm * (ecr + (float)two_a_spi * e) * (rinv * rinv);
+    // e^{-alpha^2 r^2} as 2^{-alpha^2 log2(e) r^2}, the charge product folded
+    // into it: qe = q_i q_j e^{-x^2}, enf = q_i q_j erfc(x) / r
+    const float qe = (inside ? qq : 0.0f) *
+                     __builtin_amdgcn_exp2f((float)(-alpha * alpha * 1.4426950408889634) * r2f);
+    const float enf = qe * erfcx_f(x) * rinv;
+    const float fak = fmaf((float)two_a_spi, qe, enf) * (rinv * rinv);
     acc.f0 = (double)(fak * fx);
     acc.f1 = (double)(fak * fy);
     acc.f2 = (double)(fak * fz);
-    acc.enf = qqm * ecr; // twice the energy
+    acc.enf = enf; // twice the energy
 }
 
 template <bool FP32>
@@ -837,9 +838,16 @@ constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a que
 // 127 pending (paired rounds) + the eight appends of one search step
 constexpr int kHalfQueue = 10 * kWave;
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
+// queue entry: owner o as o * 8 in bits 3..8, window slot in bits 19..24, so
+// that the entry masked or shifted is the byte offset of an 8- or 16-byte
+// element of either (e >> 16 = slot * 8, e >> 15 = slot * 16)
+constexpr int kSlotShift = 19;
+__device__ __forceinline__ unsigned int queue_entry(unsigned int o, unsigned int slot) {
+    return o << 3 | slot << kSlotShift;
+}
 
 template <bool FP32> struct RsHalfShared {
-    unsigned int qs[kHalfQueue]; // owner | window slot << 6
+    unsigned int qs[kHalfQueue]; // queue_entry(owner, window slot)
     double4 shift[27];
     double facc[3][kWave];       // the home particles' forces, each in kWave / owner-slots replicas
     double facc_nb[3][kWave];    // the window particles' share, f summed (the reaction is -f)
@@ -928,31 +936,43 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // near the cutoff.  The inclusion test is decided on r2f outside a band
     // of four times that width around rcut^2; inside it the reference's FP64
     // test runs on the FP64 positions.  The relative error of d is at most
-    // 2 u + sqrt(3) s / r: 3.2e-7 for a pair side / 100 apart.
+    // 2 u + sqrt(3) s / r: 3.2e-7 for a pair side / 100 apart.  The pair
+    // arithmetic runs in units of s: D = d / s, the window charge staged as
+    // q / s^2, alpha and 2 alpha / sqrt(pi) taken times s, so that
+    // (q_i q_j / s^2)(erfc(alpha s R) / R + 2 alpha s e^{-alpha^2 s^2 R^2} / sqrt(pi)) D / R^2
+    // is the force, and the energy sum, in units of 1 / s, is scaled once.
     const double quantum = ldexp(side, -kFixBits);
-    const float quantum_f = (float)quantum;
     const double band = 4.0 * (3.4642 * quantum / rcut + 8.0 * 0x1p-24);
-    const float r2_in = (float)(rcut * rcut * (1.0 - band)), r2_out = (float)(rcut * rcut * (1.0 + band));
+    const double rq2 = (rcut / quantum) * (rcut / quantum);
+    const float r2_in = (float)(rq2 * (1.0 - band)), r2_out = (float)(rq2 * (1.0 + band));
+    const double alpha_q = alpha * quantum, two_a_spi_q = two_a_spi * quantum;
     // one queued pair: what a round reads before its arithmetic (FP32: the
     // window particle from LDS, the home particle from sortedq through L1;
     // FP64: both positions from global memory, through L1)
     struct Fetched {
-        int o, slot;
+        unsigned int o8, s8; // owner * 8, window slot * 8
         unsigned int js;
         double4 pj, ph;
         int4 nq, hq;
     };
+    // the item's home particles (wave-uniform bases, byte offsets per lane)
+    const int4 *hq_base = FP32 ? sortedq + i0 : sortedq;
+    const double4 *ph_base = sorted + i0;
+    auto win_js_at = [&](unsigned int s8) {
+        return *reinterpret_cast<const unsigned int *>(reinterpret_cast<const char *>(sh.win_js) + (s8 >> 1));
+    };
     auto fetch = [&](int base, Fetched &f) {
         const unsigned int e = sh.qs[base + lane];
-        f.o = (int)(e & 63u);
-        f.slot = (int)(e >> 6);
+        f.o8 = e & 0x1F8u;
+        f.s8 = e >> (kSlotShift - 3);
         if constexpr (FP32) {
-            f.nq = sh.win_q[f.slot];
-            f.hq = sortedq[i0 + f.o];
+            f.nq = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(sh.win_q) +
+                                                   (e >> (kSlotShift - 4)));
+            f.hq = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(hq_base) + 2u * f.o8);
         } else {
-            f.js = sh.win_js[f.slot];
+            f.js = win_js_at(f.s8);
             f.pj = sorted[f.js & kIdxMask];
-            f.ph = sorted[i0 + f.o];
+            f.ph = *reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(ph_base) + 4u * f.o8);
         }
     };
     // the image shift of window entry js (0 unless the pair wraps)
@@ -969,16 +989,17 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
         }
     };
     auto evaluate = [&](const Fetched &f) {
-        const int o = f.o, slot = f.slot;
         PairAcc pa = {0.0, 0.0, 0.0, 0.0, 0.0f};
         if constexpr (FP32) {
-            const float dx = quantum_f * (float)(f.hq.x - f.nq.x), dy = quantum_f * (float)(f.hq.y - f.nq.y),
-                        dz = quantum_f * (float)(f.hq.z - f.nq.z);
+            const float dx = (float)(f.hq.x - f.nq.x), dy = (float)(f.hq.y - f.nq.y),
+                        dz = (float)(f.hq.z - f.nq.z);
             const float r2f = dx * dx + dy * dy + dz * dz;
             bool inside = r2f <= r2_in;
             if (__builtin_expect(!inside && r2f <= r2_out, 0)) {
-                const unsigned int js = sh.win_js[slot];
-                const double4 pj = sorted[js & kIdxMask], ph = sorted[i0 + o];
+                const unsigned int js = win_js_at(f.s8);
+                const double4 pj = sorted[js & kIdxMask],
+                              ph = *reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(ph_base) +
+                                                                      4u * f.o8);
                 double sx, sy, sz;
                 image_shift(js, sx, sy, sz);
                 double ex, ey, ez;
@@ -990,19 +1011,23 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 }
                 inside = within_cutoff(ex, ey, ez, rcut);
             }
-            pair_force_f32(dx, dy, dz, r2f, inside, __int_as_float(f.hq.w) * __int_as_float(f.nq.w), alpha,
-                           two_a_spi, pa);
+            pair_force_f32(dx, dy, dz, r2f, inside, __int_as_float(f.hq.w) * __int_as_float(f.nq.w), alpha_q,
+                           two_a_spi_q, pa);
         } else {
             double sx, sy, sz;
             image_shift(f.js, sx, sy, sz);
             pair<false>(f.ph, f.pj, sx, sy, sz, rcut, alpha, two_a_spi, sh.tab, pa);
         }
-        atomicAdd(&sh.facc[0][o + oofs], pa.f0);
-        atomicAdd(&sh.facc[1][o + oofs], pa.f1);
-        atomicAdd(&sh.facc[2][o + oofs], pa.f2);
-        atomicAdd(&sh.facc_nb[0][slot], pa.f0);
-        atomicAdd(&sh.facc_nb[1][slot], pa.f1);
-        atomicAdd(&sh.facc_nb[2][slot], pa.f2);
+        auto at = [](double *row, unsigned int ofs) {
+            return reinterpret_cast<double *>(reinterpret_cast<char *>(row) + ofs);
+        };
+        const unsigned int fo = f.o8 + 8u * (unsigned)oofs;
+        atomicAdd(at(sh.facc[0], fo), pa.f0);
+        atomicAdd(at(sh.facc[1], fo), pa.f1);
+        atomicAdd(at(sh.facc[2], fo), pa.f2);
+        atomicAdd(at(sh.facc_nb[0], f.s8), pa.f0);
+        atomicAdd(at(sh.facc_nb[1], f.s8), pa.f1);
+        atomicAdd(at(sh.facc_nb[2], f.s8), pa.f2);
         if (FP32)
             en32 += pa.enf;
         else
@@ -1091,14 +1116,14 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     const float u0 = c1 ? d[1] : d[0], u1 = c1 ? d[2] : d[1], u2 = c1 ? d[3] : d[2],
                                 u3 = c1 ? d[0] : d[3];
                     const float v[4] = {c2 ? u2 : u0, c2 ? u3 : u1, c2 ? u0 : u2, c2 ? u1 : u3};
-                    const unsigned int e0 = (unsigned)hidx | (unsigned)base << 6;
+                    const unsigned int e0 = queue_entry((unsigned)hidx, (unsigned)base);
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         const unsigned int rq = (unsigned)((col + q) & 3);
                         bool ok = v[q] <= 0.0f;
                         if (SELFW)
                             ok = ok && keep(sh.win_js[base + rq], i_own);
-                        append(ok, e0 + (rq << 6));
+                        append(ok, e0 + (rq << kSlotShift));
                     }
                 };
                 for (int s = 0; s < G; s += 2) {
@@ -1136,7 +1161,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 bool ok = filled && dx * dx + dy * dy + dz * dz <= rho2m;
                 if (selfwin)
                     ok = ok && keep(js, (unsigned)(i0 + o));
-                append(ok, (unsigned)o | (unsigned)lane << 6);
+                append(ok, queue_entry((unsigned)o, (unsigned)lane));
                 o = (o + 1 == nh) ? 0 : o + 1;
             }
         }
@@ -1232,6 +1257,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 const double dox = wx * L - (hx + 0.5) * side, doy = wy * L - (hy + 0.5) * side,
                              doz = wz * L - (hz + 0.5) * side;
                 const double inv_quantum = ldexp(1.0, kFixBits) / side;
+                const float q_scale = (float)(inv_quantum * inv_quantum);
                 // window rows: A' = (-2 (n + o), 1), o the cell centre relative to the home's
                 const float ox2 = (float)(2.0 * a * side), oy2 = (float)(2.0 * b * side),
                             oz2 = (float)(2.0 * c * side);
@@ -1257,7 +1283,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                         if constexpr (FP32) {
                             const double4 p = sorted[cb + lane];
                             nq = make_int4(to_fixed(p.x + dox, inv_quantum), to_fixed(p.y + doy, inv_quantum),
-                                           to_fixed(p.z + doz, inv_quantum), __float_as_int(v.w));
+                                           to_fixed(p.z + doz, inv_quantum), __float_as_int(v.w * q_scale));
                         }
                         const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
                         const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
@@ -1297,7 +1323,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     if (w > 0)
         process_window();
     if (FP32)
-        en_tot += (double)en32;
+        en_tot += quantum * (double)en32;
     if (active) {
         double f0 = 0.0, f1 = 0.0, f2 = 0.0;
         for (int r = 0; r < orep; r++) {
