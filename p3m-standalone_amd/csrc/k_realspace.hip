@@ -930,8 +930,9 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // FP32 form: the pair difference d = h - n' is the exact integer
     // difference of the home particle h and the window particle's image n',
     // both in fixed point relative to the home cell centre (quantum
-    // s = side 2^-kFixBits, each rounded once from FP64), scaled by s in
-    // FP32.  Per component |d~ - d| <= s + 2 u |d|, u = 2^-24, and r2f, the
+    // s = side 2^-kFixBits; each rounded once from FP64 relative to its own
+    // cell centre, whose offset from the home one is a whole number of
+    // sides), scaled by s in FP32.  Per component |d~ - d| <= s + 2 u |d|, u = 2^-24, and r2f, the
     // FP32 sum of squares, is within (2 sqrt(3) s / rcut + 8 u) rcut^2 of r^2
     // near the cutoff.  The inclusion test is decided on r2f outside a band
     // of four times that width around rcut^2; inside it the reference's FP64
@@ -1253,9 +1254,12 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                           wz = (int)((info >> 13) & 3u) - 1;
                 const unsigned int sidx = shift_index(wx, wy, wz) << 27;
                 const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
-                // FP32 form: the image relative to the home cell centre
-                const double dox = wx * L - (hx + 0.5) * side, doy = wy * L - (hy + 0.5) * side,
-                             doz = wz * L - (hz + 0.5) * side;
+                // FP32 form: the neighbour cell's centre relative to the home
+                // cell's is (a, b, c) sides, exactly 2^kFixBits quanta each
+                // (the image shift w L is the whole-mesh part of it), so a
+                // window particle's fixed-point position in the home frame is
+                // its own-cell one plus that integer offset
+                const int fox = a * (1 << kFixBits), foy = b * (1 << kFixBits), foz = c * (1 << kFixBits);
                 const double inv_quantum = ldexp(1.0, kFixBits) / side;
                 const float q_scale = (float)(inv_quantum * inv_quantum);
                 // window rows: A' = (-2 (n + o), 1), o the cell centre relative to the home's
@@ -1281,9 +1285,9 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                     if (lane < cnt) {
                         const float4 v = sorted32[cb + lane];
                         if constexpr (FP32) {
-                            const double4 p = sorted[cb + lane];
-                            nq = make_int4(to_fixed(p.x + dox, inv_quantum), to_fixed(p.y + doy, inv_quantum),
-                                           to_fixed(p.z + doz, inv_quantum), __float_as_int(v.w * q_scale));
+                            const int4 q0 = sortedq[cb + lane];
+                            nq = make_int4(q0.x + fox, q0.y + foy, q0.z + foz,
+                                           __float_as_int(__int_as_float(q0.w) * q_scale));
                         }
                         const float ax = v.x - ox2, ay = v.y - oy2, az = v.z - oz2;
                         const float nx = -0.5f * ax, ny = -0.5f * ay, nz = -0.5f * az;
