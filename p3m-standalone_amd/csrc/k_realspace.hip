@@ -835,6 +835,27 @@ k_realspace_vjp(int n, const double4 *__restrict__ sorted, const float4 *__restr
 // several times slower in the reaction sums.
 
 constexpr unsigned int kIdxMask = (1u << 27) - 1u; // sorted index bits of a queue / window entry
+
+// The half shell's neighbour cells, per lane, for reach 1 and 2: offsets
+// (a, b, c) with a > 0, or a = 0 and b > 0, or a = b = 0 and c >= 0 (the own
+// cell first), packed (a + 2) | (b + 2) << 3 | (c + 2) << 6; 0 past the
+// last one (14 for reach 1, 63 for reach 2).  A table rather than lane
+// arithmetic, so that nothing of it stays live across a kernel's items.
+struct HalfShellTable {
+    unsigned short v[2][kWave];
+};
+constexpr HalfShellTable make_half_shell() {
+    HalfShellTable t{};
+    for (int reach = 1; reach <= 2; reach++) {
+        int lane = 0;
+        for (int a = 0; a <= reach; a++)
+            for (int b = (a == 0 ? 0 : -reach); b <= reach; b++)
+                for (int c = (a == 0 && b == 0 ? 0 : -reach); c <= reach; c++)
+                    t.v[reach - 1][lane++] = (unsigned short)((a + 2) | (b + 2) << 3 | (c + 2) << 6);
+    }
+    return t;
+}
+__constant__ HalfShellTable kHalfShell = make_half_shell();
 // 127 pending (paired rounds) + the eight appends of one search step
 constexpr int kHalfQueue = 10 * kWave;
 constexpr unsigned int kNoShift = 13;              // shift_index(0, 0, 0)
@@ -848,7 +869,7 @@ __device__ __forceinline__ unsigned int queue_entry(unsigned int o, unsigned int
 
 template <bool FP32> struct RsHalfShared {
     unsigned int qs[kHalfQueue]; // queue_entry(owner, window slot)
-    double4 shift[27];
+    double4 shift[FP32 ? 1 : 27]; // image shifts by shift index (the FP32 form computes them)
     double facc[3][kWave];       // the home particles' forces, each in kWave / owner-slots replicas
     double facc_nb[3][kWave];    // the window particles' share, f summed (the reaction is -f)
     halfx4 win_a[kWave];         // (-2 n, 1) in FP16, n = n' / side, n' relative to the home cell centre
@@ -888,7 +909,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     sh.win_a[lane] = halfx4{(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)1.0f};
     sh.win_n2[lane] = 1e30f;
     sh.win_js[lane] = 0u;
-    if (lane < 27)
+    if (!FP32 && lane < 27)
         sh.shift[lane] = make_double4((lane / 9 - 1) * L, ((lane / 3) % 3 - 1) * L, (lane % 3 - 1) * L, 0.0);
     const bool tiles = he - hb > kMfmaMinCell;
     // home groups of four rounded up to a power of two (2 ... 16)
@@ -983,10 +1004,16 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
         sz = 0.0;
         const unsigned int sidx = js >> 27;
         if (sidx != kNoShift) {
-            const double4 s3 = sh.shift[sidx];
-            sx = s3.x;
-            sy = s3.y;
-            sz = s3.z;
+            if constexpr (FP32) {
+                sx = (double)((int)(sidx / 9u) - 1) * L;
+                sy = (double)((int)((sidx / 3u) % 3u) - 1) * L;
+                sz = (double)((int)(sidx % 3u) - 1) * L;
+            } else {
+                const double4 s3 = sh.shift[sidx];
+                sx = s3.x;
+                sy = s3.y;
+                sz = s3.z;
+            }
         }
     };
     auto evaluate = [&](const Fetched &f) {
@@ -1199,24 +1226,12 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // reach 1; the own cell first): offsets, image flags, particle range and
     // first item read by one vector load each, cells that are empty or beyond
     // rcut from the item's box dropped; the loop then walks the live ones
-    const int W = 2 * reach + 1;
-    const int ncand = (reach + 1) + reach * W + reach * W * W;
     bool live = false;
     int cjb = 0, cje = 0, citem = 0;
     unsigned int cinfo = 0;
-    if (lane < ncand) {
-        int a, b, c;
-        if (lane <= reach) {
-            a = 0; b = 0; c = lane;
-        } else if (lane < reach + 1 + reach * W) {
-            const int t = lane - (reach + 1);
-            a = 0; b = 1 + t / W; c = t % W - reach;
-        } else {
-            const int t = lane - (reach + 1) - reach * W;
-            a = 1 + t / (W * W);
-            b = (t % (W * W)) / W - reach;
-            c = t % W - reach;
-        }
+    const unsigned int abc = kHalfShell.v[reach - 1][lane];
+    if (abc != 0u) {
+        const int a = (int)(abc & 7u) - 2, b = (int)((abc >> 3) & 7u) - 2, c = (int)((abc >> 6) & 7u) - 2;
         int cx = hx + a, cy = hy + b, cz = hz + c, wx = 0, wy = 0, wz = 0;
         if (cx >= nc) { cx -= nc; wx = 1; }
         if (cy < 0) { cy += nc; wy = -1; } else if (cy >= nc) { cy -= nc; wy = 1; }
