@@ -186,11 +186,12 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
             __syncthreads();
             // a lane's row sums over consecutive particles of one base (the
             // tile order puts them together) stay in registers until the base
-            // changes
-            A run[P];
+            // changes, in the form's precision (a run is a few particles; the
+            // FP32 form's sums are FMAs, widened to FP64 at the LDS add)
+            T run[P];
 #pragma unroll
             for (int a2 = 0; a2 < P; a2++)
-                run[a2] = A(0);
+                run[a2] = T(0);
             for (int j = 0; j < nb; j += G) {
                 const int jj = j + grp;
                 if (row_lane && jj < nb) {
@@ -200,14 +201,14 @@ assign_rows_item(const double4 *__restrict__ sorted, int pb, int pe, long tl, in
                     const T tr = rec[W2 + ra0] * rec[W2 + P + ra1];
 #pragma unroll
                     for (int a2 = 0; a2 < P; a2++)
-                        run[a2] += (A)(tr * wz[a2]);
+                        run[a2] += tr * wz[a2];
                     const int off = woff[jj];
                     if (jj + G >= nb || woff[jj + G] != off) {
                         A *a = acc + off + ra0 * rg.Sx + ra1 * rg.Sy;
 #pragma unroll
                         for (int a2 = 0; a2 < P; a2++) {
-                            atomicAdd(&a[a2], run[a2]);
-                            run[a2] = A(0);
+                            atomicAdd(&a[a2], (A)run[a2]);
+                            run[a2] = T(0);
                         }
                     }
                 }
