@@ -4,6 +4,8 @@ the reaction added to the partner) against the oracle's Realteil
 sparse ones, clusters that wrap across the periodic boundary, both cell
 sizes (side rcut / 2, and side rcut for sparse systems), the device entry
 point accumulating into the caller's arrays, and the mixed form."""
+import math
+
 import numpy as np
 import pytest
 
@@ -161,6 +163,54 @@ def test_pairs_at_the_cutoff(pkg, orc, m, reach):
     fr_ref, e_ref = orc.realteil(s, rcut, alpha)
     assert rel_err(forces.real(), fr_ref) < 1e-13
     assert abs(system.c.energy - e_ref) <= 1e-12 * abs(e_ref)
+
+
+@pytest.mark.parametrize("m,reach", [(400, 1), (1100, 2)])
+def test_pairs_at_the_cutoff_mixed(pkg, orc, m, reach):
+    """The mixed form on the pairs of test_pairs_at_the_cutoff: their FP32
+    squared distances fall inside the band around rcut^2, where the FP64 test
+    decides on the FP64 positions with the pair's image shift (many of the
+    pairs wrap across the box).  One pair decided the other way would move
+    its two particles' forces by the pair force at rcut, f_c = 3.4e-4: each
+    particle's error is held to 1e-5 of its force plus f_c / 10 (FP32
+    rounding stays near 1e-7 of the force), besides the mixed form's 1e-6
+    on the norm."""
+    import torch
+
+    L, rcut, alpha = 12.0, 3.0, 0.8
+    assert cell_geometry(2 * m, L, rcut)[1] == reach
+    rng = np.random.default_rng(78)
+    a = rng.uniform(0.0, L, (m, 3))
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    eps = np.where(np.arange(m) % 2 == 0, -1e-9, 1e-9)
+    b = a + (rcut * (1.0 + eps))[:, None] * u
+    p = np.mod(np.concatenate([a, b]), L)
+    p[p >= L] = 0.0
+    q = np.concatenate([np.ones(m), -np.ones(m)])
+    s = {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": L}
+    n = 2 * m
+    fr_ref, e_ref = orc.realteil(s, rcut, alpha)
+    solver = pkg.Solver(3, pkg.HostSystem.from_dict(s), pkg.parameters(16, 5, alpha, rcut), pkg.MIXED)
+    dev = {k: torch.tensor(np.asarray(s[k]), dtype=torch.float64, device="cuda") for k in "xyzq"}
+    fr = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    en = torch.zeros(1, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    solver.realteil_device(n, L, *[dev[k].data_ptr() for k in "xyzq"], rcut, alpha, fr.data_ptr(),
+                           en.data_ptr())
+    solver.synchronize()
+    got = fr.cpu().numpy().reshape(3, n)
+    e = float(en.item())
+    solver.close()
+    err = rel_err(got, fr_ref)
+    fc = (math.erfc(alpha * rcut) / rcut + 2.0 * alpha / math.sqrt(math.pi) * math.exp(-(alpha * rcut) ** 2)) / rcut**2
+    dev_i = np.linalg.norm(got - fr_ref, axis=0)
+    bound_i = 1e-5 * np.linalg.norm(fr_ref, axis=0) + 0.1 * fc
+    print("mixed at the cutoff (m %d, reach %d): rel err %.2e, worst particle %.2e of its bound"
+          % (m, reach, err, (dev_i / bound_i).max()))
+    assert (dev_i <= bound_i).all()
+    assert err < 1e-6
+    assert abs(e - e_ref) <= 1e-5 * abs(e_ref)
 
 
 def test_stage_ring_pair_kernel_times(pkg):
