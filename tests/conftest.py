@@ -67,6 +67,35 @@ def fold(s):
     return out
 
 
+def cell_geometry(n, L, rcut):
+    """The library's real-space cell grid: (cells per side, reach).  Cells
+    of side >= rcut / 2 (reach 2) unless those would hold fewer than 6
+    particles on average, or the box has fewer than 5 of them per side; then
+    side >= rcut (reach 1)."""
+    nc2 = min(512, int(np.floor(L * 2 / rcut * (1 - 1e-12))))
+    nc1 = min(512, int(np.floor(L / rcut * (1 - 1e-12))))
+    if nc2 < 5 or (nc1 >= 3 and n < 6.0 * nc2 ** 3):
+        return max(nc1, 1), 1
+    return nc2, 2
+
+
+def mixed_realspace_bound(s, rcut):
+    """Bound on the mixed form's relative real-space force error: its pair
+    differences are exact differences of coordinates rounded to the quantum
+    side * 2^-29, so a pair r apart carries at most 2u + sqrt(3) s / r
+    relative error in d (u = 2^-24), about three times that in its force;
+    the closest pair bounds the whole.  Never below 1e-6."""
+    from scipy.spatial import cKDTree
+
+    L = s["L"]
+    p = np.stack([s["x"], s["y"], s["z"]], 1)
+    d, _ = cKDTree(p, boxsize=L * (1 + 1e-12)).query(p, k=2)
+    rmin = max(float(d[:, 1].min()), 1e-300)
+    nc, _ = cell_geometry(len(s["q"]), L, rcut)
+    quantum = L / nc * 2.0 ** -29
+    return max(1e-6, 3.0 * (2.0 * 2.0 ** -24 + 3.0 ** 0.5 * quantum / rmin))
+
+
 def rel_err(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))

@@ -9,7 +9,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import rel_err
+from conftest import cell_geometry, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -28,18 +28,6 @@ def corner_cluster(n_cluster, n_background, L, sigma, seed, centre=(0.0, 0.0, 0.
     if n % 2:
         q[-1] = 0.0
     return {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "q": q, "L": float(L)}
-
-
-def cell_geometry(n, L, rcut):
-    """The library's cell grid: (cells per side, reach).  Cells of side
-    >= rcut / 2 (reach 2) unless those would hold fewer than 6 particles on
-    average, or the box has fewer than 5 of them per side; then side >= rcut
-    (reach 1)."""
-    nc2 = min(512, int(np.floor(L * 2 / rcut * (1 - 1e-12))))
-    nc1 = min(512, int(np.floor(L / rcut * (1 - 1e-12))))
-    if nc2 < 5 or (nc1 >= 3 and n < 6.0 * nc2 ** 3):
-        return max(nc1, 1), 1
-    return nc2, 2
 
 
 def cell_counts(s, rcut):
