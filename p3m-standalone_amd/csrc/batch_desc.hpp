@@ -141,8 +141,8 @@ struct CellArrays {
     // relative to the cell centre in FP32: (-2 x, -2 y, -2 z, q); the sorted
     // arrays carry 64 padding entries past the last particle
     float4 *sorted32 = nullptr;
-    // relative to the cell centre in fixed point (kFixBits fractional bits
-    // per side): (x, y, z, q as FP32 bits)
+    // relative to the cell centre in fixed point (29 fractional bits per
+    // side for reach 2, 30 for reach 1): (x, y, z, q as FP32 bits)
     int4 *sortedq = nullptr;
     double4 *gsorted = nullptr; // adjoint cotangent in sorted order (made on first use)
     double *fsorted = nullptr;  // half-shell forces in sorted order, 3 n

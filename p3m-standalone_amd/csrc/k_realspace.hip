@@ -91,11 +91,15 @@ __device__ __forceinline__ int fine_key(double xf, double yf, double zf, double 
     return ((cx * nc + cy) * nc + cz) * 8 + oct;
 }
 
-// Fixed-point coordinates relative to a cell centre: kFixBits fractional
-// bits per cell side, so differences of two such coordinates in one frame
-// are exact integers.  A frame spans the (2 reach + 1)^3 cells around its
-// centre, |v| <= 2.5 sides < 2^31 / 2^kFixBits.
-constexpr int kFixBits = 29;
+// Fixed-point coordinates relative to a cell centre: fix_bits(reach)
+// fractional bits per cell side, so differences of two such coordinates in
+// one frame are exact integers.  A frame spans the (2 reach + 1)^3 cells
+// around its centre: reach 2, |v| <= 2.5 sides < 2^31 / 2^29; reach 1,
+// |v| <= 1.5 sides < 2^31 / 2^30.  A difference of two coordinates of one
+// frame stays below 2^31 in magnitude except at exactly 2 sides apart
+// (reach 1), where it wraps to -2^31 of the same magnitude; such a pair is
+// beyond rcut, which is at most one side.
+__host__ __device__ constexpr int fix_bits(int reach) { return reach == 1 ? 30 : 29; }
 
 __device__ __forceinline__ int to_fixed(double v, double inv_quantum) {
     return (int)rint(v * inv_quantum);
@@ -105,7 +109,7 @@ __device__ __forceinline__ int to_fixed(double v, double inv_quantum) {
 // of its cell (cx, cy, cz) in FP32 the row (-2 x, -2 y, -2 z, q) the
 // candidate search builds its matrix rows from; and in fixed point
 // (x, y, z, q as FP32 bits) for the FP32 pair difference.
-__device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int nc, double side,
+__device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int nc, double side, int fixbits,
                                              double4 *__restrict__ sorted,
                                              float4 *__restrict__ sorted32,
                                              int4 *__restrict__ sortedq) {
@@ -113,7 +117,7 @@ __device__ __forceinline__ void store_sorted(int pos, double4 r, int cell, int n
     const double rx = r.x - (cx + 0.5) * side, ry = r.y - (cy + 0.5) * side, rz = r.z - (cz + 0.5) * side;
     sorted[pos] = r;
     sorted32[pos] = make_float4(-2.0f * (float)rx, -2.0f * (float)ry, -2.0f * (float)rz, (float)r.w);
-    const double iq = ldexp(1.0, kFixBits) / side;
+    const double iq = ldexp(1.0, fixbits) / side;
     sortedq[pos] = make_int4(to_fixed(rx, iq), to_fixed(ry, iq), to_fixed(rz, iq), __float_as_int((float)r.w));
 }
 
@@ -141,7 +145,7 @@ __global__ void k_cell_count(int n, const double *__restrict__ x, const double *
 // g_z, 0) of the particle at pos (the adjoint's cotangent in sorted order)
 __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ z, const double *__restrict__ q, double L,
-                               int nc, int *__restrict__ key, const int *__restrict__ start,
+                               int nc, int fixbits, int *__restrict__ key, const int *__restrict__ start,
                                const int *__restrict__ rank, int *__restrict__ perm,
                                double4 *__restrict__ sorted, float4 *__restrict__ sorted32,
                                int4 *__restrict__ sortedq, const double *__restrict__ g,
@@ -160,7 +164,7 @@ __global__ void k_cell_scatter(int n, const double *__restrict__ x, const double
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, k >> 3, nc, L / nc, sorted, sorted32, sortedq);
+    store_sorted(pos, r, k >> 3, nc, L / nc, fixbits, sorted, sorted32, sortedq);
     if (g != nullptr)
         gsorted[pos] = make_double4(g[i], g[n + i], g[2 * (long)n + i], 0.0);
 }
@@ -951,7 +955,7 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // FP32 form: the pair difference d = h - n' is the exact integer
     // difference of the home particle h and the window particle's image n',
     // both in fixed point relative to the home cell centre (quantum
-    // s = side 2^-kFixBits; each rounded once from FP64 relative to its own
+    // s = side 2^-fix_bits(reach); each rounded once from FP64 relative to its own
     // cell centre, whose offset from the home one is a whole number of
     // sides), scaled by s in FP32.  Per component |d~ - d| <= s + 2 u |d|, u = 2^-24, and r2f, the
     // FP32 sum of squares, is within (2 sqrt(3) s / rcut + 8 u) rcut^2 of r^2
@@ -963,7 +967,8 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
     // q / s^2, alpha and 2 alpha / sqrt(pi) taken times s, so that
     // (q_i q_j / s^2)(erfc(alpha s R) / R + 2 alpha s e^{-alpha^2 s^2 R^2} / sqrt(pi)) D / R^2
     // is the force, and the energy sum, in units of 1 / s, is scaled once.
-    const double quantum = ldexp(side, -kFixBits);
+    const int fb = fix_bits(reach);
+    const double quantum = ldexp(side, -fb);
     const double band = 4.0 * (3.4642 * quantum / rcut + 8.0 * 0x1p-24);
     const double rq2 = (rcut / quantum) * (rcut / quantum);
     const float r2_in = (float)(rq2 * (1.0 - band)), r2_out = (float)(rq2 * (1.0 + band));
@@ -1270,12 +1275,12 @@ realspace_item_half(int home, int chunk, int cell0, const double4 *__restrict__ 
                 const unsigned int sidx = shift_index(wx, wy, wz) << 27;
                 const float fsx = (float)(wx * L), fsy = (float)(wy * L), fsz = (float)(wz * L);
                 // FP32 form: the neighbour cell's centre relative to the home
-                // cell's is (a, b, c) sides, exactly 2^kFixBits quanta each
+                // cell's is (a, b, c) sides, exactly 2^fb quanta each
                 // (the image shift w L is the whole-mesh part of it), so a
                 // window particle's fixed-point position in the home frame is
                 // its own-cell one plus that integer offset
-                const int fox = a * (1 << kFixBits), foy = b * (1 << kFixBits), foz = c * (1 << kFixBits);
-                const double inv_quantum = ldexp(1.0, kFixBits) / side;
+                const int fox = a * (1 << fb), foy = b * (1 << fb), foz = c * (1 << fb);
+                const double inv_quantum = ldexp(1.0, fb) / side;
                 const float q_scale = (float)(inv_quantum * inv_quantum);
                 // window rows: A' = (-2 (n + o), 1), o the cell centre relative to the home's
                 const float ox2 = (float)(2.0 * a * side), oy2 = (float)(2.0 * b * side),
@@ -1545,8 +1550,9 @@ static int build_cells(p3mk_cells *c, int n, double L, const double *x, const do
                        a.key, a.rank, a.fcount);
     P3MK_TRY(p3mk_launch_check("cell count"));
     P3MK_TRY(p3m::cells_scan(a, c->ncell, qu));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc, a.key,
-                       a.fstart, a.rank, a.perm, a.sorted, a.sorted32, a.sortedq, g, a.gsorted);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, s, n, x, y, z, q, L, c->nc,
+                       fix_bits(c->reach), a.key, a.fstart, a.rank, a.perm, a.sorted, a.sorted32,
+                       a.sortedq, g, a.gsorted);
     P3MK_TRY(p3mk_launch_check("cell scatter"));
     return p3m::cells_end(a, c->ncell, qu);
 }
@@ -1664,7 +1670,7 @@ __global__ void k_bcell_scatter(int ntot, const BSys *__restrict__ sys, const in
     perm[pos] = i;
     key[i] = pos;
     const double4 r = make_double4(fold(x[i], L), fold(y[i], L), fold(z[i], L), q[i]);
-    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, sorted, sorted32, sortedq);
+    store_sorted(pos, r, (k >> 3) - S.cell0, S.nc, L / S.nc, fix_bits(S.reach), sorted, sorted32, sortedq);
 }
 
 template <bool FP32>

@@ -82,17 +82,18 @@ def cell_geometry(n, L, rcut):
 def mixed_realspace_bound(s, rcut):
     """Bound on the mixed form's relative real-space force error: its pair
     differences are exact differences of coordinates rounded to the quantum
-    side * 2^-29, so a pair r apart carries at most 2u + sqrt(3) s / r
-    relative error in d (u = 2^-24), about three times that in its force;
-    the closest pair bounds the whole.  Never below 1e-6."""
+    s = side * 2^-29 (cells of side rcut / 2) or side * 2^-30 (side rcut),
+    so a pair r apart carries at most 2u + sqrt(3) s / r relative error in d
+    (u = 2^-24), about three times that in its force; the closest pair
+    bounds the whole.  Never below 1e-6."""
     from scipy.spatial import cKDTree
 
     L = s["L"]
     p = np.stack([s["x"], s["y"], s["z"]], 1)
     d, _ = cKDTree(p, boxsize=L * (1 + 1e-12)).query(p, k=2)
     rmin = max(float(d[:, 1].min()), 1e-300)
-    nc, _ = cell_geometry(len(s["q"]), L, rcut)
-    quantum = L / nc * 2.0 ** -29
+    nc, reach = cell_geometry(len(s["q"]), L, rcut)
+    quantum = L / nc * 2.0 ** -(30 if reach == 1 else 29)
     return max(1e-6, 3.0 * (2.0 * 2.0 ** -24 + 3.0 ** 0.5 * quantum / rmin))
 
 
